@@ -1,0 +1,192 @@
+"""Benchmark of the event-driven CSR scatter (bm.event.csrmv, transpose=True).
+
+Workload (one "step" = one operator call over one spike vector):
+  100,000 x 100,000 CSR, FixedProb p = 0.01 (1000 targets per row,
+  nnz = 1e8, int32 indices), Bernoulli spike vectors at 0.1 / 1 / 5 / 10 %,
+  weights homogeneous 1.0, heterogeneous normal(0.5, 0.1) f32, and the same
+  heterogeneous weights stored in bf16 (accumulated in f32).
+  Headline: 10 % spikes, heterogeneous f32 weights.  All inputs are
+  synthetic, seeded and resident in HBM before the timed region.
+
+Metric: synaptic events/s = (sum over active rows of their row length) per
+second of whole-operator wall time (zero-fill + scatter), plus the scatter
+kernel's algorithmic HBM bytes / its average device duration (HIP events on
+the launch stream) against the 8 TB/s MI355X peak.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-sweep] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+M = N = 100_000
+PROB = 0.01
+K_ROW = int(N * PROB)
+RATES = (0.001, 0.01, 0.05, 0.10)
+HEADLINE = ("heter_f32", 0.10)
+
+
+def algorithmic_bytes(active_rows, kind):
+    """Bytes the scatter must move once: events (1 B each), the indptr pair
+    of each active row, indices and weights of active rows, one write of
+    the output."""
+    w_bytes = {"homo": 0, "heter_f32": 4, "heter_bf16": 2}[kind]
+    return M * 1 + active_rows * 8 + active_rows * K_ROW * (4 + w_bytes) + N * 4
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-sweep", action="store_true", help="time the headline configuration only")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs on a single MI355X and "
+              "has no multi-GPU path.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from brainpy_amd import _lib
+    from brainpy_amd.connect import fixed_prob_csr_device
+
+    dev = _lib.device()
+    L = _lib.lib()
+    stream = torch.cuda.current_stream()
+    sp = _lib.stream_ptr()
+    print(f"# building {M}x{N} CSR, {K_ROW}/row, nnz={M * K_ROW:.3g}", file=sys.stderr, flush=True)
+    indices, indptr = fixed_prob_csr_device(M, N, PROB, seed=2024, device=dev)
+    nnz = indices.numel()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(7)
+    w_f32 = torch.empty(nnz, dtype=torch.float32, device=dev).normal_(0.5, 0.1, generator=gen)
+    weights = {
+        "homo": torch.ones(1, dtype=torch.float32, device=dev),
+        "heter_f32": w_f32,
+        "heter_bf16": w_f32.to(torch.bfloat16),
+    }
+    spikes = {}
+    for r in RATES:
+        g = torch.Generator(device=dev)
+        g.manual_seed(int(r * 1e6) + 11)
+        spikes[r] = torch.rand(M, generator=g, device=dev) < r
+    out = torch.empty(N, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    def call(kind, r):
+        w = weights[kind]
+        rc = L.bpx_event_csrmv(sp, _lib.ptr(w), _lib.bpx_dtype(w), w.numel(), _lib.ptr(indices), _lib.ptr(indptr),
+                               _lib.BPX_I32, nnz, _lib.ptr(spikes[r]), _lib.BPX_BOOL, _lib.RULE_POSITIVE, M, N, 1,
+                               _lib.ptr(out))
+        if rc != 0:
+            raise RuntimeError(f"bpx_event_csrmv failed ({rc})")
+
+    def measure(kind, r, steps, warmup):
+        active = int(spikes[r].sum().item())
+        events = active * K_ROW
+        for _ in range(warmup):
+            call(kind, r)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            call(kind, r)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        # device time of the scatter kernel alone, on the launch stream
+        _lib.timer_reset()
+        _lib.timer_enable(True)
+        for _ in range(steps):
+            call(kind, r)
+        torch.cuda.synchronize()
+        _lib.timer_enable(False)
+        ms, launches = _lib.timer_read(_lib.KID["event_scatter"])
+        kern_s = ms / 1e3 / max(launches, 1)
+        abytes = algorithmic_bytes(active, kind)
+        return {
+            "kind": kind, "rate": r, "active_rows": active, "events_per_call": events,
+            "ms_per_step": wall / steps * 1e3, "events_per_s": events * steps / wall,
+            "kernel_us": kern_s * 1e6, "kernel_launches": launches, "algorithmic_bytes": abytes,
+            "kernel_GBs": abytes / kern_s / 1e9,
+        }
+
+    sweep = []
+    if not args.no_sweep:
+        for kind in ("homo", "heter_f32", "heter_bf16"):
+            for r in RATES:
+                res = measure(kind, r, max(20, args.steps // 4), max(5, args.warmup // 4))
+                sweep.append(res)
+                print(f"# {kind:10s} rate={r:<6} {res['ms_per_step']*1e3:8.1f} us/call "
+                      f"kernel {res['kernel_us']:8.1f} us {res['kernel_GBs']:8.1f} GB/s "
+                      f"{res['events_per_s']:.3e} ev/s", file=sys.stderr, flush=True)
+    head = measure(HEADLINE[0], HEADLINE[1], args.steps, args.warmup)
+    print(f"# headline {head}", file=sys.stderr, flush=True)
+
+    # CPU baseline: the oracle's serial scatter (the reference CPU loop
+    # order, event/_csr_matvec.py:321-340), one thread, on a bounded number
+    # of calls of the headline workload
+    cpu = None
+    if not args.no_cpu:
+        from oracle import orc
+        h_idx = indices.cpu().numpy()
+        h_ptr = indptr.cpu().numpy()
+        h_w = weights[HEADLINE[0]].cpu().numpy()
+        h_ev = spikes[HEADLINE[1]].cpu().numpy().view(np.uint8)
+        h_out = np.empty(N, np.float32)
+        orc.event_csrmv_scatter_f32(h_w, h_idx, h_ptr, h_ev, (M, N), out=h_out)   # warm-up
+        calls = 0
+        t0 = time.perf_counter()
+        while True:
+            orc.event_csrmv_scatter_f32(h_w, h_idx, h_ptr, h_ev, (M, N), out=h_out)
+            calls += 1
+            el = time.perf_counter() - t0
+            if el >= args.cpu_seconds or calls >= 2000:
+                break
+        # the GPU result of the same call agrees with the CPU one
+        call(*HEADLINE)
+        torch.cuda.synchronize()
+        rel = float(np.max(np.abs(out.cpu().numpy() - h_out) / np.maximum(np.abs(h_out), 1.0)))
+        cpu = {"value": head["events_per_call"] * calls / el, "unit": "synaptic events/s", "cores": 1,
+               "kind": "port",
+               "sample": f"{calls} calls of the headline workload ({HEADLINE[1]*100:.0f}% spikes, heter f32) "
+                         f"in {el:.1f} s: serial scatter of oracle/bpx_oracle.c, gcc -O2, 1 thread",
+               "gpu_vs_cpu_max_rel_diff": rel}
+        print(f"# cpu {cpu}", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "synaptic events/s (event csrmv transpose, 100k x 100k, p=0.01)",
+        "value": head["events_per_s"],
+        "unit": "synaptic events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded FixedProb CSR and Bernoulli spikes, generated on the GPU)",
+        "config": {"workload": "event_csrmv_transpose_100kx100k_p0.01_rate0.10_heter_f32", "m": M, "n": N,
+                   "nnz": nnz, "spike_rate": HEADLINE[1], "weights": HEADLINE[0], "index_dtype": "int32",
+                   "active_rows": head["active_rows"], "events_per_step": head["events_per_call"]},
+        "roofline": {"bound": "hbm", "achieved": head["kernel_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": head["kernel_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_csr_scatter (event, heter f32)", "kernel_us": head["kernel_us"],
+                     "algorithmic_bytes_per_launch": head["algorithmic_bytes"]},
+        "cpu_baseline": cpu,
+        "sweep": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()} for s in sweep],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

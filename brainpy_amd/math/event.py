@@ -1,0 +1,209 @@
+"""Event-driven operators: ``bm.event.csrmv``, ``bm.event.csrmv_taichi``,
+``bm.event.info`` on the gfx950 library.
+
+Signatures, validation and error messages follow
+``brainpy/_src/math/event/_csr_matvec.py:38-110``,
+``event/_csr_matvec_taichi.py:344-452`` and
+``event/_info_collection.py:24-43``.  Inputs may be numpy arrays, python
+scalars or torch tensors; results are torch tensors on the GPU.
+"""
+from typing import Tuple
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ._common import acc_dtype, data_tensor, index_tensor, ndim
+
+__all__ = ["csrmv", "csrmv_taichi", "info", "csrmv_fixed_order", "csr_to_csc", "csrmv_add"]
+
+
+def _check(data, indices, indptr, events, shape, transpose, allowed_index, index_msg):
+    if ndim(data) == 1:
+        if data.shape[0] not in [1, indices.shape[0]]:
+            raise ValueError('The size of data should be 1 or be consistent with indices.'
+                             f'But we got {tuple(data.shape)} != {tuple(indices.shape)}, {tuple(data.shape)} != 1.')
+    else:
+        raise ValueError('data should be a scalar or 1D vector. '
+                         f'But we got {ndim(data)}-D array.')
+    if ndim(indices) != 1:
+        raise ValueError('indices should be a 1D vector with integer type.')
+    if ndim(indptr) != 1:
+        raise ValueError('indptr should be a 1D vector with integer type.')
+    if indices.dtype not in allowed_index:
+        raise ValueError(index_msg.format('indices'))
+    if indptr.dtype not in allowed_index:
+        raise ValueError(index_msg.format('indptr'))
+    if ndim(events) != 1:
+        raise ValueError('events should be a 1D vector.')
+    if len(shape) != 2:
+        raise ValueError('shape should be a length-2 tuple.')
+    if transpose:
+        if events.shape[0] != shape[0]:
+            raise ValueError(f'Shape mismatch, vec ({events.shape[0]},) @ mat {shape}.')
+    else:
+        if events.shape[0] != shape[1]:
+            raise ValueError(f'Shape mismatch, mat {shape} @ vec ({events.shape[0]},).')
+
+
+_PUBLIC_INDEX = {torch.int32, torch.int64, np.dtype(np.int32), np.dtype(np.int64), np.dtype(np.uint32),
+                 np.dtype(np.uint64)}
+_TAICHI_INDEX = _PUBLIC_INDEX | {torch.int8, torch.int16, torch.uint8, np.dtype(np.int8), np.dtype(np.int16),
+                                 np.dtype(np.uint8), np.dtype(np.uint16)}
+
+
+def _as_array(x):
+    if isinstance(x, (torch.Tensor, np.ndarray)):
+        return x
+    return np.asarray(x)
+
+
+def _events_tensor(events):
+    ev = _lib.to_device(events)
+    if ev.dtype == torch.bool or ev.dtype in (torch.float32, torch.float64):
+        return ev
+    return ev.to(torch.float32)
+
+
+def _run(data, indices, indptr, events, shape, transpose, rule):
+    d = data_tensor(data)
+    idx = index_tensor(indices)
+    ptr = index_tensor(indptr)
+    if ptr.dtype != idx.dtype:
+        ptr = ptr.to(idx.dtype)
+    ev = _events_tensor(events)
+    if d.dtype == torch.float16:
+        d = d.to(torch.float32)
+    m, n = int(shape[0]), int(shape[1])
+    if ptr.shape[0] != m + 1:
+        raise ValueError(f'indptr should have shape ({m + 1},), but we got {tuple(ptr.shape)}.')
+    out = torch.empty(n if transpose else m, dtype=acc_dtype(d.dtype), device=ev.device)
+    rc = _lib.lib().bpx_event_csrmv(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), d.shape[0], _lib.ptr(idx),
+                                    _lib.ptr(ptr), _lib.bpx_dtype(idx), idx.shape[0], _lib.ptr(ev),
+                                    _lib.bpx_dtype(ev), rule, m, n, int(bool(transpose)), _lib.ptr(out))
+    _lib.check(rc, "bpx_event_csrmv")
+    return out
+
+
+def csrmv(data, indices, indptr, events, *, shape: Tuple[int, int], transpose: bool = False) -> torch.Tensor:
+    """Product of a sparse CSR matrix and a dense event vector.
+
+    ``transpose=True`` computes ``events @ A`` event-driven: only rows with a
+    truthy event are read.  Bool events are truthy when set; float events
+    when ``> 0`` (the Numba rule of event/_csr_matvec.py:349).  Weights may
+    be f32, f64 or bf16 (bf16 is stored at half the bytes and accumulated in
+    f32).  Returns ``(shape[1] if transpose else shape[0],)``.
+    Reference: brainpy/_src/math/event/_csr_matvec.py:38-110.
+    """
+    data, indices, indptr, events = (_as_array(x) for x in (data, indices, indptr, events))
+    if ndim(data) == 0:
+        data = data.reshape(1)
+    _check(data, indices, indptr, events, shape, transpose, _PUBLIC_INDEX,
+           '{} should be a 1D vector with int32 or int64 type.')
+    return _run(data, indices, indptr, events, shape, transpose, _lib.RULE_POSITIVE)
+
+
+def csrmv_taichi(data, indices, indptr, events, *, shape: Tuple[int, int], transpose: bool = False):
+    """Taichi twin of :func:`csrmv` (float events truthy when ``!= 0``);
+    returns a 1-tuple like the reference, or a bare zero vector when
+    ``indices`` is empty.  Reference: event/_csr_matvec_taichi.py:344-452."""
+    data, indices, indptr, events = (_as_array(x) for x in (data, indices, indptr, events))
+    if ndim(data) == 0:
+        data = data.reshape(1)
+    _check(data, indices, indptr, events, shape, transpose, _TAICHI_INDEX,
+           '{} should be a 1D vector with int8, int16, int32, int64, uint8, uint16, uint32 or uint64 type.')
+    if indices.shape[0] == 0:
+        d = data_tensor(data)
+        return torch.zeros(shape[1] if transpose else shape[0], dtype=acc_dtype(d.dtype), device=_lib.device())
+    return (_run(data, indices, indptr, events, shape, transpose, _lib.RULE_NONZERO),)
+
+
+def csr_to_csc(indices, indptr, shape, data=None):
+    """CSR -> CSC with ascending row indices inside every column (stable
+    counting sort on the host; a one-time set-up step).  Returns numpy
+    ``(csc_rows int32, csc_colptr int32, csc_data or None)``."""
+    indices = np.asarray(indices.cpu() if isinstance(indices, torch.Tensor) else indices).astype(np.int64)
+    indptr = np.asarray(indptr.cpu() if isinstance(indptr, torch.Tensor) else indptr).astype(np.int64)
+    m, n = shape
+    rows = np.repeat(np.arange(m, dtype=np.int64), np.diff(indptr))
+    order = np.argsort(indices, kind="stable")
+    colptr = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(indices, minlength=n), out=colptr[1:])
+    csc_data = None
+    if data is not None:
+        d = np.asarray(data.cpu() if isinstance(data, torch.Tensor) else data)
+        csc_data = d if d.size == 1 else d[order]
+    return rows[order].astype(np.int32), colptr.astype(np.int32), csc_data
+
+
+def csrmv_fixed_order(csc_data, csc_rows, csc_colptr, events, *, shape: Tuple[int, int],
+                      rule: int = _lib.RULE_POSITIVE, out: torch.Tensor = None) -> torch.Tensor:
+    """Deterministic ``events @ A`` (transpose event csrmv) from the CSC form
+    of :func:`csr_to_csc`: every output is summed in ascending pre-synaptic
+    order, bit-identical to the serial CPU scatter of
+    event/_csr_matvec.py:321-340 regardless of scheduling.  With ``out``
+    given, the sums are added into it (``out += events @ A``)."""
+    d = data_tensor(csc_data).to(torch.float32)
+    rows = _lib.to_device(csc_rows, torch.int32)
+    colptr = _lib.to_device(csc_colptr, torch.int32)
+    ev = _events_tensor(events)
+    if ev.dtype == torch.float64:
+        ev = ev.to(torch.float32)
+    m, n = int(shape[0]), int(shape[1])
+    accumulate = out is not None
+    if out is None:
+        out = torch.empty(n, dtype=torch.float32, device=ev.device)
+    elif out.dtype != torch.float32 or out.shape != (n,) or not out.is_contiguous():
+        raise ValueError(f'out must be a contiguous float32 vector of shape ({n},).')
+    rc = _lib.lib().bpx_event_csrmv_fixed_order(_lib.stream_ptr(), _lib.ptr(d), _lib.BPX_F32, d.shape[0],
+                                                _lib.ptr(rows), _lib.ptr(colptr), rows.shape[0], _lib.ptr(ev),
+                                                _lib.bpx_dtype(ev), rule, m, n, int(accumulate), _lib.ptr(out))
+    _lib.check(rc, "bpx_event_csrmv_fixed_order")
+    return out
+
+
+def csrmv_add(out, data, indices, indptr, events, *, shape: Tuple[int, int], rule: int = _lib.RULE_POSITIVE):
+    """``out += events @ A`` (transpose event csrmv accumulated in place,
+    FP32 atomics); the conductance update ``g += csrmv(...)`` of
+    dyn/synapses/abstract_models.py:133-134 without a temporary."""
+    d = data_tensor(data)
+    idx = index_tensor(indices)
+    ptr = index_tensor(indptr)
+    if ptr.dtype != idx.dtype:
+        ptr = ptr.to(idx.dtype)
+    ev = _events_tensor(events)
+    if ev.dtype == torch.float64:
+        ev = ev.to(torch.float32)
+    m, n = int(shape[0]), int(shape[1])
+    if out.dtype != torch.float32 or out.shape != (n,) or not out.is_contiguous():
+        raise ValueError(f'out must be a contiguous float32 vector of shape ({n},).')
+    rc = _lib.lib().bpx_event_csrmv_add(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), d.shape[0], _lib.ptr(idx),
+                                        _lib.ptr(ptr), _lib.bpx_dtype(idx), idx.shape[0], _lib.ptr(ev),
+                                        _lib.bpx_dtype(ev), rule, m, n, _lib.ptr(out))
+    _lib.check(rc, "bpx_event_csrmv_add")
+    return out
+
+
+def info(events):
+    """Collect event information: ``(ids, num)`` with ``ids`` the ascending
+    indices of truthy events padded with -1 and ``num`` their count (shape
+    ``(1,)``; batched ``[B, N]`` input gives ``[B, N]`` and ``[B]``).
+    Reference: brainpy/_src/math/event/_info_collection.py:24-43, 54-66."""
+    ev = _lib.to_device(events)
+    if ev.ndim not in (1, 2):
+        raise TypeError('Only support 1D boolean vector.')
+    if ev.dtype not in (torch.bool, torch.float32, torch.float64, torch.int32):
+        ev = ev.to(torch.float32)
+    batched = ev.ndim == 2
+    b, n = (ev.shape[0], ev.shape[1]) if batched else (1, ev.shape[0])
+    ids = torch.empty((b, n), dtype=torch.int32, device=ev.device)
+    num = torch.empty(b, dtype=torch.int32, device=ev.device)
+    L = _lib.lib()
+    wsb = L.bpx_event_info_workspace(b, n)
+    ws = torch.empty(max(int(wsb), 4), dtype=torch.uint8, device=ev.device)
+    rc = L.bpx_event_info(_lib.stream_ptr(), _lib.ptr(ev), _lib.bpx_dtype(ev), b, n, _lib.ptr(ids), _lib.ptr(num),
+                          _lib.ptr(ws), ws.numel())
+    _lib.check(rc, "bpx_event_info")
+    if batched:
+        return ids, num
+    return ids.reshape(n), num.reshape(1)

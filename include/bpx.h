@@ -1,0 +1,186 @@
+/*
+ * bpx.h -- C ABI of libbpx.so, the gfx950 (MI355X) implementation of
+ * BrainPy's event-driven synaptic hot path.
+ *
+ * Conventions
+ *  - Every entry point takes a HIP stream (`void*` = hipStream_t, NULL = the
+ *    default stream) and device pointers.  Nothing is allocated and nothing
+ *    synchronises the host; inputs are read-only.
+ *  - Outputs are fully written by the call (the library zero-fills before
+ *    accumulating, on the same stream): callers need not pre-zero them.  The
+ *    reference's kernels accumulate with `+=` into outputs that their
+ *    launcher must zero (event/_csr_matvec_taichi.py:45,172).
+ *  - Return value: BPX_OK (0) or a negative BPX_ERR_* code; bpx_status_string
+ *    gives its text.
+ *  - Stateless and reentrant; concurrent calls on different streams are safe.
+ *
+ * Reference paths are relative to the chaoming0625/BrainPy 2.5.0 tree.
+ */
+#ifndef BPX_H_
+#define BPX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define BPX_OK 0
+#define BPX_ERR_DTYPE (-1)      /* unsupported dtype or dtype combination */
+#define BPX_ERR_SHAPE (-2)      /* inconsistent sizes */
+#define BPX_ERR_WORKSPACE (-3)  /* workspace missing or too small */
+#define BPX_ERR_LAUNCH (-4)     /* a kernel launch failed */
+#define BPX_ERR_ARG (-5)        /* invalid scalar argument */
+
+/* dtype codes */
+#define BPX_BOOL 0   /* 1 byte, truthy if non-zero */
+#define BPX_F32 1
+#define BPX_F64 2
+#define BPX_I32 3
+#define BPX_I64 4
+#define BPX_BF16 5   /* weights only; accumulated in f32 */
+#define BPX_F16 6    /* weights only; accumulated in f32 */
+
+/* float-event truthiness */
+#define BPX_RULE_POSITIVE 0  /* event > 0  (Numba kernels, event/_csr_matvec.py:349) */
+#define BPX_RULE_NONZERO 1   /* event != 0 (Taichi kernels, event/_csr_matvec_taichi.py:70) */
+
+/* jitconn weight distributions and layouts */
+#define BPX_DIST_HOMO 0
+#define BPX_DIST_UNIFORM 1
+#define BPX_DIST_NORMAL 2
+#define BPX_LAYOUT_LANE32 0  /* Taichi GPU kernels: 32 streams per row/column */
+#define BPX_LAYOUT_ROW 1     /* Taichi CPU kernels: one stream per row/column */
+
+const char* bpx_status_string(int status);
+const char* bpx_version(void);
+
+/*
+ * Spike compaction.  Replaces the `nonzero_*` brainpylib GPU target and the
+ * Numba `_event_info` CPU target of bm.event.info
+ * (event/_info_collection.py:24-43 wrapper, :78-121 GPU translation,
+ * :143-154 CPU loop, batched :54-66).
+ * events[batch, n] -> ids[batch, n] (ascending indices of truthy events,
+ * then -1), num[batch].  Float events are truthy if != 0.
+ */
+size_t bpx_event_info_workspace(int64_t batch, int64_t n);
+int bpx_event_info(void* stream, const void* events, int ev_dtype, int64_t batch, int64_t n,
+                   int32_t* ids, int32_t* num, void* workspace, size_t workspace_bytes);
+
+/*
+ * Event-driven CSR matvec.  Replaces the brainpylib targets
+ * `event_csrmv{_homo,_heter}{_float,_double}{_int,_long}{_bool,}{_transpose,}`
+ * (event/_csr_matvec.py:453-500) and the Taichi kernels of csrmv_taichi
+ * (event/_csr_matvec_taichi.py:34-312).
+ *   transpose=1: out[n] = sum over rows i with a truthy event of row i of A
+ *   transpose=0: out[m] = A restricted to columns with a truthy event, row sums
+ * data: data_len == 1 (homogeneous) or == nnz; dtype f32/f64/bf16/f16.
+ * indices/indptr: int32 or int64 (index_dtype, both the same).
+ * events: bool/f32/f64 with ev_rule for float events.
+ * out: f64 if data is f64, else f32.
+ */
+int bpx_event_csrmv(void* stream, const void* data, int data_dtype, int64_t data_len,
+                    const void* indices, const void* indptr, int index_dtype, int64_t nnz,
+                    const void* events, int ev_dtype, int ev_rule,
+                    int64_t m, int64_t n, int transpose, void* out);
+
+/*
+ * CSR matvec with a dense vector.  Replaces `cuda_csr_matvec` (cuSPARSE,
+ * sparse/_csr_mv.py:197-210) and `csrmv_{homo,heter}_{scalar,vector}`
+ * (sparse/_csr_mv.py:257-478), Numba CPU (sparse/_csr_mv.py:126-165) and the
+ * Taichi kernels (sparse/_csr_mv_taichi.py:28-155).
+ * vector dtype must equal the accumulation type (f32, or f64 for f64 data).
+ */
+int bpx_csrmv(void* stream, const void* data, int data_dtype, int64_t data_len,
+              const void* indices, const void* indptr, int index_dtype, int64_t nnz,
+              const void* vector, int vec_dtype, int64_t m, int64_t n, int transpose, void* out);
+
+/*
+ * Fixed-order event scatter: the transpose event csrmv computed as a gather
+ * over a CSC copy whose row indices ascend within each column, so every
+ * output is summed in ascending pre-synaptic order -- bit-identical to the
+ * serial CPU scatter of event/_csr_matvec.py:321-340 and independent of
+ * scheduling.  csc_data follows the CSC order (data_len 1 = homogeneous).
+ * accumulate=1 adds the column sums into out (out[c] = out[c] + sum_c, the
+ * order of `g += csrmv(...)`); accumulate=0 overwrites out.
+ */
+int bpx_event_csrmv_fixed_order(void* stream, const void* csc_data, int data_dtype, int64_t data_len,
+                                const int32_t* csc_rows, const int32_t* csc_colptr, int64_t nnz,
+                                const void* events, int ev_dtype, int ev_rule,
+                                int64_t m, int64_t n, int accumulate, void* out);
+
+/*
+ * Accumulating transpose event csrmv: out[n] += events @ A (no zero-fill).
+ * No reference counterpart: the reference adds a fresh csrmv result into
+ * the conductance (`g += csrmv(...)`, dyn/synapses/abstract_models.py:
+ * 133-134); the network step uses this form to add straight into g.
+ * Same kernels and arguments as bpx_event_csrmv with transpose=1.
+ */
+int bpx_event_csrmv_add(void* stream, const void* data, int data_dtype, int64_t data_len,
+                        const void* indices, const void* indptr, int index_dtype, int64_t nnz,
+                        const void* events, int ev_dtype, int ev_rule, int64_t m, int64_t n, void* out);
+
+/*
+ * Just-in-time connectivity matvec y = M @ v, M never stored.  Replaces the
+ * Taichi kernels of jitconn/_event_matvec_taichi.py:38-1106 (event) and
+ * jitconn/_matvec_taichi.py:30-746 (dense vector), and stands in for the
+ * brainpylib targets `gpu_jit_{event_,}csrmv_prob_{homo,uniform,normal}_v2_*`
+ * (jitconn/_event_matvec.py:198-229, jitconn/_matvec.py:341-378).
+ * num_row x num_col is the matrix shape after the transpose swap of
+ * jitconn/_matvec_taichi.py:197-207 (num_row = len(out), num_col = len(vec)).
+ *   event=1: vec is events (bool or f32); event=0: vec is a dense f32 vector.
+ *   dist: BPX_DIST_*; (w0, w1) = weight | (low, high) | (mu, sigma).
+ *   clen = ceil(1/p)*2-1; seed as u32.
+ * out: f32 [num_row].
+ */
+int bpx_jitconn_mv(void* stream, const void* vec, int vec_dtype, int event,
+                   int dist, float w0, float w1, int32_t clen, uint32_t seed,
+                   int64_t num_row, int64_t num_col, int outdim_parallel, int layout, float* out);
+
+/*
+ * Fused LIF + Expon + COBA exponential-Euler step of one population (f32).
+ * Replaces the per-neuron integrator of LifRefLTC / LifRef
+ * (dyn/neurons/lif.py:461-500, 550-556), Expon (dyn/synapses/
+ * abstract_models.py:121-134), COBA (dyn/outs/outputs.py:53-54) and
+ * ExponentialEuler (integrators/ode/exponential.py:357-362).
+ * g: host array of n_g (<= 4) device pointers to conductances [n];
+ * tau_g, E_g, c_g: host arrays (c_g = dt*exprel(-dt/tau_g) in f32);
+ * I_vec: optional per-neuron external input (NULL -> I_ext).
+ */
+int bpx_lif_coba_step(void* stream, int64_t n, int ltc, int n_g,
+                      float* V, float* t_last, uint8_t* spike, float* const* g,
+                      const float* tau_g, const float* E_g, const float* c_g,
+                      float I_ext, const float* I_vec,
+                      float V_rest, float V_th, float V_reset, float tau, float tau_ref, float R,
+                      float t, float dt);
+
+/*
+ * Host helper: the Expon exp_euler factor c = dt * exprel(dt * (-(1/tau)))
+ * in f32 with the library's deterministic exp, i.e. the `dt*phi` of
+ * integrators/ode/exponential.py:357-362 for dg/dt = -g/tau
+ * (dyn/synapses/abstract_models.py:118-119).  Pass it as c_g above.
+ */
+float bpx_expon_coef(float tau, float dt);
+
+/* Plain device copy (bandwidth calibration of the HBM counters). */
+int bpx_copy(void* stream, const void* src, void* dst, int64_t bytes);
+
+/*
+ * Kernel timer.  While enabled, the dominant kernel of each entry point is
+ * launched with HIP start/stop events on the caller's stream; bpx_timer_read
+ * synchronises those events and returns the summed device milliseconds and
+ * launch count for one kernel id (0 event scatter, 1 event gather,
+ * 2 csr scatter, 3 csr gather, 4 jitconn outdim, 5 jitconn scatter,
+ * 6 event info, 7 lif step, 8 fixed order, 9 copy).
+ */
+int bpx_timer_enable(int on);
+int bpx_timer_read(int kernel_id, double* total_ms, int64_t* launches);
+int bpx_timer_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BPX_H_ */

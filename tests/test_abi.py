@@ -1,0 +1,96 @@
+"""The C-ABI library loads and exports every entry point include/bpx.h
+declares (no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "bpx.h")
+LIB = os.path.join(ROOT, "brainpy_amd", "libbpx.so")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bpx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    for must in ("bpx_event_csrmv", "bpx_csrmv", "bpx_event_info", "bpx_jitconn_mv", "bpx_lif_coba_step",
+                 "bpx_event_csrmv_fixed_order", "bpx_status_string"):
+        assert must in names
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libbpx.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libbpx.so not built")
+def test_host_only_entry_points():
+    lib = ctypes.CDLL(LIB)
+    lib.bpx_status_string.restype = ctypes.c_char_p
+    assert lib.bpx_status_string(0) == b"ok"
+    assert b"dtype" in lib.bpx_status_string(-1)
+    lib.bpx_version.restype = ctypes.c_char_p
+    assert b"gfx950" in lib.bpx_version()
+    lib.bpx_event_info_workspace.restype = ctypes.c_size_t
+    lib.bpx_event_info_workspace.argtypes = [ctypes.c_int64, ctypes.c_int64]
+    assert lib.bpx_event_info_workspace(1, 4096) == 8
+    assert lib.bpx_event_info_workspace(2, 4097) == 32
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libbpx.so not built")
+def test_argument_validation_without_launch():
+    # invalid arguments are rejected before anything touches the device
+    lib = ctypes.CDLL(LIB)
+    f = lib.bpx_jitconn_mv
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                  ctypes.c_float, ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                  ctypes.c_int, ctypes.c_void_p]
+    assert f(None, None, 1, 1, 0, 1.0, 0.0, 0, 1, 10, 10, 1, 0, None) == -5       # clen < 1
+    assert f(None, None, 1, 1, 7, 1.0, 0.0, 3, 1, 10, 10, 1, 0, None) == -5       # bad dist
+    assert f(None, None, 2, 1, 0, 1.0, 0.0, 3, 1, 10, 10, 1, 0, None) == -1       # f64 events
+    g = lib.bpx_event_csrmv
+    g.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                  ctypes.c_int64, ctypes.c_int, ctypes.c_void_p]
+    assert g(None, None, 1, 5, None, None, 3, 7, None, 0, 0, 3, 3, 1, None) == -2  # data_len mismatch
+    assert g(None, None, 1, 1, None, None, 3, 7, None, 0, 9, 3, 3, 1, None) == -5  # bad rule
+
+
+def test_fastmod_exact():
+    # the u32 division by an invariant divisor used for `bits % clen`
+    # (brainpy_amd/csrc/bpx_rng.h) is exact: compile its host side and compare
+    import subprocess
+    import tempfile
+    src = r'''
+#include <stdio.h>
+#include <stdint.h>
+#include "bpx_rng.h"
+int main(){
+  uint32_t ds[] = {1u,2u,3u,7u,19u,99u,199u,19999u,65535u,65536u,1000003u,2147483647u,2147483648u,4294967295u};
+  uint64_t x = 88172645463325252ull; long bad = 0;
+  for (unsigned di = 0; di < sizeof(ds)/4; ++di) {
+    bpx::DivU32 dv = bpx::DivU32::make(ds[di]);
+    for (int i = 0; i < 200000; ++i) {
+      x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+      uint32_t n = (uint32_t)x;
+      if (i < 64) n = i; if (i >= 64 && i < 128) n = 0xFFFFFFFFu - (i - 64);
+      if (dv.mod(n) != n % ds[di] || dv.div(n) != n / ds[di]) ++bad;
+    }
+  }
+  printf("%ld\n", bad); return bad != 0;
+}'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.cpp")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "t")
+        subprocess.check_call(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "brainpy_amd", "csrc"), c, "-o", exe])
+        out = subprocess.run([exe], capture_output=True, text=True)
+        assert out.returncode == 0 and out.stdout.strip() == "0", out.stdout

@@ -1,0 +1,187 @@
+"""GPU parity: bm.event.csrmv / csrmv_taichi / info and bm.sparse.csrmv
+against the oracle and the FP64 closed form, through the C ABI."""
+import numpy as np
+import pytest
+import torch
+
+import brainpy_amd.math as bm
+from oracle import orc
+from tests.helpers import (bernoulli, bf16_bits_to_f32, dense_of, f32_to_bf16_bits, fixed_prob_csr,
+                           random_csr)
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(100, 200), (200, 100), (10, 1000), (1000, 10), (1, 1), (257, 129), (3000, 50)]
+RTOL = 1e-5   # per-call FP32 current vs FP64, relative
+
+
+def _tol_check(y, ref, scale):
+    err = np.abs(y.astype(np.float64) - ref)
+    assert (err <= RTOL * np.maximum(np.abs(ref), scale) + 1e-6).all(), float(err.max())
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("transpose", [True, False])
+@pytest.mark.parametrize("homo", [True, False])
+@pytest.mark.parametrize("idx_dtype", [np.int32, np.int64])
+def test_event_csrmv_vs_fp64(shape, transpose, homo, idx_dtype):
+    m, n = shape
+    indices, indptr = random_csr(m, n, 0.3, seed=m * 7 + n, dtype=idx_dtype)
+    rng = np.random.default_rng(0)
+    data = np.float32(1.5) if homo else rng.normal(0.5, 0.1, indices.size).astype(np.float32)
+    ev = bernoulli(m if transpose else n, 0.3, 1)
+    D = dense_of(data, indices, indptr, shape)
+    ref = ev.astype(np.float64) @ D if transpose else D @ ev.astype(np.float64)
+    y = bm.event.csrmv(data, indices, indptr, ev, shape=shape, transpose=transpose).cpu().numpy()
+    assert y.dtype == np.float32 and y.shape == ref.shape
+    scale = np.abs(D).sum(axis=0 if transpose else 1).max() if D.size else 1.0
+    _tol_check(y, ref, scale)
+    # bool == float 0/1
+    y2 = bm.event.csrmv(data, indices, indptr, ev.astype(np.float32), shape=shape, transpose=transpose)
+    _tol_check(y2.cpu().numpy(), ref, scale)
+
+
+@pytest.mark.parametrize("transpose", [True, False])
+def test_event_csrmv_homo_integer_exact(transpose):
+    # homo weight 1.0 and bool events: all partial sums are exact integers,
+    # so the GPU result is bit-identical to the oracle whatever the order
+    indices, indptr = fixed_prob_csr(2000, 1500, 0.05, 3)
+    ev = bernoulli(2000 if transpose else 1500, 0.2, 4)
+    y = bm.event.csrmv(np.float32(1.0), indices, indptr, ev, shape=(2000, 1500), transpose=transpose)
+    ref = orc.event_csrmv(np.float32(1.0), indices, indptr, ev, (2000, 1500), transpose)
+    np.testing.assert_array_equal(y.cpu().numpy(), ref)
+
+
+def test_event_rules_and_taichi_twin():
+    indices, indptr = fixed_prob_csr(300, 200, 0.1, 5)
+    ev = np.zeros(300, np.float32)
+    ev[::3] = -1.0
+    ev[1::3] = 2.0
+    a = bm.event.csrmv(np.float32(1.0), indices, indptr, ev, shape=(300, 200), transpose=True).cpu().numpy()
+    b = bm.event.csrmv_taichi(np.float32(1.0), indices, indptr, ev, shape=(300, 200), transpose=True)
+    assert isinstance(b, tuple) and len(b) == 1
+    np.testing.assert_array_equal(a, orc.event_csrmv(np.float32(1.0), indices, indptr, ev, (300, 200), True,
+                                                     rule=orc.RULE_POSITIVE))
+    np.testing.assert_array_equal(b[0].cpu().numpy(),
+                                  orc.event_csrmv(np.float32(1.0), indices, indptr, ev, (300, 200), True,
+                                                  rule=orc.RULE_NONZERO))
+
+
+def test_f64_weights():
+    indices, indptr = random_csr(120, 90, 0.2, 6)
+    w = np.random.default_rng(1).normal(0.5, 0.1, indices.size)
+    ev = bernoulli(120, 0.3, 2)
+    y = bm.event.csrmv(w, indices, indptr, ev, shape=(120, 90), transpose=True)
+    assert y.dtype == torch.float64
+    np.testing.assert_allclose(y.cpu().numpy(), ev @ dense_of(w, indices, indptr, (120, 90)), rtol=1e-12)
+
+
+@pytest.mark.parametrize("transpose", [True, False])
+def test_bf16_weights(transpose):
+    m, n = 500, 400
+    indices, indptr = fixed_prob_csr(m, n, 0.05, 7)
+    w = np.random.default_rng(2).normal(0.5, 0.1, indices.size).astype(np.float32)
+    bits = f32_to_bf16_bits(w)
+    wb = torch.from_numpy(bits.view(np.int16)).view(torch.bfloat16)
+    ev = bernoulli(m if transpose else n, 0.2, 3)
+    y = bm.event.csrmv(wb, indices, indptr, ev, shape=(m, n), transpose=transpose).cpu().numpy()
+    Db = dense_of(bf16_bits_to_f32(bits), indices, indptr, (m, n))
+    ref = ev @ Db if transpose else Db @ ev
+    np.testing.assert_allclose(y, ref, rtol=1e-5, atol=1e-5)
+    # torch's own bf16 -> f32 conversion agrees with the bit pattern
+    np.testing.assert_array_equal(wb.float().numpy(), bf16_bits_to_f32(bits))
+
+
+def test_edge_cases():
+    # empty indices, all-false / all-true events, empty rows, m not a multiple of 64
+    shape = (130, 70)
+    indices, indptr = random_csr(*shape, 0.05, 8)
+    for ev in (np.zeros(130, bool), np.ones(130, bool)):
+        y = bm.event.csrmv(np.float32(1.0), indices, indptr, ev, shape=shape, transpose=True).cpu().numpy()
+        np.testing.assert_array_equal(y, orc.event_csrmv(np.float32(1.0), indices, indptr, ev, shape, True))
+    z = bm.event.csrmv_taichi(np.float32(1.0), np.zeros(0, np.int32), np.zeros(131, np.int32),
+                              np.ones(130, bool), shape=shape, transpose=True)
+    assert isinstance(z, torch.Tensor) and z.shape == (70,) and float(z.abs().sum()) == 0.0
+    y = bm.event.csrmv(np.float32(1.0), np.zeros(0, np.int32), np.zeros(131, np.int32), np.ones(130, bool),
+                       shape=shape, transpose=True)
+    assert float(y.abs().sum()) == 0.0
+
+
+def test_validation_errors():
+    indices, indptr = fixed_prob_csr(10, 20, 0.2, 1)
+    with pytest.raises(ValueError, match="Shape mismatch"):
+        bm.event.csrmv(1.0, indices, indptr, np.ones(11, bool), shape=(10, 20), transpose=True)
+    with pytest.raises(ValueError, match="size of data"):
+        bm.event.csrmv(np.ones(3, np.float32), indices, indptr, np.ones(10, bool), shape=(10, 20), transpose=True)
+    with pytest.raises(ValueError, match="int32 or int64"):
+        bm.event.csrmv(1.0, indices.astype(np.int16), indptr, np.ones(10, bool), shape=(10, 20), transpose=True)
+
+
+def test_repeat_calls_reuse_output_semantics():
+    # outputs are fully rewritten: two calls give the same values
+    indices, indptr = fixed_prob_csr(1000, 800, 0.05, 9)
+    ev = bernoulli(1000, 0.1, 5)
+    a = bm.event.csrmv(np.float32(1.0), indices, indptr, ev, shape=(1000, 800), transpose=True)
+    b = bm.event.csrmv(np.float32(1.0), indices, indptr, ev, shape=(1000, 800), transpose=True)
+    assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 4095, 4096, 4097, 100000, 1000000])
+@pytest.mark.parametrize("dtype", [np.bool_, np.float32])
+def test_event_info_exact(n, dtype):
+    ev = bernoulli(n, 0.02, n + 1).astype(dtype)
+    ids, num = bm.event.info(ev)
+    rid, rnum = orc.event_info(ev)
+    np.testing.assert_array_equal(ids.cpu().numpy(), rid)
+    np.testing.assert_array_equal(num.cpu().numpy(), rnum)
+
+
+def test_event_info_batched_exact():
+    ev = np.stack([bernoulli(100000, r, 9) for r in (0.0, 0.001, 0.1, 1.0)])
+    ids, num = bm.event.info(ev)
+    rid, rnum = orc.event_info(ev)
+    np.testing.assert_array_equal(ids.cpu().numpy(), rid)
+    np.testing.assert_array_equal(num.cpu().numpy(), rnum)
+
+
+def test_fixed_order_bit_exact_vs_serial_scatter():
+    m, n = 3000, 2500
+    indices, indptr = fixed_prob_csr(m, n, 0.02, 11, sort=False)
+    w = np.random.default_rng(4).normal(0.5, 0.1, indices.size).astype(np.float32)
+    ev = bernoulli(m, 0.1, 6)
+    rows, colptr, cd = bm.event.csr_to_csc(indices, indptr, (m, n), data=w)
+    y = bm.event.csrmv_fixed_order(cd, rows, colptr, ev, shape=(m, n)).cpu().numpy()
+    np.testing.assert_array_equal(y, orc.event_csrmv(w, indices, indptr, ev, (m, n), True))
+    yh = bm.event.csrmv_fixed_order(np.float32(0.6), rows, colptr, ev, shape=(m, n)).cpu().numpy()
+    np.testing.assert_array_equal(yh, orc.event_csrmv(np.float32(0.6), indices, indptr, ev, (m, n), True))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("transpose", [True, False])
+@pytest.mark.parametrize("homo", [True, False])
+def test_sparse_csrmv_vs_fp64(shape, transpose, homo):
+    m, n = shape
+    indices, indptr = random_csr(m, n, 0.3, seed=m + 3 * n)
+    rng = np.random.default_rng(5)
+    data = np.float32(0.7) if homo else rng.standard_normal(indices.size).astype(np.float32)
+    x = rng.standard_normal(m if transpose else n).astype(np.float32)
+    D = dense_of(data, indices, indptr, shape)
+    ref = x.astype(np.float64) @ D if transpose else D @ x.astype(np.float64)
+    y = bm.sparse.csrmv(data, indices, indptr, x, shape=shape, transpose=transpose).cpu().numpy()
+    scale = (np.abs(D) * np.abs(x.astype(np.float64))[:, None] if transpose else
+             np.abs(D) * np.abs(x.astype(np.float64))[None, :]).sum(axis=0 if transpose else 1)
+    err = np.abs(y - ref)
+    assert (err <= 1e-5 * np.maximum(scale, 1.0)).all(), float(err.max())
+    (yt,) = bm.sparse.csrmv_taichi(data, indices, indptr, x, shape=shape, transpose=transpose)
+    assert (np.abs(yt.cpu().numpy() - ref) <= 1e-5 * np.maximum(scale, 1.0)).all()
+
+
+def test_sparse_converters():
+    indices, indptr = random_csr(40, 30, 0.2, 12)
+    w = np.random.default_rng(6).standard_normal(indices.size).astype(np.float32)
+    D = bm.sparse.csr_to_dense(w, indices, indptr, shape=(40, 30)).cpu().numpy()
+    np.testing.assert_allclose(D, dense_of(w, indices, indptr, (40, 30)), rtol=1e-7)
+    r, c = bm.sparse.csr_to_coo(indices, indptr)
+    idx2, ptr2 = bm.sparse.coo_to_csr(r, c, num_row=40)
+    np.testing.assert_array_equal(idx2.cpu().numpy(), indices)
+    np.testing.assert_array_equal(ptr2.cpu().numpy(), indptr)

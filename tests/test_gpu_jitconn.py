@@ -1,0 +1,123 @@
+"""GPU parity: bm.jitconn operators against the oracle.
+
+Connectivity (indices) must be bit-exact: checked through one-hot probes
+(column c of M = M @ e_c), with homogeneous weight 1 and bool events
+(integer sums, exact in any order), and with uniform weights on one-hot
+probes (one connection per output, so the weight itself is compared
+bit-exactly).  Normal weights use device log/sqrt/sin: tolerance only.
+"""
+import numpy as np
+import pytest
+import torch
+
+import brainpy_amd.math as bm
+from oracle import np_ref, orc
+
+pytestmark = pytest.mark.gpu
+
+LAYOUTS = {"lane32": orc.LAYOUT_LANE32, "row": orc.LAYOUT_ROW}
+SHAPES = [(100, 200), (2, 1000), (1000, 2), (64, 64), (333, 517)]
+
+
+def _oracle(vec, dist, vmode, od, layout, shape, transpose, w0, w1, p, seed):
+    num_row, num_col = (shape[1], shape[0]) if transpose else shape
+    return orc.jitconn_mv(vec, dist, vmode, od, LAYOUTS[layout], num_row, num_col, w0, w1, np_ref.conn_len(p), seed)
+
+
+@pytest.mark.parametrize("layout", ["lane32", "row"])
+@pytest.mark.parametrize("od", [True, False])
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("p", [0.01, 0.1, 0.5])
+@pytest.mark.parametrize("transpose", [False, True])
+def test_event_homo_exact(layout, od, shape, p, transpose):
+    n_in = shape[0] if transpose else shape[1]
+    ev = np.random.default_rng(n_in).random(n_in) < 0.3
+    y = bm.jitconn.event_mv_prob_homo(ev, 1.0, p, 1234, shape=shape, transpose=transpose, outdim_parallel=od,
+                                      layout=layout).cpu().numpy()
+    ref = _oracle(ev, "homo", 1, od, layout, shape, transpose, 1.0, 0.0, p, 1234)
+    np.testing.assert_array_equal(y, ref)
+
+
+@pytest.mark.parametrize("layout", ["lane32", "row"])
+@pytest.mark.parametrize("od", [True, False])
+@pytest.mark.parametrize("shape", [(100, 200), (1000, 2), (33, 70)])
+@pytest.mark.parametrize("p", [0.1, 0.5])
+def test_connectivity_one_hot_exact(layout, od, shape, p):
+    M = bm.jitconn.get_conn_matrix(p, 42, shape=shape, outdim_parallel=od, layout=layout).cpu().numpy()
+    ref = np_ref.dense("homo", od, layout, shape[0], shape[1], 1.0, 0.0, np_ref.conn_len(p), 42)
+    np.testing.assert_array_equal(M, ref.astype(np.float32))
+
+
+@pytest.mark.parametrize("layout", ["lane32", "row"])
+@pytest.mark.parametrize("od", [True, False])
+def test_uniform_weights_one_hot_exact(layout, od):
+    shape, p = (150, 40), 0.2
+    Mref = np_ref.dense("uniform", od, layout, shape[0], shape[1], 0.0, 1.0, np_ref.conn_len(p), 7)
+    dev = torch.device("cuda")
+    for c in range(shape[1]):
+        e = torch.zeros(shape[1], dtype=torch.bool, device=dev)
+        e[c] = True
+        y = bm.jitconn.event_mv_prob_uniform(e, 0.0, 1.0, p, 7, shape=shape, outdim_parallel=od,
+                                             layout=layout).cpu().numpy()
+        np.testing.assert_array_equal(y, Mref[:, c].astype(np.float32))
+
+
+@pytest.mark.parametrize("layout", ["lane32", "row"])
+@pytest.mark.parametrize("od", [True, False])
+@pytest.mark.parametrize("dist", ["uniform", "normal"])
+@pytest.mark.parametrize("vmode", [0, 1, 2])
+def test_weighted_vs_oracle(layout, od, dist, vmode):
+    shape, p, seed = (400, 300), 0.05, 99
+    rng = np.random.default_rng(1)
+    w0, w1 = (0.0, 1.0) if dist == "uniform" else (0.5, 0.1)
+    if vmode == 0:
+        vec = rng.standard_normal(300).astype(np.float32)
+        f = bm.jitconn.mv_prob_uniform if dist == "uniform" else bm.jitconn.mv_prob_normal
+    else:
+        vec = rng.random(300) < 0.3
+        if vmode == 2:
+            vec = vec.astype(np.float32) * rng.choice([1.0, 2.0, -1.0], 300).astype(np.float32)
+        f = bm.jitconn.event_mv_prob_uniform if dist == "uniform" else bm.jitconn.event_mv_prob_normal
+    y = f(vec, w0, w1, p, seed, shape=shape, outdim_parallel=od, layout=layout).cpu().numpy()
+    ref = _oracle(vec, dist, vmode, od, layout, shape, False, w0, w1, p, seed)
+    np.testing.assert_allclose(y, ref, rtol=1e-5, atol=2e-5)
+
+
+@pytest.mark.parametrize("od", [True, False])
+def test_dense_homo_vs_oracle(od):
+    shape, p = (500, 700), 0.05
+    x = np.random.default_rng(2).standard_normal(700).astype(np.float32)
+    for layout in ("lane32", "row"):
+        y = bm.jitconn.mv_prob_homo(x, 0.7, p, 5, shape=shape, outdim_parallel=od, layout=layout).cpu().numpy()
+        ref = _oracle(x, "homo", 0, od, layout, shape, False, 0.7, 0.0, p, 5)
+        np.testing.assert_allclose(y, ref, rtol=1e-5, atol=1e-5)
+
+
+def test_transpose_shape_swap_identity():
+    # (shape, transpose) == (shape[::-1], not transpose)
+    # (jitconn/tests/test_event_matvec_taichi.py:75-83)
+    ev = np.random.default_rng(4).random(120) < 0.5
+    a = bm.jitconn.event_mv_prob_homo(ev, 1.0, 0.1, 9, shape=(120, 80), transpose=True)
+    b = bm.jitconn.event_mv_prob_homo(ev, 1.0, 0.1, 9, shape=(80, 120), transpose=False)
+    assert torch.equal(a, b)
+
+
+def test_validation():
+    with pytest.raises(ValueError, match="Shape mismatch"):
+        bm.jitconn.event_mv_prob_homo(np.ones(10, bool), 1.0, 0.1, 1, shape=(5, 11))
+    with pytest.raises(ValueError, match="outdim_parallel"):
+        bm.jitconn.event_mv_prob_homo(np.ones(10, bool), 1.0, 0.1, 1, shape=(5, 10), outdim_parallel=1)
+
+
+@pytest.mark.parametrize("od", [True, False])
+def test_full_size_1m(od):
+    # 1M x 1M, p = 1e-4, seed 42, 2% bool spikes, homo weight 1: the whole
+    # output is an integer count, compared bit-exactly with the oracle
+    n = 1_000_000
+    ev = np.random.default_rng(42).random(n) < 0.02
+    y = bm.jitconn.event_mv_prob_homo(ev, 1.0, 1e-4, 42, shape=(n, n), outdim_parallel=od).cpu().numpy()
+    ref = orc.jitconn_mv(ev, "homo", 1, od, orc.LAYOUT_LANE32, n, n, 1.0, 0.0, np_ref.conn_len(1e-4), 42)
+    np.testing.assert_array_equal(y, ref)
+    # deterministic across calls (outdim_parallel writes each row once)
+    y2 = bm.jitconn.event_mv_prob_homo(ev, 1.0, 1e-4, 42, shape=(n, n), outdim_parallel=od).cpu().numpy()
+    np.testing.assert_array_equal(y, y2)
