@@ -32,14 +32,19 @@ PROB = 0.01
 K_ROW = int(N * PROB)
 RATES = (0.001, 0.01, 0.05, 0.10)
 HEADLINE = ("heter_f32", 0.10)
+TILES = 32   # output tiles per copy on MI355X (CUs per XCD); set from the plan at run time
 
 
-def algorithmic_bytes(active_rows, kind):
-    """Bytes the scatter must move once: events (1 B each), the indptr pair
-    of each active row, indices and weights of active rows, one write of
-    the output."""
+def algorithmic_bytes(active_rows, kind, path="csr"):
+    """Bytes the scatter must move once: events (1 B each), the row offsets
+    of each active row, column indices and weights of active rows, one write
+    of the output.  The plain CSR path reads 4-byte column indices and one
+    offset pair per active row; the tiled layout reads 2-byte tile-local
+    indices and one offset pair per active row and tile."""
     w_bytes = {"homo": 0, "heter_f32": 4, "heter_bf16": 2}[kind]
-    return M * 1 + active_rows * 8 + active_rows * K_ROW * (4 + w_bytes) + N * 4
+    if path == "csr":
+        return M * 1 + active_rows * 8 + active_rows * K_ROW * (4 + w_bytes) + N * 4
+    return M * 1 + active_rows * 8 * TILES + active_rows * K_ROW * (2 + w_bytes) + N * 4
 
 
 def main():
@@ -81,9 +86,24 @@ def main():
         g.manual_seed(int(r * 1e6) + 11)
         spikes[r] = torch.rand(M, generator=g, device=dev) < r
     out = torch.empty(N, dtype=torch.float32, device=dev)
+    from brainpy_amd.math.event import TiledCSR
+    print("# building tiled plans", file=sys.stderr, flush=True)
+    plans = {
+        "homo": TiledCSR(weights["homo"], indices, indptr, shape=(M, N)),
+        "heter_f32": TiledCSR(w_f32, indices, indptr, shape=(M, N)),
+        "heter_bf16": TiledCSR(w_f32, indices, indptr, shape=(M, N), weight_dtype=torch.bfloat16),
+    }
     torch.cuda.synchronize()
+    import ctypes
 
-    def call(kind, r):
+    def call_tiled(kind, r):
+        plan = plans[kind]
+        rc = L.bpx_event_csrmv_tiled(sp, ctypes.byref(plan.plan), _lib.ptr(spikes[r]), _lib.BPX_BOOL,
+                                     _lib.RULE_POSITIVE, 0, _lib.ptr(out))
+        if rc != 0:
+            raise RuntimeError(f"bpx_event_csrmv_tiled failed ({rc})")
+
+    def call_csr(kind, r):
         w = weights[kind]
         rc = L.bpx_event_csrmv(sp, _lib.ptr(w), _lib.bpx_dtype(w), w.numel(), _lib.ptr(indices), _lib.ptr(indptr),
                                _lib.BPX_I32, nnz, _lib.ptr(spikes[r]), _lib.BPX_BOOL, _lib.RULE_POSITIVE, M, N, 1,
@@ -91,7 +111,9 @@ def main():
         if rc != 0:
             raise RuntimeError(f"bpx_event_csrmv failed ({rc})")
 
-    def measure(kind, r, steps, warmup):
+    def measure(kind, r, steps, warmup, path="tiled"):
+        call = call_tiled if path == "tiled" else call_csr
+        kid = _lib.KID["tiled"] if path == "tiled" else _lib.KID["event_scatter"]
         active = int(spikes[r].sum().item())
         events = active * K_ROW
         for _ in range(warmup):
@@ -109,26 +131,39 @@ def main():
             call(kind, r)
         torch.cuda.synchronize()
         _lib.timer_enable(False)
-        ms, launches = _lib.timer_read(_lib.KID["event_scatter"])
+        ms, launches = _lib.timer_read(kid)
         kern_s = ms / 1e3 / max(launches, 1)
-        abytes = algorithmic_bytes(active, kind)
+        abytes = algorithmic_bytes(active, kind, path)
         return {
-            "kind": kind, "rate": r, "active_rows": active, "events_per_call": events,
+            "path": path, "kind": kind, "rate": r, "active_rows": active, "events_per_call": events,
             "ms_per_step": wall / steps * 1e3, "events_per_s": events * steps / wall,
             "kernel_us": kern_s * 1e6, "kernel_launches": launches, "algorithmic_bytes": abytes,
             "kernel_GBs": abytes / kern_s / 1e9,
         }
 
+    global TILES
+    TILES = plans["heter_f32"].plan.tiles
+    # the tiled path must agree with the plain CSR path
+    for kind in ("homo", "heter_f32"):
+        call_csr(kind, HEADLINE[1])
+        ref = out.clone()
+        call_tiled(kind, HEADLINE[1])
+        torch.cuda.synchronize()
+        err = float(((out - ref).abs() / ref.abs().clamp_min(1.0)).max())
+        if err > 1e-5:
+            raise RuntimeError(f"tiled path disagrees with the CSR path ({kind}: {err})")
     sweep = []
     if not args.no_sweep:
-        for kind in ("homo", "heter_f32", "heter_bf16"):
-            for r in RATES:
-                res = measure(kind, r, max(20, args.steps // 4), max(5, args.warmup // 4))
-                sweep.append(res)
-                print(f"# {kind:10s} rate={r:<6} {res['ms_per_step']*1e3:8.1f} us/call "
-                      f"kernel {res['kernel_us']:8.1f} us {res['kernel_GBs']:8.1f} GB/s "
-                      f"{res['events_per_s']:.3e} ev/s", file=sys.stderr, flush=True)
-    head = measure(HEADLINE[0], HEADLINE[1], args.steps, args.warmup)
+        for path in ("csr", "tiled"):
+            for kind in ("homo", "heter_f32", "heter_bf16"):
+                for r in RATES:
+                    res = measure(kind, r, max(20, args.steps // 4), max(5, args.warmup // 4), path)
+                    sweep.append(res)
+                    print(f"# {path:5s} {kind:10s} rate={r:<6} {res['ms_per_step']*1e3:8.1f} us/call "
+                          f"kernel {res['kernel_us']:8.1f} us {res['kernel_GBs']:8.1f} GB/s "
+                          f"{res['events_per_s']:.3e} ev/s", file=sys.stderr, flush=True)
+    head = measure(HEADLINE[0], HEADLINE[1], args.steps, args.warmup, "tiled")
+    plain = measure(HEADLINE[0], HEADLINE[1], args.steps, args.warmup, "csr")
     print(f"# headline {head}", file=sys.stderr, flush=True)
 
     # CPU baseline: the oracle's serial scatter (the reference CPU loop
@@ -152,7 +187,7 @@ def main():
             if el >= args.cpu_seconds or calls >= 2000:
                 break
         # the GPU result of the same call agrees with the CPU one
-        call(*HEADLINE)
+        call_tiled(*HEADLINE)
         torch.cuda.synchronize()
         rel = float(np.max(np.abs(out.cpu().numpy() - h_out) / np.maximum(np.abs(h_out), 1.0)))
         cpu = {"value": head["events_per_call"] * calls / el, "unit": "synaptic events/s", "cores": 1,
@@ -177,11 +212,15 @@ def main():
         "data": "synthetic (seeded FixedProb CSR and Bernoulli spikes, generated on the GPU)",
         "config": {"workload": "event_csrmv_transpose_100kx100k_p0.01_rate0.10_heter_f32", "m": M, "n": N,
                    "nnz": nnz, "spike_rate": HEADLINE[1], "weights": HEADLINE[0], "index_dtype": "int32",
+                   "path": "tiled CSR plan (built once before timing)",
                    "active_rows": head["active_rows"], "events_per_step": head["events_per_call"]},
         "roofline": {"bound": "hbm", "achieved": head["kernel_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": head["kernel_GBs"] / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_csr_scatter (event, heter f32)", "kernel_us": head["kernel_us"],
+                     "kernel": "k_tiled_scatter (heter f32, bool events)", "kernel_us": head["kernel_us"],
                      "algorithmic_bytes_per_launch": head["algorithmic_bytes"]},
+        "plain_csr_path": {"events_per_s": plain["events_per_s"], "ms_per_step": plain["ms_per_step"],
+                           "kernel_us": plain["kernel_us"], "kernel_GBs": plain["kernel_GBs"],
+                           "kernel": "k_csr_scatter (stateless bm.event.csrmv)"},
         "cpu_baseline": cpu,
         "sweep": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()} for s in sweep],
     }

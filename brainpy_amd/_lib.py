@@ -26,6 +26,17 @@ _TORCH2BPX = {
 
 _lib = None
 
+
+class TiledCSRPlan(ctypes.Structure):
+    """Mirror of bpx_tiled_csr (include/bpx.h)."""
+    _fields_ = [("m", ctypes.c_int64), ("n", ctypes.c_int64), ("nnz", ctypes.c_int64), ("tiles", ctypes.c_int32),
+                ("tile_len", ctypes.c_int32), ("weight_dtype", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("rowtile_offset", ctypes.c_int64), ("lidx_offset", ctypes.c_int64),
+                ("weights_offset", ctypes.c_int64), ("homo_offset", ctypes.c_int64), ("mem", ctypes.c_void_p)]
+
+
+TILED_HOMO = -1
+
 # function name -> (argtypes, restype)
 _vp, _i64, _i32, _u32, _f32, _sz = (ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_uint32,
                                      ctypes.c_float, ctypes.c_size_t)
@@ -44,6 +55,9 @@ SIGNATURES = {
     "bpx_lif_coba_step": ([_vp, _i64, _i32, _i32, _vp, _vp, _vp, ctypes.POINTER(_vp), _vp, _vp, _vp, _f32, _vp,
                            _f32, _f32, _f32, _f32, _f32, _f32, _f32, _f32], _i32),
     "bpx_copy": ([_vp, _vp, _vp, _i64], _i32),
+    "bpx_tiled_csr_layout": ([_i64, _i64, _i64, _i32, ctypes.POINTER(TiledCSRPlan), ctypes.POINTER(_sz)], _i32),
+    "bpx_tiled_csr_build": ([_vp, _vp, _i32, _i64, _vp, _vp, _i32, ctypes.POINTER(TiledCSRPlan), _vp, _sz], _i32),
+    "bpx_event_csrmv_tiled": ([_vp, ctypes.POINTER(TiledCSRPlan), _vp, _i32, _i32, _i32, _vp], _i32),
     "bpx_expon_coef": ([_f32, _f32], _f32),
     "bpx_timer_enable": ([_i32], _i32),
     "bpx_timer_read": ([_i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)], _i32),
@@ -141,4 +155,4 @@ def timer_read(kernel_id):
 
 
 KID = dict(event_scatter=0, event_gather=1, csr_scatter=2, csr_gather=3, jit_outdim=4, jit_scatter=5, info=6,
-           lif=7, fixed_order=8, copy=9)
+           lif=7, fixed_order=8, copy=9, net=10, tiled=11)

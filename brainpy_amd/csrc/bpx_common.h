@@ -91,6 +91,7 @@ enum KernelId {
   KID_FIXED_ORDER = 8,
   KID_COPY = 9,
   KID_NET = 10,
+  KID_TILED = 11,
   KID_COUNT = 16
 };
 

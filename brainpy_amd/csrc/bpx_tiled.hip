@@ -1,0 +1,356 @@
+// bpx_tiled.hip -- XCD/LDS-tiled event scatter ("tiled CSR plan").
+//
+// The transpose event csrmv is bound by global FP32 atomics when every
+// synaptic event becomes one atomic on a random output address: MI355X
+// executes ~20 G 64-byte atomic segments per second, so 1e7 events cost
+// ~0.5 ms whatever their bytes.  The tiled layout removes them:
+//
+//   * the output [n] is cut into P tiles of T floats, P = CUs per XCD, and
+//     each of the 8 XCDs holds one full copy of the output spread over the
+//     LDS of its CUs (block b runs on XCD b % 8; its tile is (b / 8) % P);
+//   * the rows are dealt to the 8 copies in 64-row chunks (chunk q -> copy
+//     q % 8); inside a block every thread walks rows of its copy, and for
+//     each active row reads only that row's tile-p segment (row-tile offsets
+//     precomputed), accumulating with LDS atomics at 16-bit tile-local
+//     column indices;
+//   * at the end each block adds its non-zero LDS entries into the output
+//     with coalesced global atomics: 8 copies x n/16 segments per call
+//     instead of one segment per synaptic event.
+//
+// Every synapse is read exactly once per call (2 B index + weight bytes);
+// events are read once from HBM and re-read from L2 by the P blocks of a
+// copy.  The layout is built once per matrix (bpx_tiled_csr_build), like a
+// sparse-library preprocess; the result equals bpx_event_csrmv transpose=1.
+#include "bpx_common.h"
+#include "bpx_rng.h"
+
+namespace bpx {
+
+constexpr int kTiledThreads = 1024;
+constexpr int kBuildThreads = 128;
+constexpr int kMaxTiles = 64;
+
+static int device_cus() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) return 256;
+  return cus;
+}
+
+// ---------------------------------------------------------------------------
+// build: per row, count elements per tile -> row-tile offsets; then place
+// tile-local indices (u16) and weights in tile-grouped order (stable)
+// ---------------------------------------------------------------------------
+template <typename I>
+__global__ __launch_bounds__(kBuildThreads) void k_tiled_count(const I* __restrict__ indices,
+                                                               const I* __restrict__ indptr, int64_t nnz, int64_t m,
+                                                               int64_t n, int P, DivU32 dT, int32_t* __restrict__ rowtile) {
+  __shared__ int32_t s_cnt[kBuildThreads * kMaxTiles];
+  int32_t* cnt = s_cnt + threadIdx.x * kMaxTiles;
+  const int64_t r = (int64_t)blockIdx.x * kBuildThreads + threadIdx.x;
+  if (r >= m) return;
+  for (int p = 0; p < P; ++p) cnt[p] = 0;
+  int64_t s = (int64_t)indptr[r], t = (int64_t)indptr[r + 1];
+  s = s < 0 ? 0 : s;
+  t = t > nnz ? nnz : t;
+  for (int64_t j = s; j < t; ++j) {
+    const int64_t c = (int64_t)indices[j];
+    if ((uint64_t)c < (uint64_t)n) cnt[dT.div((uint32_t)c)] += 1;
+  }
+  int32_t* rt = rowtile + r * (P + 1);
+  int32_t acc = (int32_t)(s > t ? t : s);
+  for (int p = 0; p < P; ++p) {
+    rt[p] = acc;
+    acc += cnt[p];
+  }
+  rt[P] = acc;
+}
+
+template <typename I, typename V, typename W>
+__global__ __launch_bounds__(kBuildThreads) void k_tiled_fill(const V* __restrict__ data, int heter,
+                                                              const I* __restrict__ indices,
+                                                              const I* __restrict__ indptr, int64_t nnz, int64_t m,
+                                                              int64_t n, int P, DivU32 dT,
+                                                              const int32_t* __restrict__ rowtile,
+                                                              uint16_t* __restrict__ lidx, W* __restrict__ w) {
+  __shared__ int32_t s_cur[kBuildThreads * kMaxTiles];
+  int32_t* cur = s_cur + threadIdx.x * kMaxTiles;
+  const int64_t r = (int64_t)blockIdx.x * kBuildThreads + threadIdx.x;
+  if (r >= m) return;
+  const int32_t* rt = rowtile + r * (P + 1);
+  for (int p = 0; p < P; ++p) cur[p] = rt[p];
+  int64_t s = (int64_t)indptr[r], t = (int64_t)indptr[r + 1];
+  s = s < 0 ? 0 : s;
+  t = t > nnz ? nnz : t;
+  for (int64_t j = s; j < t; ++j) {
+    const int64_t c = (int64_t)indices[j];
+    if ((uint64_t)c >= (uint64_t)n) continue;
+    const uint32_t p = dT.div((uint32_t)c);
+    const int32_t pos = cur[p]++;
+    lidx[pos] = (uint16_t)((uint32_t)c - p * dT.d);
+    if (heter) {
+      const float v = to_f32(data[j]);
+      if constexpr (sizeof(W) == 2) {
+        // round-to-nearest-even f32 -> bf16
+        uint32_t u = __float_as_uint(v);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        w[pos].bits = (uint16_t)(u >> 16);
+      } else {
+        w[pos] = v;
+      }
+    }
+  }
+}
+
+__global__ void k_store_scalar(const void* src, int src_dtype, float* dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    float v;
+    if (src_dtype == BPX_F64) v = (float)*(const double*)src;
+    else if (src_dtype == BPX_BF16) v = to_f32(*(const bf16_t*)src);
+    else v = *(const float*)src;
+    *dst = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exec
+// ---------------------------------------------------------------------------
+template <int WK /*0 homo, 1 f32, 2 bf16*/, typename E>
+__global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* __restrict__ rowtile,
+                                                                 const uint16_t* __restrict__ lidx,
+                                                                 const void* __restrict__ wptr,
+                                                                 const float* __restrict__ homo_w,
+                                                                 const E* __restrict__ ev, int rule, int64_t m,
+                                                                 int64_t n, int P, int T, float* __restrict__ out) {
+  extern __shared__ float s_tile[];
+  const int b = blockIdx.x;
+  const int x = b & 7;                 // XCD of this block (round-robin dispatch)
+  const int p = (b >> 3) % P;          // output tile
+  const int copy = x;                  // one output copy per XCD
+  const int64_t t0 = (int64_t)p * T;
+  const int tlen = (int)((n - t0) < (int64_t)T ? (n - t0) : (int64_t)T);
+  for (int k = threadIdx.x; k < T; k += kTiledThreads) s_tile[k] = 0.f;
+  __syncthreads();
+  float hw = 0.f;
+  if constexpr (WK == 0) hw = *homo_w;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t nchunks = (m + 63) >> 6;
+  // chunks of this copy: q = copy + 8*i; the block's 16 waves take them in turn
+  for (int64_t q = copy + 8 * (int64_t)wave; q < nchunks; q += 8 * (kTiledThreads / 64)) {
+    const int64_t r = (q << 6) + lane;
+    if (r >= m) continue;
+    if (!ev_true(ev[r], rule)) continue;
+    const int32_t* rt = rowtile + r * (P + 1) + p;
+    int32_t j = rt[0];
+    const int32_t e = rt[1];
+    for (; j + 3 < e; j += 4) {
+      uint16_t li[4];
+      float wv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        li[u] = lidx[j + u];
+        if constexpr (WK == 1) wv[u] = ((const float*)wptr)[j + u];
+        else if constexpr (WK == 2) wv[u] = to_f32(((const bf16_t*)wptr)[j + u]);
+        else wv[u] = hw;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (li[u] < tlen) atomicAdd(s_tile + li[u], wv[u]);
+    }
+    for (; j < e; ++j) {
+      const uint16_t li = lidx[j];
+      float wv;
+      if constexpr (WK == 1) wv = ((const float*)wptr)[j];
+      else if constexpr (WK == 2) wv = to_f32(((const bf16_t*)wptr)[j]);
+      else wv = hw;
+      if (li < tlen) atomicAdd(s_tile + li, wv);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < tlen; k += kTiledThreads) {
+    const float v = s_tile[k];
+    if (v != 0.f) atomic_add(out + t0 + k, v);
+  }
+}
+
+template <typename O>
+__global__ __launch_bounds__(256) void k_fill_zero_t(O* __restrict__ out, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = O(0);
+}
+
+struct TiledShape {
+  int P, T;
+};
+
+// largest tile the device's per-block LDS holds (floats), capped by the
+// 16-bit tile-local index
+static int max_tile_len() {
+  int dev = 0, shm = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 16384;
+  if (hipDeviceGetAttribute(&shm, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || shm <= 0)
+    return 16384;
+  int t = shm / (int)sizeof(float);
+  return t < BPX_TILED_MAX_TILE ? t : BPX_TILED_MAX_TILE;
+}
+
+static TiledShape tiled_shape(int64_t n) {
+  int cus = device_cus();
+  int P = cus / 8;
+  if (P < 1) P = 1;
+  if (P > kMaxTiles) P = kMaxTiles;
+  int64_t T = (n + P - 1) / P;
+  if (T < 1) T = 1;
+  return {P, (int)T};
+}
+
+}  // namespace bpx
+
+using namespace bpx;
+
+extern "C" {
+
+int bpx_tiled_csr_layout(int64_t m, int64_t n, int64_t nnz, int weight_dtype, bpx_tiled_csr* plan,
+                         size_t* bytes) {
+  if (!plan || !bytes) return BPX_ERR_ARG;
+  if (m < 0 || n < 1 || nnz < 0 || nnz > INT32_MAX) return BPX_ERR_SHAPE;
+  if (weight_dtype != BPX_F32 && weight_dtype != BPX_BF16 && weight_dtype != BPX_TILED_HOMO) return BPX_ERR_DTYPE;
+  TiledShape ts = tiled_shape(n);
+  if (ts.T > max_tile_len()) return BPX_ERR_SHAPE;
+  plan->m = m;
+  plan->n = n;
+  plan->nnz = nnz;
+  plan->tiles = ts.P;
+  plan->tile_len = ts.T;
+  plan->weight_dtype = weight_dtype;
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    size_t o = off;
+    off += (b + 255) & ~(size_t)255;
+    return o;
+  };
+  size_t o_rt = take((size_t)m * (ts.P + 1) * sizeof(int32_t));
+  size_t o_li = take((size_t)nnz * sizeof(uint16_t));
+  size_t wb = weight_dtype == BPX_F32 ? 4 : weight_dtype == BPX_BF16 ? 2 : 0;
+  size_t o_w = take((size_t)nnz * wb);
+  size_t o_h = take(sizeof(float));
+  plan->rowtile_offset = (int64_t)o_rt;
+  plan->lidx_offset = (int64_t)o_li;
+  plan->weights_offset = (int64_t)o_w;
+  plan->homo_offset = (int64_t)o_h;
+  *bytes = off;
+  return BPX_OK;
+}
+
+int bpx_tiled_csr_build(void* stream, const void* data, int data_dtype, int64_t data_len, const void* indices,
+                        const void* indptr, int index_dtype, bpx_tiled_csr* plan, void* mem, size_t mem_bytes) {
+  if (!plan || !mem) return BPX_ERR_ARG;
+  size_t need = 0;
+  bpx_tiled_csr chk = *plan;
+  int rc = bpx_tiled_csr_layout(plan->m, plan->n, plan->nnz, plan->weight_dtype, &chk, &need);
+  if (rc != BPX_OK) return rc;
+  if (mem_bytes < need) return BPX_ERR_WORKSPACE;
+  *plan = chk;
+  plan->mem = mem;
+  const int64_t m = plan->m, n = plan->n, nnz = plan->nnz;
+  const bool homo = plan->weight_dtype == BPX_TILED_HOMO;
+  if (homo ? data_len != 1 : data_len != nnz) return BPX_ERR_SHAPE;
+  if (data_dtype != BPX_F32 && data_dtype != BPX_F64 && data_dtype != BPX_BF16) return BPX_ERR_DTYPE;
+  hipStream_t s = (hipStream_t)stream;
+  char* base = (char*)mem;
+  int32_t* rowtile = (int32_t*)(base + plan->rowtile_offset);
+  uint16_t* lidx = (uint16_t*)(base + plan->lidx_offset);
+  void* w = base + plan->weights_offset;
+  float* hw = (float*)(base + plan->homo_offset);
+  const DivU32 dT = DivU32::make((uint32_t)plan->tile_len);
+  const int P = plan->tiles;
+  const unsigned blocks = (unsigned)((m + kBuildThreads - 1) / kBuildThreads);
+  if (homo) hipLaunchKernelGGL(k_store_scalar, dim3(1), dim3(64), 0, s, data, data_dtype, hw);
+  if (m == 0) {
+    BPX_CHECK_LAUNCH();
+    return BPX_OK;
+  }
+#define BUILD_IDX(I)                                                                                           \
+  do {                                                                                                         \
+    hipLaunchKernelGGL(k_tiled_count<I>, dim3(blocks), dim3(kBuildThreads), 0, s, (const I*)indices,          \
+                       (const I*)indptr, nnz, m, n, P, dT, rowtile);                                           \
+    if (homo || plan->weight_dtype == BPX_F32) {                                                               \
+      if (data_dtype == BPX_F64)                                                                               \
+        hipLaunchKernelGGL((k_tiled_fill<I, double, float>), dim3(blocks), dim3(kBuildThreads), 0, s,          \
+                           (const double*)data, homo ? 0 : 1, (const I*)indices, (const I*)indptr, nnz, m, n, P, \
+                           dT, rowtile, lidx, (float*)w);                                                      \
+      else if (data_dtype == BPX_BF16)                                                                         \
+        hipLaunchKernelGGL((k_tiled_fill<I, bf16_t, float>), dim3(blocks), dim3(kBuildThreads), 0, s,          \
+                           (const bf16_t*)data, homo ? 0 : 1, (const I*)indices, (const I*)indptr, nnz, m, n, P, \
+                           dT, rowtile, lidx, (float*)w);                                                      \
+      else                                                                                                     \
+        hipLaunchKernelGGL((k_tiled_fill<I, float, float>), dim3(blocks), dim3(kBuildThreads), 0, s,           \
+                           (const float*)data, homo ? 0 : 1, (const I*)indices, (const I*)indptr, nnz, m, n, P, \
+                           dT, rowtile, lidx, (float*)w);                                                      \
+    } else {                                                                                                   \
+      if (data_dtype == BPX_F64)                                                                               \
+        hipLaunchKernelGGL((k_tiled_fill<I, double, bf16_t>), dim3(blocks), dim3(kBuildThreads), 0, s,         \
+                           (const double*)data, 1, (const I*)indices, (const I*)indptr, nnz, m, n, P, dT,       \
+                           rowtile, lidx, (bf16_t*)w);                                                          \
+      else if (data_dtype == BPX_BF16)                                                                         \
+        hipLaunchKernelGGL((k_tiled_fill<I, bf16_t, bf16_t>), dim3(blocks), dim3(kBuildThreads), 0, s,         \
+                           (const bf16_t*)data, 1, (const I*)indices, (const I*)indptr, nnz, m, n, P, dT,       \
+                           rowtile, lidx, (bf16_t*)w);                                                          \
+      else                                                                                                     \
+        hipLaunchKernelGGL((k_tiled_fill<I, float, bf16_t>), dim3(blocks), dim3(kBuildThreads), 0, s,          \
+                           (const float*)data, 1, (const I*)indices, (const I*)indptr, nnz, m, n, P, dT,        \
+                           rowtile, lidx, (bf16_t*)w);                                                          \
+    }                                                                                                          \
+  } while (0)
+  if (index_dtype == BPX_I32) BUILD_IDX(int32_t);
+  else if (index_dtype == BPX_I64) BUILD_IDX(int64_t);
+  else return BPX_ERR_DTYPE;
+#undef BUILD_IDX
+  BPX_CHECK_LAUNCH();
+  return BPX_OK;
+}
+
+int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* events, int ev_dtype, int ev_rule,
+                          int accumulate, float* out) {
+  if (!plan || !plan->mem) return BPX_ERR_ARG;
+  if (ev_rule != BPX_RULE_POSITIVE && ev_rule != BPX_RULE_NONZERO) return BPX_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t m = plan->m, n = plan->n;
+  if (!accumulate) {
+    int64_t zb = (n + 255) / 256;
+    if (zb > 4096) zb = 4096;
+    hipLaunchKernelGGL(k_fill_zero_t<float>, dim3((unsigned)zb), dim3(256), 0, s, out, n);
+  }
+  if (m == 0) {
+    BPX_CHECK_LAUNCH();
+    return BPX_OK;
+  }
+  const char* base = (const char*)plan->mem;
+  const int32_t* rowtile = (const int32_t*)(base + plan->rowtile_offset);
+  const uint16_t* lidx = (const uint16_t*)(base + plan->lidx_offset);
+  const void* w = base + plan->weights_offset;
+  const float* hw = (const float*)(base + plan->homo_offset);
+  const int P = plan->tiles, T = plan->tile_len;
+  const unsigned blocks = (unsigned)(8 * P);
+  const size_t shm = (size_t)T * sizeof(float);
+#define TL(WK, E)                                                                                             \
+  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E>), dim3(blocks), dim3(kTiledThreads), shm, s, rowtile, lidx, w, hw, \
+             (const E*)events, ev_rule, m, n, P, T, out)
+#define TL_E(WK)                                     \
+  do {                                               \
+    if (ev_dtype == BPX_BOOL) TL(WK, uint8_t);       \
+    else if (ev_dtype == BPX_F32) TL(WK, float);     \
+    else return BPX_ERR_DTYPE;                       \
+  } while (0)
+  if (plan->weight_dtype == BPX_TILED_HOMO) TL_E(0);
+  else if (plan->weight_dtype == BPX_F32) TL_E(1);
+  else TL_E(2);
+#undef TL_E
+#undef TL
+  BPX_CHECK_LAUNCH();
+  return BPX_OK;
+}
+
+}  // extern "C"

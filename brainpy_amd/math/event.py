@@ -7,6 +7,7 @@ Signatures, validation and error messages follow
 ``event/_info_collection.py:24-43``.  Inputs may be numpy arrays, python
 scalars or torch tensors; results are torch tensors on the GPU.
 """
+import ctypes
 from typing import Tuple
 
 import numpy as np
@@ -15,7 +16,7 @@ import torch
 from .. import _lib
 from ._common import acc_dtype, data_tensor, index_tensor, ndim
 
-__all__ = ["csrmv", "csrmv_taichi", "info", "csrmv_fixed_order", "csr_to_csc", "csrmv_add"]
+__all__ = ["csrmv", "csrmv_taichi", "info", "csrmv_fixed_order", "csr_to_csc", "csrmv_add", "TiledCSR"]
 
 
 def _check(data, indices, indptr, events, shape, transpose, allowed_index, index_msg):
@@ -207,3 +208,67 @@ def info(events):
     if batched:
         return ids, num
     return ids.reshape(n), num.reshape(1)
+
+
+class TiledCSR:
+    """A CSR matrix re-laid out once for the event-driven scatter
+    (bpx_tiled_csr_build): output tiles in the LDS of every CU, one copy per
+    XCD, 16-bit tile-local column indices.  ``plan.csrmv(events)`` equals
+    ``csrmv(data, indices, indptr, events, shape=shape, transpose=True)``
+    but replaces one global atomic per synaptic event by LDS accumulation and
+    one coalesced flush per tile.  ``weight_dtype`` may be ``torch.float32``
+    or ``torch.bfloat16`` (heterogeneous weights stored at half the bytes and
+    accumulated in f32); homogeneous ``data`` (size 1) stores no weights."""
+
+    def __init__(self, data, indices, indptr, *, shape: Tuple[int, int], weight_dtype=None):
+        d = data_tensor(_as_array(data))
+        idx = index_tensor(_as_array(indices))
+        ptr = index_tensor(_as_array(indptr))
+        if ptr.dtype != idx.dtype:
+            ptr = ptr.to(idx.dtype)
+        m, n = int(shape[0]), int(shape[1])
+        if ptr.shape[0] != m + 1:
+            raise ValueError(f'indptr should have shape ({m + 1},), but we got {tuple(ptr.shape)}.')
+        if d.shape[0] not in (1, idx.shape[0]):
+            raise ValueError('The size of data should be 1 or be consistent with indices.')
+        if d.dtype == torch.float16:
+            d = d.to(torch.float32)
+        homo = d.shape[0] == 1
+        if homo:
+            wcode = _lib.TILED_HOMO
+        elif weight_dtype in (None, torch.float32):
+            wcode = _lib.BPX_F32
+        elif weight_dtype == torch.bfloat16:
+            wcode = _lib.BPX_BF16
+        else:
+            raise TypeError(f'weight_dtype must be float32 or bfloat16, got {weight_dtype}')
+        L = _lib.lib()
+        self.plan = _lib.TiledCSRPlan()
+        nbytes = ctypes.c_size_t()
+        _lib.check(L.bpx_tiled_csr_layout(m, n, idx.shape[0], wcode, ctypes.byref(self.plan), ctypes.byref(nbytes)),
+                   "bpx_tiled_csr_layout")
+        self.mem = torch.empty(int(nbytes.value), dtype=torch.uint8, device=d.device)
+        _lib.check(L.bpx_tiled_csr_build(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), d.shape[0], _lib.ptr(idx),
+                                         _lib.ptr(ptr), _lib.bpx_dtype(idx), ctypes.byref(self.plan),
+                                         _lib.ptr(self.mem), self.mem.numel()), "bpx_tiled_csr_build")
+        self.shape = (m, n)
+        self.weight_dtype = wcode
+
+    def csrmv(self, events, *, out: torch.Tensor = None, rule: int = _lib.RULE_POSITIVE) -> torch.Tensor:
+        """``events @ A`` (a new vector), or ``out += events @ A`` when
+        ``out`` is given."""
+        ev = _events_tensor(events)
+        if ev.dtype == torch.float64:
+            ev = ev.to(torch.float32)
+        m, n = self.shape
+        if ev.ndim != 1 or ev.shape[0] != m:
+            raise ValueError(f'Shape mismatch, vec ({ev.shape[0]},) @ mat {self.shape}.')
+        accumulate = out is not None
+        if out is None:
+            out = torch.empty(n, dtype=torch.float32, device=ev.device)
+        elif out.dtype != torch.float32 or out.shape != (n,) or not out.is_contiguous():
+            raise ValueError(f'out must be a contiguous float32 vector of shape ({n},).')
+        rc = _lib.lib().bpx_event_csrmv_tiled(_lib.stream_ptr(), ctypes.byref(self.plan), _lib.ptr(ev),
+                                              _lib.bpx_dtype(ev), rule, int(accumulate), _lib.ptr(out))
+        _lib.check(rc, "bpx_event_csrmv_tiled")
+        return out

@@ -123,6 +123,42 @@ int bpx_event_csrmv_add(void* stream, const void* data, int data_dtype, int64_t 
                         const void* events, int ev_dtype, int ev_rule, int64_t m, int64_t n, void* out);
 
 /*
+ * Tiled CSR plan for the event-driven scatter (transpose event csrmv).
+ * A one-time re-layout of a CSR matrix (like a sparse-library SpMV
+ * preprocess) after which bpx_event_csrmv_tiled computes exactly
+ * bpx_event_csrmv(transpose=1) with LDS accumulation instead of one global
+ * atomic per synaptic event: the output is cut into one tile per CU of an
+ * XCD (tile_len floats, 16-bit tile-local column indices), each XCD holds
+ * one copy of the output in the LDS of its CUs, and rows carry per-tile
+ * segment offsets.  No reference counterpart (the reference re-reads plain
+ * CSR each call, event/_csr_matvec.py:453-500); it serves the same operator.
+ * Usage: bpx_tiled_csr_layout (sizes, fills the plan's shape fields) ->
+ * caller allocates `bytes` of device memory -> bpx_tiled_csr_build ->
+ * bpx_event_csrmv_tiled per call.  Requires nnz < 2^31 and tile_len no
+ * larger than min(32768, per-block LDS / 4) (n <= that * CUs-per-XCD).  Weights: f32, bf16 (rounded to nearest even
+ * at build, accumulated in f32) or homogeneous (data_len 1).
+ */
+#define BPX_TILED_HOMO (-1)
+#define BPX_TILED_MAX_TILE 32768
+typedef struct {
+  int64_t m, n, nnz;
+  int32_t tiles;        /* P: output tiles per copy (= CUs per XCD) */
+  int32_t tile_len;     /* T: floats per tile */
+  int32_t weight_dtype; /* BPX_F32, BPX_BF16 or BPX_TILED_HOMO */
+  int32_t reserved;
+  int64_t rowtile_offset, lidx_offset, weights_offset, homo_offset; /* byte offsets in mem */
+  void* mem;            /* device memory holding the plan (set by build) */
+} bpx_tiled_csr;
+
+int bpx_tiled_csr_layout(int64_t m, int64_t n, int64_t nnz, int weight_dtype, bpx_tiled_csr* plan, size_t* bytes);
+int bpx_tiled_csr_build(void* stream, const void* data, int data_dtype, int64_t data_len,
+                        const void* indices, const void* indptr, int index_dtype,
+                        bpx_tiled_csr* plan, void* mem, size_t mem_bytes);
+/* out[n] = events @ A (accumulate=0) or out += events @ A (accumulate=1) */
+int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* events, int ev_dtype,
+                          int ev_rule, int accumulate, float* out);
+
+/*
  * Just-in-time connectivity matvec y = M @ v, M never stored.  Replaces the
  * Taichi kernels of jitconn/_event_matvec_taichi.py:38-1106 (event) and
  * jitconn/_matvec_taichi.py:30-746 (dense vector), and stands in for the

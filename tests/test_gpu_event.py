@@ -185,3 +185,48 @@ def test_sparse_converters():
     idx2, ptr2 = bm.sparse.coo_to_csr(r, c, num_row=40)
     np.testing.assert_array_equal(idx2.cpu().numpy(), indices)
     np.testing.assert_array_equal(ptr2.cpu().numpy(), indptr)
+
+
+@pytest.mark.parametrize("shape", [(100, 200), (1000, 3000), (3000, 50), (257, 129), (5000, 100000)])
+@pytest.mark.parametrize("wkind", ["homo", "f32", "bf16"])
+@pytest.mark.parametrize("sort", [True, False])
+def test_tiled_plan_vs_oracle(shape, wkind, sort):
+    m, n = shape
+    p = min(0.5, 50.0 / n)
+    indices, indptr = bm_connect_csr(m, n, p, 13, sort)
+    rng = np.random.default_rng(7)
+    if wkind == "homo":
+        data = np.float32(1.0)
+    else:
+        data = rng.normal(0.5, 0.1, indices.size).astype(np.float32)
+    plan = bm.event.TiledCSR(data, indices, indptr, shape=shape,
+                             weight_dtype=torch.bfloat16 if wkind == "bf16" else None)
+    for rate in (0.0, 0.05, 0.5, 1.0):
+        ev = bernoulli(m, rate, int(rate * 100) + m)
+        y = plan.csrmv(ev).cpu().numpy()
+        if wkind == "homo":
+            np.testing.assert_array_equal(y, orc.event_csrmv(data, indices, indptr, ev, shape, True))
+        else:
+            w = bf16_bits_to_f32(f32_to_bf16_bits(data)) if wkind == "bf16" else data
+            D = dense_of(w, indices, indptr, shape)
+            ref = ev.astype(np.float64) @ D
+            scale = np.abs(D).sum(axis=0).max() if D.size else 1.0
+            _tol_check(y, ref, scale)
+
+
+def test_tiled_plan_accumulate_and_float_events():
+    m, n = 2000, 7000
+    indices, indptr = bm_connect_csr(m, n, 0.01, 21, False)
+    plan = bm.event.TiledCSR(np.float32(0.5), indices, indptr, shape=(m, n))
+    ev = np.zeros(m, np.float32)
+    ev[::4] = 1.0
+    ev[1::4] = -2.0
+    base = torch.full((n,), 3.0, dtype=torch.float32, device="cuda")
+    plan.csrmv(ev, out=base)
+    ref = np.float32(3.0) + orc.event_csrmv(np.float32(0.5), indices, indptr, ev, (m, n), True)
+    np.testing.assert_array_equal(base.cpu().numpy(), ref)
+
+
+def bm_connect_csr(m, n, p, seed, sort):
+    from brainpy_amd.connect import fixed_prob_csr as fp
+    return fp(m, n, p, seed, sort=sort)
