@@ -5,16 +5,17 @@
 // executes ~20 G 64-byte atomic segments per second, so 1e7 events cost
 // ~0.5 ms whatever their bytes.  The tiled layout removes them:
 //
-//   * the output [n] is cut into P tiles of T floats, P = CUs per XCD, and
-//     each of the 8 XCDs holds one full copy of the output spread over the
-//     LDS of its CUs (block b runs on XCD b % 8; its tile is (b / 8) % P);
-//   * the rows are dealt to the 8 copies in 64-row chunks (chunk q -> copy
-//     q % 8); inside a block every thread walks rows of its copy, and for
-//     each active row reads only that row's tile-p segment (row-tile offsets
-//     precomputed), accumulating with LDS atomics at 16-bit tile-local
-//     column indices;
+//   * the output [n] is cut into P tiles of T entries, P = CUs per XCD, and
+//     each XCD holds C full copies of the output (C = 2) spread over the LDS
+//     of its CUs (block b runs on XCD b % 8; its tile is (b / 8) % P);
+//   * the rows are dealt to the 8C copies in 64-row chunks (chunk q -> copy
+//     q % 8C); inside a block each wave walks chunks of its copy, lists the
+//     active rows with their tile-p segment (row-tile offsets precomputed),
+//     and groups of 16 lanes stream the segments, accumulating at 16-bit
+//     tile-local column indices with LDS integer atomics in 64-bit fixed
+//     point (gfx950's LDS float atomics run ~20x slower than integer ones);
 //   * at the end each block adds its non-zero LDS entries into the output
-//     with coalesced global atomics: 8 copies x n/16 segments per call
+//     with coalesced global atomics: 8C copies x n/16 segments per call
 //     instead of one segment per synaptic event.
 //
 // Every synapse is read exactly once per call (2 B index + weight bytes);
@@ -23,6 +24,10 @@
 // sparse-library preprocess; the result equals bpx_event_csrmv transpose=1.
 #include "bpx_common.h"
 #include "bpx_rng.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
 
 namespace bpx {
 
@@ -102,6 +107,24 @@ __global__ __launch_bounds__(kBuildThreads) void k_tiled_fill(const V* __restric
   }
 }
 
+template <typename V>
+__global__ __launch_bounds__(256) void k_absmax(const V* __restrict__ data, int64_t len, uint32_t* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  float mx = 0.f;
+  for (; i < len; i += stride) {
+    const float v = fabsf((float)to_f32(data[i]));
+    mx = (v > mx || v != v) ? v : mx;    // NaN propagates
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const float y = __shfl_xor(mx, o, 64);
+    mx = (y > mx || y != y) ? y : mx;
+  }
+  // non-negative floats (and NaN/inf) order like their bit patterns
+  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(mx));
+}
+
 __global__ void k_store_scalar(const void* src, int src_dtype, float* dst) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     float v;
@@ -114,63 +137,125 @@ __global__ void k_store_scalar(const void* src, int src_dtype, float* dst) {
 
 // ---------------------------------------------------------------------------
 // exec
+//
+// Each wave walks its chunks of 64 rows four at a time: the next four
+// chunks' events are loaded before the current ones are processed.  Active
+// rows of a batch (ballot + mbcnt) are listed in LDS together with their
+// tile-p segment [start, end); then 8 groups of 8 lanes take two listed rows
+// each per round and issue 4+4 element loads per lane before their LDS
+// atomics.
 // ---------------------------------------------------------------------------
-template <int WK /*0 homo, 1 f32, 2 bf16*/, typename E>
+constexpr int kChunksPerBatch = 4;
+constexpr int kListCap = 64 * kChunksPerBatch;
+constexpr int kWavesPerBlock = kTiledThreads / 64;
+
+// weight -> 64-bit fixed point (w * 2^shift, exact for |w| >= 2^(23-shift))
+__device__ __forceinline__ long long fixed_of(float w, double scale) { return __double2ll_rn((double)w * scale); }
+
+template <int WK>
+__device__ __forceinline__ float tiled_weight(const void* __restrict__ wptr, int32_t j, float hw) {
+  if constexpr (WK == 1) return ((const float*)wptr)[j];
+  else if constexpr (WK == 2) return to_f32(((const bf16_t*)wptr)[j]);
+  else return hw;
+}
+
+template <int WK, typename E, int G>
 __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* __restrict__ rowtile,
                                                                  const uint16_t* __restrict__ lidx,
                                                                  const void* __restrict__ wptr,
                                                                  const float* __restrict__ homo_w,
                                                                  const E* __restrict__ ev, int rule, int64_t m,
-                                                                 int64_t n, int P, int T, float* __restrict__ out) {
-  extern __shared__ float s_tile[];
+                                                                 int64_t n, int P, int T, int cpx, double fx_scale,
+                                                                 double fx_inv, float* __restrict__ out) {
+  extern __shared__ unsigned long long s_acc[];
+  __shared__ int2 s_seg[kWavesPerBlock][kListCap];
   const int b = blockIdx.x;
   const int x = b & 7;                 // XCD of this block (round-robin dispatch)
-  const int p = (b >> 3) % P;          // output tile
-  const int copy = x;                  // one output copy per XCD
+  const int jb = b >> 3;
+  const int p = jb % P;                // output tile
+  const int copy = x * cpx + jb / P;   // output copy (cpx copies per XCD)
+  const int ncopy = 8 * cpx;
   const int64_t t0 = (int64_t)p * T;
   const int tlen = (int)((n - t0) < (int64_t)T ? (n - t0) : (int64_t)T);
-  for (int k = threadIdx.x; k < T; k += kTiledThreads) s_tile[k] = 0.f;
+  for (int k = threadIdx.x; k < T; k += kTiledThreads) s_acc[k] = 0ull;
   __syncthreads();
   float hw = 0.f;
   if constexpr (WK == 0) hw = *homo_w;
+  const long long hfx = fixed_of(hw, fx_scale);
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
+  constexpr int NGRP = 64 / G;          // lane groups per wave
+  constexpr int U = 32 / G;             // elements per lane and row per pass
+  const int g = lane / G, l8 = lane % G;
+  int2* seg = s_seg[wave];
   const int64_t nchunks = (m + 63) >> 6;
-  // chunks of this copy: q = copy + 8*i; the block's 16 waves take them in turn
-  for (int64_t q = copy + 8 * (int64_t)wave; q < nchunks; q += 8 * (kTiledThreads / 64)) {
-    const int64_t r = (q << 6) + lane;
-    if (r >= m) continue;
-    if (!ev_true(ev[r], rule)) continue;
-    const int32_t* rt = rowtile + r * (P + 1) + p;
-    int32_t j = rt[0];
-    const int32_t e = rt[1];
-    for (; j + 3 < e; j += 4) {
-      uint16_t li[4];
-      float wv[4];
+  const int64_t cstep = (int64_t)ncopy * kWavesPerBlock;    // chunk stride of one wave
+  int64_t q0 = copy + (int64_t)ncopy * wave;                // first chunk of this wave
+
+  auto load_batch = [&](int64_t qb, bool* act) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        li[u] = lidx[j + u];
-        if constexpr (WK == 1) wv[u] = ((const float*)wptr)[j + u];
-        else if constexpr (WK == 2) wv[u] = to_f32(((const bf16_t*)wptr)[j + u]);
-        else wv[u] = hw;
+    for (int u = 0; u < kChunksPerBatch; ++u) {
+      const int64_t q = qb + cstep * u;
+      const int64_t r = (q << 6) + lane;
+      act[u] = q < nchunks && r < m && ev_true(ev[r], rule);
+    }
+  };
+  bool act[kChunksPerBatch];
+  load_batch(q0, act);
+  while (q0 < nchunks) {
+    // list the batch's active rows with their tile-p segments
+    int cnt = 0;
+#pragma unroll
+    for (int u = 0; u < kChunksPerBatch; ++u) {
+      const uint64_t mask = __ballot(act[u]);
+      if (act[u]) {
+        const int64_t r = ((q0 + cstep * u) << 6) + lane;
+        const int32_t* rt = rowtile + r * (P + 1) + p;
+        seg[cnt + lanes_below(mask)] = make_int2(rt[0], rt[1]);
       }
+      cnt += __popcll(mask);
+    }
+    // prefetch the next batch's events
+    const int64_t qn = q0 + cstep * kChunksPerBatch;
+    bool nact[kChunksPerBatch];
+    load_batch(qn, nact);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int e = g; e < cnt; e += 2 * NGRP) {
+      const int2 sa = seg[e];
+      const int2 sb = (e + NGRP < cnt) ? seg[e + NGRP] : make_int2(0, 0);
+      for (int32_t k = l8; sa.x + k < sa.y || sb.x + k < sb.y; k += 32) {
+        uint16_t la[U], lb[U];
+        float wa[U], wb[U];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (li[u] < tlen) atomicAdd(s_tile + li[u], wv[u]);
+        for (int u = 0; u < U; ++u) {
+          const int32_t ja = sa.x + k + G * u, jb2 = sb.x + k + G * u;
+          la[u] = 0xFFFF;
+          lb[u] = 0xFFFF;
+          if (ja < sa.y) { la[u] = lidx[ja]; wa[u] = tiled_weight<WK>(wptr, ja, hw); }
+          if (jb2 < sb.y) { lb[u] = lidx[jb2]; wb[u] = tiled_weight<WK>(wptr, jb2, hw); }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (la[u] < tlen)
+            __hip_atomic_fetch_add(s_acc + la[u], (unsigned long long)(WK == 0 ? hfx : fixed_of(wa[u], fx_scale)),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (lb[u] < tlen)
+            __hip_atomic_fetch_add(s_acc + lb[u], (unsigned long long)(WK == 0 ? hfx : fixed_of(wb[u], fx_scale)),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
     }
-    for (; j < e; ++j) {
-      const uint16_t li = lidx[j];
-      float wv;
-      if constexpr (WK == 1) wv = ((const float*)wptr)[j];
-      else if constexpr (WK == 2) wv = to_f32(((const bf16_t*)wptr)[j]);
-      else wv = hw;
-      if (li < tlen) atomicAdd(s_tile + li, wv);
-    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < kChunksPerBatch; ++u) act[u] = nact[u];
+    q0 = qn;
   }
   __syncthreads();
   for (int k = threadIdx.x; k < tlen; k += kTiledThreads) {
-    const float v = s_tile[k];
-    if (v != 0.f) atomic_add(out + t0 + k, v);
+    const long long a = (long long)s_acc[k];
+    if (a != 0) atomic_add(out + t0 + k, (float)((double)a * fx_inv));
   }
 }
 
@@ -181,20 +266,31 @@ __global__ __launch_bounds__(256) void k_fill_zero_t(O* __restrict__ out, int64_
   for (; i < n; i += stride) out[i] = O(0);
 }
 
+// output copies per XCD (2 by default: two 1024-thread blocks per CU hide
+// the memory latency of the row walk; BPX_TILED_COPIES overrides for tuning)
+static int tiled_copies_per_xcd() {
+  static int c = [] {
+    const char* e = getenv("BPX_TILED_COPIES");
+    int v = e ? atoi(e) : 2;
+    return v >= 1 && v <= 4 ? v : 2;
+  }();
+  return c;
+}
+
+// lanes cooperating on one row segment (16 by default; BPX_TILED_GROUP overrides for tuning)
+static int tiled_group_width() {
+  static int g = [] {
+    const char* e = getenv("BPX_TILED_GROUP");
+    int v = e ? atoi(e) : 16;
+    return (v == 8 || v == 16 || v == 32) ? v : 16;
+  }();
+  return g;
+}
+
 struct TiledShape {
   int P, T;
 };
 
-// largest tile the device's per-block LDS holds (floats), capped by the
-// 16-bit tile-local index
-static int max_tile_len() {
-  int dev = 0, shm = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 16384;
-  if (hipDeviceGetAttribute(&shm, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || shm <= 0)
-    return 16384;
-  int t = shm / (int)sizeof(float);
-  return t < BPX_TILED_MAX_TILE ? t : BPX_TILED_MAX_TILE;
-}
 
 static TiledShape tiled_shape(int64_t n) {
   int cus = device_cus();
@@ -204,6 +300,20 @@ static TiledShape tiled_shape(int64_t n) {
   int64_t T = (n + P - 1) / P;
   if (T < 1) T = 1;
   return {P, (int)T};
+}
+
+// largest tile the device's per-block LDS holds (floats), capped by the
+// 16-bit tile-local index
+static int max_tile_len() {
+  int dev = 0, shm = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 16384;
+  if (hipDeviceGetAttribute(&shm, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || shm <= 0)
+    return 16384;
+  // the segment lists of the exec kernel take kWavesPerBlock * kListCap int2
+  // of static LDS; keep the whole block within 64 KiB
+  if (shm > 65536) shm = 65536;
+  int t = (shm - (int)(kWavesPerBlock * kListCap * sizeof(int2))) / (int)sizeof(unsigned long long);
+  return t < BPX_TILED_MAX_TILE ? t : BPX_TILED_MAX_TILE;
 }
 
 }  // namespace bpx
@@ -235,7 +345,7 @@ int bpx_tiled_csr_layout(int64_t m, int64_t n, int64_t nnz, int weight_dtype, bp
   size_t o_li = take((size_t)nnz * sizeof(uint16_t));
   size_t wb = weight_dtype == BPX_F32 ? 4 : weight_dtype == BPX_BF16 ? 2 : 0;
   size_t o_w = take((size_t)nnz * wb);
-  size_t o_h = take(sizeof(float));
+  size_t o_h = take(2 * sizeof(float));
   plan->rowtile_offset = (int64_t)o_rt;
   plan->lidx_offset = (int64_t)o_li;
   plan->weights_offset = (int64_t)o_w;
@@ -268,6 +378,29 @@ int bpx_tiled_csr_build(void* stream, const void* data, int data_dtype, int64_t 
   const int P = plan->tiles;
   const unsigned blocks = (unsigned)((m + kBuildThreads - 1) / kBuildThreads);
   if (homo) hipLaunchKernelGGL(k_store_scalar, dim3(1), dim3(64), 0, s, data, data_dtype, hw);
+  // fixed-point scale of the LDS accumulators: |sum| <= max|w| * m < 2^62
+  {
+    uint32_t* amax = (uint32_t*)(base + plan->homo_offset + sizeof(float));
+    hipLaunchKernelGGL(k_fill_zero_t<uint32_t>, dim3(1), dim3(64), 0, s, amax, (int64_t)1);
+    const int64_t len = homo ? 1 : nnz;
+    int64_t ab = (len + 255) / 256;
+    if (ab > 1024) ab = 1024;
+    if (ab < 1) ab = 1;
+    if (data_dtype == BPX_F64) hipLaunchKernelGGL(k_absmax<double>, dim3((unsigned)ab), dim3(256), 0, s, (const double*)data, len, amax);
+    else if (data_dtype == BPX_BF16) hipLaunchKernelGGL(k_absmax<bf16_t>, dim3((unsigned)ab), dim3(256), 0, s, (const bf16_t*)data, len, amax);
+    else hipLaunchKernelGGL(k_absmax<float>, dim3((unsigned)ab), dim3(256), 0, s, (const float*)data, len, amax);
+    uint32_t bits = 0;
+    if (hipMemcpyAsync(&bits, amax, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return BPX_ERR_LAUNCH;
+    if (hipStreamSynchronize(s) != hipSuccess) return BPX_ERR_LAUNCH;
+    float mx;
+    memcpy(&mx, &bits, sizeof(float));
+    if (!(mx == mx) || mx == INFINITY) return BPX_ERR_ARG;      // non-finite weights
+    double bound = (double)mx * (double)(m > 1 ? m : 1);
+    int e = bound > 0.0 ? (int)ceil(log2(bound)) : -100;
+    int shift = 61 - e;
+    if (shift > 200) shift = 200;
+    plan->fx_shift = shift;
+  }
   if (m == 0) {
     BPX_CHECK_LAUNCH();
     return BPX_OK;
@@ -333,11 +466,20 @@ int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* e
   const void* w = base + plan->weights_offset;
   const float* hw = (const float*)(base + plan->homo_offset);
   const int P = plan->tiles, T = plan->tile_len;
-  const unsigned blocks = (unsigned)(8 * P);
-  const size_t shm = (size_t)T * sizeof(float);
-#define TL(WK, E)                                                                                             \
-  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E>), dim3(blocks), dim3(kTiledThreads), shm, s, rowtile, lidx, w, hw, \
-             (const E*)events, ev_rule, m, n, P, T, out)
+  const int cpx = tiled_copies_per_xcd();
+  const int grp = tiled_group_width();
+  const unsigned blocks = (unsigned)(8 * cpx * P);
+  const size_t shm = (size_t)T * sizeof(unsigned long long);
+  const double fx_scale = ldexp(1.0, plan->fx_shift), fx_inv = ldexp(1.0, -plan->fx_shift);
+#define TLG(WK, E, G)                                                                                         \
+  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E, G>), dim3(blocks), dim3(kTiledThreads), shm, s, rowtile, lidx, w, \
+             hw, (const E*)events, ev_rule, m, n, P, T, cpx, fx_scale, fx_inv, out)
+#define TL(WK, E)                       \
+  do {                                  \
+    if (grp == 16) TLG(WK, E, 16);      \
+    else if (grp == 32) TLG(WK, E, 32); \
+    else TLG(WK, E, 8);                 \
+  } while (0)
 #define TL_E(WK)                                     \
   do {                                               \
     if (ev_dtype == BPX_BOOL) TL(WK, uint8_t);       \
@@ -349,6 +491,7 @@ int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* e
   else TL_E(2);
 #undef TL_E
 #undef TL
+#undef TLG
   BPX_CHECK_LAUNCH();
   return BPX_OK;
 }

@@ -134,8 +134,14 @@ int bpx_event_csrmv_add(void* stream, const void* data, int data_dtype, int64_t 
  * CSR each call, event/_csr_matvec.py:453-500); it serves the same operator.
  * Usage: bpx_tiled_csr_layout (sizes, fills the plan's shape fields) ->
  * caller allocates `bytes` of device memory -> bpx_tiled_csr_build ->
- * bpx_event_csrmv_tiled per call.  Requires nnz < 2^31 and tile_len no
- * larger than min(32768, per-block LDS / 4) (n <= that * CUs-per-XCD).  Weights: f32, bf16 (rounded to nearest even
+ * bpx_event_csrmv_tiled per call.  Requires nnz < 2^31 and a tile that fits
+ * 32 KiB of LDS next to the kernel's row lists (tile_len <= 4096,
+ * n <= 4096 * CUs-per-XCD = 131072 on MI355X).  Accumulation inside a tile
+ * is in 64-bit fixed point (LDS integer atomics; gfx950's LDS float atomics
+ * are ~20x slower): fx_shift = 61 - ceil(log2(max|w| * m)) keeps every sum
+ * below 2^62, weights with |w| >= 2^(23 - fx_shift) convert exactly, and each
+ * copy's tile sum is rounded to f32 once.  The build synchronises the stream
+ * once (to read max|w|) and rejects non-finite weights (BPX_ERR_ARG).  Weights: f32, bf16 (rounded to nearest even
  * at build, accumulated in f32) or homogeneous (data_len 1).
  */
 #define BPX_TILED_HOMO (-1)
@@ -145,7 +151,7 @@ typedef struct {
   int32_t tiles;        /* P: output tiles per copy (= CUs per XCD) */
   int32_t tile_len;     /* T: floats per tile */
   int32_t weight_dtype; /* BPX_F32, BPX_BF16 or BPX_TILED_HOMO */
-  int32_t reserved;
+  int32_t fx_shift;     /* LDS accumulators hold sum(w) * 2^fx_shift as int64 */
   int64_t rowtile_offset, lidx_offset, weights_offset, homo_offset; /* byte offsets in mem */
   void* mem;            /* device memory holding the plan (set by build) */
 } bpx_tiled_csr;
