@@ -47,6 +47,73 @@ def algorithmic_bytes(active_rows, kind, path="csr"):
     return M * 1 + active_rows * 8 * TILES + active_rows * K_ROW * (2 + w_bytes) + N * 4
 
 
+INT_PEAK_TOPS = 39.3     # 256 CUs x 64 lanes x 2.4 GHz, 32-bit integer VALU
+OPS_PER_DRAW = 24        # LFSR88 next (18) + 2 xor + modulo + 3 add/compare
+
+
+def bench_jitconn(L, _lib, torch, steps, warmup):
+    """1M x 1M just-in-time connectivity, p = 1e-4 (clen 19999), seed 42,
+    2 % Bernoulli bool spikes; homo / uniform [0,1] / normal (0.5, 0.1);
+    outdim_parallel True (row-parallel, the default) and False (event-driven
+    scatter).  Work = LFSR88 draws, counted exactly from the generated
+    connections: per stream 1 initial draw + 1 per connection (+1 uniform /
+    +2 normal weight draws per connection)."""
+    import ctypes
+    from brainpy_amd.math.jitconn import conn_len
+    n = 1_000_000
+    dev = _lib.device()
+    g = torch.Generator(device=dev)
+    g.manual_seed(42)
+    ev = torch.rand(n, generator=g, device=dev) < 0.02
+    ones = torch.ones(n, dtype=torch.float32, device=dev)
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    clen = conn_len(1e-4)
+    sp = _lib.stream_ptr()
+    res = []
+    for od in (1, 0):
+        # exact connection counts of this layout: M @ ones (all streams) and
+        # M @ events (streams of active columns, for the scatter form)
+        L.bpx_jitconn_mv(sp, _lib.ptr(ones), _lib.BPX_F32, 0, 0, 1.0, 0.0, clen, 42, n, n, od, 0, _lib.ptr(out))
+        conns_all = float(out.double().sum())
+        L.bpx_jitconn_mv(sp, _lib.ptr(ev), _lib.BPX_BOOL, 1, 0, 1.0, 0.0, clen, 42, n, n, od, 0, _lib.ptr(out))
+        events = float(out.double().sum())       # synaptic events of this call
+        n_active = int(ev.sum())
+        streams = n * 32 if od else n_active * 32
+        conns = conns_all if od else events
+        for dist, (w0, w1), extra in ((0, (1.0, 0.0), 0), (1, (0.0, 1.0), 1), (2, (0.5, 0.1), 2)):
+            def call():
+                rc = L.bpx_jitconn_mv(sp, _lib.ptr(ev), _lib.BPX_BOOL, 1, dist, w0, w1, clen, 42, n, n, od, 0,
+                                      _lib.ptr(out))
+                if rc != 0:
+                    raise RuntimeError(f"bpx_jitconn_mv failed ({rc})")
+            for _ in range(warmup):
+                call()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                call()
+            torch.cuda.synchronize()
+            wall = (time.perf_counter() - t0) / steps
+            _lib.timer_reset()
+            _lib.timer_enable(True)
+            for _ in range(steps):
+                call()
+            torch.cuda.synchronize()
+            _lib.timer_enable(False)
+            ms, cnt = _lib.timer_read(_lib.KID["jit_outdim"] if od else _lib.KID["jit_scatter"])
+            kern = ms / 1e3 / max(cnt, 1)
+            draws = streams + conns * (1 + extra)
+            ops = draws * OPS_PER_DRAW
+            r = {"dist": ["homo", "uniform", "normal"][dist], "outdim_parallel": bool(od), "us_per_call": wall * 1e6,
+                 "kernel_us": kern * 1e6, "events_per_call": events, "events_per_s": events / wall,
+                 "draws": draws, "int_tops": ops / kern / 1e12, "int_frac": ops / kern / 1e12 / INT_PEAK_TOPS}
+            res.append(r)
+            print(f"# jitconn od={od} {r['dist']:7s} {r['us_per_call']:9.1f} us/call kernel {r['kernel_us']:9.1f} us "
+                  f"{r['int_tops']:6.2f} Tops ({100*r['int_frac']:.1f}%) {r['events_per_s']:.3e} ev/s",
+                  file=sys.stderr, flush=True)
+    return {"workload": "jitconn_event_mv_prob_1Mx1M_p1e-4_seed42_rate0.02", "results": res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -55,6 +122,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="time the headline configuration only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the jitconn and network legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs on a single MI355X and "
@@ -166,6 +234,10 @@ def main():
     plain = measure(HEADLINE[0], HEADLINE[1], args.steps, args.warmup, "csr")
     print(f"# headline {head}", file=sys.stderr, flush=True)
 
+    extra = {}
+    if not args.no_extra:
+        extra["jitconn"] = bench_jitconn(L, _lib, torch, max(10, args.steps // 10), 3)
+
     # CPU baseline: the oracle's serial scatter (the reference CPU loop
     # order, event/_csr_matvec.py:321-340), one thread, on a bounded number
     # of calls of the headline workload
@@ -223,6 +295,7 @@ def main():
                            "kernel": "k_csr_scatter (stateless bm.event.csrmv)"},
         "cpu_baseline": cpu,
         "sweep": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()} for s in sweep],
+        "extra": extra,
     }
     print(json.dumps(line), flush=True)
 

@@ -1,0 +1,231 @@
+// bpx_net.hip -- batched E/I COBA network step: many independent networks
+// advanced by one launch per step.
+//
+// Per launch (step i), every 256-thread block owns 256 neurons of one
+// network (a tile from the host-built tile table):
+//   1. update: for each neuron, g_exc += S(c_exc), g_inh += S(c_inh), where
+//      c_* are the numbers of pre-synaptic spikes of step i-1 delivered to it
+//      and S(c) = ((0 + w) + w) + ... (c terms) is exactly the homogeneous
+//      event csrmv / jitconn output of the reference (`out[row] += w` into a
+//      zeroed output, then `g += out`: dyn/synapses/abstract_models.py:
+//      133-134); the counts are zeroed for reuse; then the fused LIF/Expon/
+//      COBA update (bpx_lif.h) in the reference order.
+//   2. deliver: every neuron that spiked in step i regenerates or reads its
+//      targets and adds 1 to their count for step i+1 (integer atomics into
+//      the other count buffer).  Connectivity is either just-in-time (the
+//      lane32 scatter form of jitconn/_event_matvec_taichi.py:92-123,
+//      outdim_parallel=False: 32 lanes per spiking neuron, stream key
+//      seed + pre*32 + lane) or a stored CSR per network.
+// Integer counts make every conductance update exact and independent of
+// scheduling, so a batched run, a network run alone, and the serial CPU
+// restatement give the same spike raster.
+// Double-buffered counts remove the grid-wide barrier between the two
+// phases: step i reads/zeroes buffer i%2 and fills buffer (i+1)%2.
+#include "bpx_common.h"
+#include "bpx_lif.h"
+#include "bpx_rng.h"
+
+namespace bpx {
+
+constexpr int kNetThreads = 256;
+
+__device__ __forceinline__ float repeat_sum(uint32_t c, float w) {
+  float s = 0.f;
+  for (uint32_t k = 0; k < c; ++k) s = s + w;
+  return s;
+}
+
+struct NetArgs {
+  const bpx_net_desc* nets;
+  const int32_t* tiles;        // pairs (net, first local neuron)
+  int64_t total;               // neurons in the pools
+  float* V;
+  float* t_last;
+  float* g_exc;
+  float* g_inh;
+  uint32_t* counts;            // [2 buffers][2 kinds][total]
+  uint8_t* spikes;             // [total] spike flags of this step (nullable)
+  unsigned long long* events;  // [n_nets] delivered synaptic events (nullable)
+  unsigned int* spike_count;   // [n_nets] spikes (nullable)
+  LifParams p;
+  float tau_e, tau_i, E_e, E_i, c_e, c_i;
+  float dt;
+  int64_t step;                // used when d_step is null
+  const int64_t* d_step;       // device step counter (graph replays)
+};
+
+__global__ void k_net_advance(int64_t* d_step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *d_step += 1;
+}
+
+template <int CONN>
+__global__ __launch_bounds__(kNetThreads) void k_net_step(NetArgs a) {
+  __shared__ int s_spk[kNetThreads];
+  __shared__ int s_nspk;
+  __shared__ unsigned long long s_events;
+  const int net = a.tiles[2 * blockIdx.x];
+  const int start = a.tiles[2 * blockIdx.x + 1];
+  const bpx_net_desc d = a.nets[net];
+  const int N = d.n_exc + d.n_inh;
+  if (threadIdx.x == 0) {
+    s_nspk = 0;
+    s_events = 0ull;
+  }
+  __syncthreads();
+  const int j = start + (int)threadIdx.x;
+  const int64_t total = a.total;
+  const int64_t step = a.d_step ? *a.d_step : a.step;
+  const float t = (float)step * a.dt;
+  uint32_t* cur = a.counts + (step & 1) * 2 * total;
+  uint32_t* nxt = a.counts + ((step + 1) & 1) * 2 * total;
+  if (j < N) {
+    const int64_t gid = d.off + j;
+    const uint32_t ce = cur[gid], ci = cur[total + gid];
+    cur[gid] = 0u;
+    cur[total + gid] = 0u;
+    float g[2];
+    g[0] = a.g_exc[gid];
+    g[1] = a.g_inh[gid];
+    if (ce) g[0] = g[0] + repeat_sum(ce, d.w_exc);
+    if (ci) g[1] = g[1] + repeat_sum(ci, d.w_inh);
+    const float tau_g[2] = {a.tau_e, a.tau_i};
+    const float E_g[2] = {a.E_e, a.E_i};
+    const float c_g[2] = {a.c_e, a.c_i};
+    float V = a.V[gid], tl = a.t_last[gid];
+    const bool sp = lif_update<2>(V, tl, g, tau_g, E_g, c_g, d.I_ext, a.p, t, a.dt);
+    a.V[gid] = V;
+    a.t_last[gid] = tl;
+    a.g_exc[gid] = g[0];
+    a.g_inh[gid] = g[1];
+    if (a.spikes) a.spikes[gid] = sp ? 1 : 0;
+    if (sp) s_spk[atomicAdd(&s_nspk, 1)] = j;
+  }
+  __syncthreads();
+  const int nspk = s_nspk;
+  if (nspk == 0) return;
+  // deliver: 8 groups of 32 lanes, one spiking neuron per group
+  const int grp = threadIdx.x >> 5;
+  const uint32_t l = threadIdx.x & 31;
+  unsigned long long ev = 0ull;
+  for (int e = grp; e < nspk; e += kNetThreads / 32) {
+    const int p = s_spk[e];
+    const bool exc = p < d.n_exc;
+    uint32_t* cnt = nxt + (exc ? 0 : total) + d.off;     // post ids are network-local
+    if constexpr (CONN == BPX_NET_JIT) {
+      const int col = exc ? p : p - d.n_exc;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        // projections from this population: to E (post offset 0), to I (offset n_exc)
+        const int k = exc ? q : 2 + q;                   // E2E, E2I, I2E, I2I
+        const int64_t num_row = q == 0 ? d.n_exc : d.n_inh;
+        const int post_off = q == 0 ? 0 : d.n_exc;
+        if (num_row <= 0) continue;
+        const DivU32 dv = {d.div[k][0], d.div[k][1], d.div[k][2], d.div[k][3]};
+        const uint32_t step = (uint32_t)(((num_row + 1) >> 5) > 1 ? ((num_row + 1) >> 5) : 1);
+        uint32_t r = step * l - 1u;
+        uint32_t end = r + step;
+        if ((int64_t)end > num_row) end = (uint32_t)num_row;
+        Lfsr88 key(d.seed[k] + (uint32_t)(col * 32) + l);
+        r += (uint32_t)rand_inc(key, dv);
+        while (r < end) {
+          atomicAdd(cnt + post_off + r, 1u);
+          ++ev;
+          r += (uint32_t)rand_inc(key, dv);
+        }
+      }
+    } else {
+      const int32_t s = d.indptr[p], t2 = d.indptr[p + 1];
+      for (int32_t q = s + (int32_t)l; q < t2; q += 32) {
+        const int32_t tgt = d.indices[q];
+        if ((uint32_t)tgt < (uint32_t)N) {
+          atomicAdd(cnt + tgt, 1u);
+          ++ev;
+        }
+      }
+    }
+  }
+  if (a.events || a.spike_count) {
+    // block totals, one global atomic each
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ev += __shfl_xor(ev, o, 64);
+    if ((threadIdx.x & 63) == 0 && ev) atomicAdd(&s_events, ev);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (a.events) atomicAdd(a.events + net, s_events);
+      if (a.spike_count) atomicAdd(a.spike_count + net, (unsigned int)nspk);
+    }
+  }
+}
+
+}  // namespace bpx
+
+using namespace bpx;
+
+extern "C" {
+
+void bpx_net_prepare(bpx_net_desc* d) {
+  // divisor magic of each projection's clen (jitconn)
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t c = d->clen[k] > 0 ? (uint32_t)d->clen[k] : 1u;
+    const DivU32 dv = DivU32::make(c);
+    d->div[k][0] = dv.d;
+    d->div[k][1] = dv.m;
+    d->div[k][2] = dv.s1;
+    d->div[k][3] = dv.s2;
+  }
+}
+
+int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, int n_tiles, int conn_kind,
+                 int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts,
+                 uint8_t* spikes, unsigned long long* events, unsigned int* spike_count,
+                 const bpx_net_params* prm, int64_t step, const int64_t* d_step) {
+  if (!nets || !tiles || !prm) return BPX_ERR_ARG;
+  if (n_tiles < 0 || total < 0) return BPX_ERR_SHAPE;
+  if (conn_kind != BPX_NET_JIT && conn_kind != BPX_NET_CSR) return BPX_ERR_ARG;
+  if (n_tiles == 0) return BPX_OK;
+  NetArgs a;
+  a.nets = nets;
+  a.tiles = tiles;
+  a.total = total;
+  a.V = V;
+  a.t_last = t_last;
+  a.g_exc = g_exc;
+  a.g_inh = g_inh;
+  a.counts = counts;
+  a.spikes = spikes;
+  a.events = events;
+  a.spike_count = spike_count;
+  a.p.V_rest = prm->V_rest;
+  a.p.V_th = prm->V_th;
+  a.p.V_reset = prm->V_reset;
+  a.p.tau = prm->tau;
+  a.p.tau_ref = prm->tau_ref;
+  a.p.R = prm->R;
+  a.p.ltc = prm->ltc;
+  a.p.cV = prm->dt * exprel_det(prm->dt * (-(1.0f / prm->tau)));
+  a.tau_e = prm->tau_e;
+  a.tau_i = prm->tau_i;
+  a.E_e = prm->E_e;
+  a.E_i = prm->E_i;
+  a.c_e = prm->c_e;
+  a.c_i = prm->c_i;
+  a.dt = prm->dt;
+  a.step = step;
+  a.d_step = d_step;
+  hipStream_t s = (hipStream_t)stream;
+  if (conn_kind == BPX_NET_JIT)
+    BPX_LAUNCH(KID_NET, k_net_step<BPX_NET_JIT>, dim3((unsigned)n_tiles), dim3(kNetThreads), 0, s, a);
+  else
+    BPX_LAUNCH(KID_NET, k_net_step<BPX_NET_CSR>, dim3((unsigned)n_tiles), dim3(kNetThreads), 0, s, a);
+  BPX_CHECK_LAUNCH();
+  return BPX_OK;
+}
+
+int bpx_net_advance(void* stream, int64_t* d_step) {
+  if (!d_step) return BPX_ERR_ARG;
+  hipLaunchKernelGGL(k_net_advance, dim3(1), dim3(64), 0, (hipStream_t)stream, d_step);
+  BPX_CHECK_LAUNCH();
+  return BPX_OK;
+}
+
+}  // extern "C"

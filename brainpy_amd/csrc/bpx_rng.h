@@ -38,7 +38,7 @@ struct Lfsr88 {
 struct DivU32 {
   uint32_t d, m, s1, s2;
 
-  static DivU32 make(uint32_t d) {
+  BPX_HD static DivU32 make(uint32_t d) {
     DivU32 r;
     r.d = d;
     uint32_t l = 0;

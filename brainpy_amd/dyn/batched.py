@@ -1,0 +1,223 @@
+"""Batched E/I COBA networks: many independent networks advanced by one
+launch per step (``bpx_net_step``).
+
+Each network is the ``EICOBA_PostAlign`` model of
+examples/dynamics_simulation/COBA.py:58-109 (LifRefLTC or LifRef neurons,
+Expon + COBA synapses, homogeneous weights, no delay) with either
+just-in-time connectivity (``bm.jitconn.event_mv_prob_homo`` with
+``outdim_parallel=False``: every spiking neuron regenerates its targets from
+LFSR88 streams, nothing stored) or a stored CSR.  Per step the kernel
+updates all neurons (adding the conductance delivered by the previous step's
+spikes) and then delivers this step's spikes as integer counts, which makes
+every network's trajectory exact and independent of what else runs in the
+batch.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..math.jitconn import conn_len
+from .coba import expon_coef
+
+NET_JIT, NET_CSR = 0, 1
+TILE = 256
+
+
+class NetDesc(ctypes.Structure):
+    """Mirror of bpx_net_desc (include/bpx.h)."""
+    _fields_ = [("off", ctypes.c_int64), ("n_exc", ctypes.c_int32), ("n_inh", ctypes.c_int32),
+                ("seed", ctypes.c_uint32 * 4), ("clen", ctypes.c_int32 * 4), ("div", (ctypes.c_uint32 * 4) * 4),
+                ("w_exc", ctypes.c_float), ("w_inh", ctypes.c_float), ("I_ext", ctypes.c_float),
+                ("reserved", ctypes.c_int32), ("indptr", ctypes.c_void_p), ("indices", ctypes.c_void_p)]
+
+
+class NetParams(ctypes.Structure):
+    """Mirror of bpx_net_params (include/bpx.h)."""
+    _fields_ = [(k, ctypes.c_float) for k in ("V_rest", "V_th", "V_reset", "tau", "tau_ref", "R", "tau_e", "tau_i",
+                                              "E_e", "E_i", "c_e", "c_i", "dt")] + [("ltc", ctypes.c_int32)]
+
+
+def _register():
+    L = _lib.lib()
+    if not hasattr(L, "_bpx_net_registered"):
+        L.bpx_net_prepare.argtypes = [ctypes.POINTER(NetDesc)]
+        L.bpx_net_prepare.restype = None
+        L.bpx_net_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.POINTER(NetParams), ctypes.c_int64, ctypes.c_void_p]
+        L.bpx_net_step.restype = ctypes.c_int
+        L.bpx_net_advance.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.bpx_net_advance.restype = ctypes.c_int
+        L._bpx_net_registered = True
+    return L
+
+
+def bf16_round(x):
+    """Round a float32 to the nearest bfloat16 value (ties to even)."""
+    u = np.array([x], np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return float(r.astype(np.uint32).view(np.float32)[0])
+
+
+def combined_csr(csrs, n_exc, n_inh):
+    """One row per neuron over network-local post ids from the four
+    projection CSRs {name: (indices, indptr)} (E2E, E2I, I2E, I2I)."""
+    rows = []
+    for p in range(n_exc + n_inh):
+        parts = []
+        if p < n_exc:
+            for name, off in (("E2E", 0), ("E2I", n_exc)):
+                idx, ptr = csrs[name]
+                parts.append(idx[ptr[p]:ptr[p + 1]].astype(np.int64) + off)
+        else:
+            q = p - n_exc
+            for name, off in (("I2E", 0), ("I2I", n_exc)):
+                idx, ptr = csrs[name]
+                parts.append(idx[ptr[q]:ptr[q + 1]].astype(np.int64) + off)
+        rows.append(np.concatenate(parts))
+    indptr = np.zeros(n_exc + n_inh + 1, np.int64)
+    np.cumsum([len(r) for r in rows], out=indptr[1:])
+    return np.concatenate(rows).astype(np.int32), indptr.astype(np.int32)
+
+
+class BatchedCOBA:
+    """A batch of independent COBA networks sharing neuron parameters.
+
+    ``specs``: list of dicts with keys ``n_exc``, ``n_inh``, ``seed``,
+    ``w_exc``, ``w_inh``, ``I_ext`` and either ``in_degree`` (just-in-time
+    connectivity with p = in_degree / N for all four projections) or ``csr``
+    = {E2E, E2I, I2E, I2I: (indices, indptr)}.  All specs must use the same
+    kind.  ``bf16=True`` rounds the homogeneous weights to bfloat16.
+    """
+
+    def __init__(self, specs, V_rest=-60.0, V_th=-50.0, V_reset=-60.0, tau=20.0, tau_ref=5.0, R=1.0, tau_e=5.0,
+                 tau_i=10.0, E_e=0.0, E_i=-80.0, dt=0.1, ltc=True, bf16=False, record=False, V_init=None):
+        self.dev = _lib.device()
+        L = _register()
+        self.kind = NET_CSR if "csr" in specs[0] else NET_JIT
+        self.specs = specs
+        n = len(specs)
+        self.descs = (NetDesc * n)()
+        self.sizes = []
+        self._keep = []
+        off = 0
+        self.tiles = []
+        for k, sp in enumerate(specs):
+            d = self.descs[k]
+            ne, ni = int(sp["n_exc"]), int(sp["n_inh"])
+            N = ne + ni
+            d.off, d.n_exc, d.n_inh = off, ne, ni
+            w_e, w_i = float(sp["w_exc"]), float(sp["w_inh"])
+            if bf16:
+                w_e, w_i = bf16_round(w_e), bf16_round(w_i)
+            d.w_exc, d.w_inh, d.I_ext = w_e, w_i, float(sp["I_ext"])
+            if self.kind == NET_JIT:
+                p = float(sp["in_degree"]) / N
+                c = conn_len(min(p, 1.0))
+                base = int(sp["seed"])
+                for q in range(4):
+                    d.seed[q] = (base * 4 + q) & 0xFFFFFFFF
+                    d.clen[q] = c
+            else:
+                idx, ptr = combined_csr(sp["csr"], ne, ni)
+                ti = torch.from_numpy(idx).to(self.dev)
+                tp = torch.from_numpy(ptr).to(self.dev)
+                self._keep += [ti, tp]
+                d.indptr, d.indices = tp.data_ptr(), ti.data_ptr()
+                for q in range(4):
+                    d.clen[q] = 1
+            L.bpx_net_prepare(ctypes.byref(d))
+            for s in range(0, N, TILE):
+                self.tiles += [k, s]
+            self.sizes.append(N)
+            off += N
+        self.total = off
+        self.d_descs = torch.frombuffer(bytearray(bytes(self.descs)), dtype=torch.uint8).to(self.dev)
+        self.d_tiles = torch.tensor(self.tiles, dtype=torch.int32, device=self.dev)
+        self.n_tiles = len(self.tiles) // 2
+        self.params = NetParams(V_rest=V_rest, V_th=V_th, V_reset=V_reset, tau=tau, tau_ref=tau_ref, R=R,
+                                tau_e=tau_e, tau_i=tau_i, E_e=E_e, E_i=E_i, c_e=float(expon_coef(tau_e, dt)),
+                                c_i=float(expon_coef(tau_i, dt)), dt=dt, ltc=int(ltc))
+        self.dt = np.float32(dt)
+        if V_init is None:
+            parts = []
+            for sp, N in zip(specs, self.sizes):
+                rng = np.random.Generator(np.random.PCG64(int(sp["seed"]) + 7))
+                parts.append(rng.uniform(V_rest, V_th, N).astype(np.float32))
+            V_init = np.concatenate(parts)
+        self.V0 = np.asarray(V_init, np.float32)
+        self.record = record
+        self.reset()
+
+    def reset(self):
+        d = self.dev
+        T = self.total
+        self.V = torch.from_numpy(self.V0.copy()).to(d)
+        self.t_last = torch.full((T,), -1e7, dtype=torch.float32, device=d)
+        self.g_exc = torch.zeros(T, dtype=torch.float32, device=d)
+        self.g_inh = torch.zeros(T, dtype=torch.float32, device=d)
+        self.counts = torch.zeros(4 * T, dtype=torch.int32, device=d)
+        self.spikes = torch.zeros(T, dtype=torch.uint8, device=d)
+        self.events = torch.zeros(len(self.specs), dtype=torch.int64, device=d)
+        self.spike_count = torch.zeros(len(self.specs), dtype=torch.int32, device=d)
+        self.d_step = torch.zeros(1, dtype=torch.int64, device=d)
+        self.graph = None
+        self.i = 0
+
+    def step(self, spikes_out=None, count=True, device_step=False):
+        """One step of every network.  `spikes_out` (uint8 [total]) receives
+        this step's spike flags.  With `device_step` the step index is read
+        from (and advanced in) the device counter, as graph replays need."""
+        L = _lib.lib()
+        sp = spikes_out if spikes_out is not None else (self.spikes if self.record else None)
+        s = _lib.stream_ptr()
+        rc = L.bpx_net_step(s, _lib.ptr(self.d_descs), _lib.ptr(self.d_tiles), self.n_tiles, self.kind,
+                            self.total, _lib.ptr(self.V), _lib.ptr(self.t_last), _lib.ptr(self.g_exc),
+                            _lib.ptr(self.g_inh), _lib.ptr(self.counts), _lib.ptr(sp),
+                            _lib.ptr(self.events) if count else None, _lib.ptr(self.spike_count) if count else None,
+                            ctypes.byref(self.params), self.i, _lib.ptr(self.d_step) if device_step else None)
+        _lib.check(rc, "bpx_net_step")
+        if device_step:
+            _lib.check(L.bpx_net_advance(s, _lib.ptr(self.d_step)), "bpx_net_advance")
+        self.i += 1
+
+    def run(self, steps, raster=False):
+        """Advance `steps` steps; with `raster` returns uint8 [steps, total]."""
+        out = torch.empty((steps, self.total), dtype=torch.uint8, device=self.dev) if raster else None
+        for k in range(steps):
+            self.step(out[k] if raster else None)
+        return out
+
+    def capture(self, steps):
+        """Capture `steps` steps in one HIP graph (each step one batched
+        launch plus the device step-counter increment); replay() advances the
+        batch by `steps` steps per call, continuing in time."""
+        self.d_step.fill_(self.i)
+        torch.cuda.synchronize()
+        i0 = self.i
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                for _ in range(steps):
+                    self.step(device_step=True)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        # capture records the launches without running them
+        self.d_step.fill_(i0)
+        self.i = i0
+        self.graph = g
+        self._graph_steps = steps
+        return g
+
+    def replay(self):
+        self.graph.replay()
+        self.i += self._graph_steps
+
+    def net_slice(self, k):
+        off = int(self.descs[k].off)
+        return slice(off, off + self.sizes[k])

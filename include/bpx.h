@@ -206,6 +206,59 @@ int bpx_lif_coba_step(void* stream, int64_t n, int ltc, int n_g,
  */
 float bpx_expon_coef(float tau, float dt);
 
+/*
+ * Batched E/I COBA network step.  Advances any number of independent
+ * networks (LifRefLTC/LifRef neurons, Expon + COBA synapses, homogeneous
+ * weights, delay None: the EICOBA_PostAlign model of
+ * examples/dynamics_simulation/COBA.py:58-109) by one step in one launch.
+ * No single reference counterpart: it fuses, per step, the four event
+ * projections (event csrmv, event/_csr_matvec.py:38-110, or jitconn
+ * event_mv_prob_homo with outdim_parallel=False,
+ * jitconn/_event_matvec_taichi.py:92-123) and the population updates
+ * (dyn/neurons/lif.py:461-500) in the reference order.
+ * State pools are indexed by off + local neuron id (E first, then I);
+ * counts is [2][2][total] uint32, zero at start.  The step index (time
+ * t = step*dt, count buffer parity) is `step`, or *d_step when d_step is a
+ * device counter -- the form for HIP graphs, where bpx_net_advance(d_step)
+ * after each step makes every replay continue in time.
+ * tiles: device array of (network, first local neuron) int32 pairs, one per
+ * 256-neuron block.  nets: device array of bpx_net_desc prepared with
+ * bpx_net_prepare (host).  jitconn projections use seed[k], clen[k] for
+ * k = E2E, E2I, I2E, I2I with matrix shape (post, pre); CSR networks give one
+ * row per neuron listing local post ids (E targets of E neurons feed the
+ * excitatory conductance, targets of I neurons the inhibitory one).
+ * spikes/events/spike_count are optional outputs.
+ */
+#define BPX_NET_JIT 0
+#define BPX_NET_CSR 1
+typedef struct {
+  int64_t off;               /* first pool index of the network */
+  int32_t n_exc, n_inh;
+  uint32_t seed[4];          /* jitconn seeds: E2E, E2I, I2E, I2I */
+  int32_t clen[4];           /* jitconn ceil(1/p)*2-1 per projection */
+  uint32_t div[4][4];        /* filled by bpx_net_prepare */
+  float w_exc, w_inh;        /* homogeneous weights */
+  float I_ext;               /* constant input */
+  int32_t reserved;
+  const int32_t* indptr;     /* CSR networks: [N+1] */
+  const int32_t* indices;    /* CSR networks: local post ids */
+} bpx_net_desc;
+
+typedef struct {
+  float V_rest, V_th, V_reset, tau, tau_ref, R;
+  float tau_e, tau_i, E_e, E_i;
+  float c_e, c_i;            /* bpx_expon_coef(tau_e, dt), bpx_expon_coef(tau_i, dt) */
+  float dt;
+  int32_t ltc;               /* 1 LifRefLTC, 0 LifRef */
+} bpx_net_params;
+
+void bpx_net_prepare(bpx_net_desc* desc);
+int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, int n_tiles, int conn_kind,
+                 int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts,
+                 uint8_t* spikes, unsigned long long* events, unsigned int* spike_count,
+                 const bpx_net_params* params, int64_t step, const int64_t* d_step);
+int bpx_net_advance(void* stream, int64_t* d_step);
+
 /* Plain device copy (bandwidth calibration of the HBM counters). */
 int bpx_copy(void* stream, const void* src, void* dst, int64_t bytes);
 

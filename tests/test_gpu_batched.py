@@ -1,0 +1,111 @@
+"""GPU parity of the batched network step (bpx_net_step).
+
+* just-in-time networks: the spike raster equals the oracle's serial run
+  (lane32 event-driven jitconn, outdim_parallel=False, then the neuron step)
+  bit for bit;
+* CSR networks: equals the serial CSR oracle of the EICOBA model;
+* a network gives the same raster alone and inside a batch of other
+  networks, with and without HIP-graph capture.
+"""
+import numpy as np
+import pytest
+import torch
+
+from brainpy_amd.dyn import EICOBANet
+from brainpy_amd.dyn.batched import BatchedCOBA
+from brainpy_amd.math.jitconn import conn_len
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+PARAMS = (-60.0, -50.0, -60.0, 20.0, 5.0, 1.0)
+
+
+def _jit_specs(sizes, seed0=100):
+    rng = np.random.default_rng(seed0)
+    specs = []
+    for k, N in enumerate(sizes):
+        ne = int(N * 0.8)
+        specs.append(dict(n_exc=ne, n_inh=N - ne, seed=seed0 + k, w_exc=float(rng.uniform(0.3, 0.9)),
+                          w_inh=float(rng.uniform(4, 9)), I_ext=float(rng.uniform(15, 25)), in_degree=80))
+    return specs
+
+
+def _oracle_jit(batch, k, steps):
+    sp = batch.specs[k]
+    d = batch.descs[k]
+    ne, ni = sp["n_exc"], sp["n_inh"]
+    V = batch.V0[batch.net_slice(k)].copy()
+    VE, VI = V[:ne].copy(), V[ne:].copy()
+    tl = {"E": np.full(ne, -1e7, np.float32), "I": np.full(ni, -1e7, np.float32)}
+    s = {"E": np.zeros(ne, np.uint8), "I": np.zeros(ni, np.uint8)}
+    g = {"E": [np.zeros(ne, np.float32), np.zeros(ne, np.float32)],
+         "I": [np.zeros(ni, np.float32), np.zeros(ni, np.float32)]}
+    Vs = {"E": VE, "I": VI}
+    size = {"E": ne, "I": ni}
+    prm = batch.params
+    c_g = [np.float32(prm.c_e), np.float32(prm.c_i)]
+    raster = np.zeros((steps, ne + ni), np.uint8)
+    # (projection index, pre, post, conductance group, weight)
+    projs = [(0, "E", "E", 0, d.w_exc), (1, "E", "I", 0, d.w_exc), (2, "I", "E", 1, d.w_inh),
+             (3, "I", "I", 1, d.w_inh)]
+    for i in range(steps):
+        t = np.float32(i) * np.float32(prm.dt)
+        for q, pre, post, kind, w in projs:
+            out = orc.jitconn_mv(s[pre].view(bool), "homo", 1, False, orc.LAYOUT_LANE32, size[post], size[pre],
+                                 np.float32(w), 0.0, d.clen[q], d.seed[q])
+            g[post][kind] += out
+        for p in ("E", "I"):
+            orc.lif_coba_step(Vs[p], tl[p], s[p], g[p], [prm.tau_e, prm.tau_i], [prm.E_e, prm.E_i], c_g, d.I_ext,
+                              PARAMS, t, prm.dt, ltc=bool(prm.ltc))
+        raster[i, :ne] = s["E"]
+        raster[i, ne:] = s["I"]
+    return raster
+
+
+@pytest.mark.parametrize("ltc", [True, False])
+def test_jit_batch_vs_oracle(ltc):
+    batch = BatchedCOBA(_jit_specs([1000, 1500, 4000]), ltc=ltc)
+    raster = batch.run(100, raster=True).cpu().numpy()
+    for k in range(3):
+        ref = _oracle_jit(batch, k, 100)
+        assert ref.sum() > 0
+        np.testing.assert_array_equal(raster[:, batch.net_slice(k)], ref)
+
+
+def test_alone_equals_in_mix_and_graph():
+    specs = _jit_specs([1200, 3000, 800, 5000, 2000], seed0=7)
+    mix = BatchedCOBA(specs)
+    r_mix = mix.run(60, raster=True).cpu().numpy()
+    for k in (0, 3):
+        alone = BatchedCOBA([specs[k]], V_init=mix.V0[mix.net_slice(k)])
+        r_alone = alone.run(60, raster=True).cpu().numpy()
+        np.testing.assert_array_equal(r_mix[:, mix.net_slice(k)], r_alone)
+    # graph capture: 3 replays of 20 steps == 60 eager steps
+    g = BatchedCOBA(specs)
+    g.capture(20)
+    for _ in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+    np.testing.assert_array_equal(g.V.cpu().numpy(), mix.V.cpu().numpy())
+    np.testing.assert_array_equal(g.t_last.cpu().numpy(), mix.t_last.cpu().numpy())
+    assert int(g.events.sum()) == int(mix.events.sum()) > 0
+
+
+def test_bf16_weights_rounded():
+    specs = _jit_specs([2000], seed0=3)
+    b = BatchedCOBA(specs, bf16=True)
+    w = np.array([b.descs[0].w_exc, b.descs[0].w_inh], np.float32)
+    assert (w.view(np.uint32) & 0xFFFF == 0).all()
+    b.run(20)
+
+
+def test_csr_network_vs_eicoba_oracle():
+    net = EICOBANet(seed=5, mode="fixed_order")
+    csrs = {name: (net.csr[name][1], net.csr[name][2]) for name in ("E2E", "E2I", "I2E", "I2I")}
+    spec = dict(n_exc=net.n_exc, n_inh=net.n_inh, seed=5, w_exc=0.6, w_inh=6.7, I_ext=20.0, csr=csrs)
+    b = BatchedCOBA([spec], V_init=net.V0)
+    raster = b.run(100, raster=True).cpu().numpy()
+    ref = net.run(100).cpu().numpy()
+    assert ref.sum() > 0
+    np.testing.assert_array_equal(raster, ref)
