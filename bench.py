@@ -114,6 +114,162 @@ def bench_jitconn(L, _lib, torch, steps, warmup):
     return {"workload": "jitconn_event_mv_prob_1Mx1M_p1e-4_seed42_rate0.02", "results": res}
 
 
+def _percentile(x, q):
+    return float(np.percentile(np.asarray(x, np.float64), q)) if len(x) else None
+
+
+def bench_coba(_lib, torch, steps=1000):
+    """C1: 4000-neuron E/I COBA network (3200 E + 800 I, FixedProb p = 0.02,
+    we = 0.6, wi = 6.7, input 20 mV, V0 ~ U[-60, -50], dt = 0.1 ms, 1000 steps,
+    LifRefLTC, exp_euler).  GPU: the batched step kernel on this one network
+    (CSR connectivity), HIP-graph captured; plain op-by-op GPU path; CPU: the
+    oracle's serial restatement (1 thread)."""
+    from brainpy_amd.dyn import EICOBANet
+    from brainpy_amd.dyn.batched import BatchedCOBA, combined_csr
+    from oracle import orc
+    net = EICOBANet(seed=0, mode="atomic")
+    csrs = {k: (net.csr[k][1], net.csr[k][2]) for k in ("E2E", "E2I", "I2E", "I2I")}
+    spec = dict(n_exc=3200, n_inh=800, seed=0, w_exc=0.6, w_inh=6.7, I_ext=20.0, csr=csrs)
+    b = BatchedCOBA([spec], V_init=net.V0)
+    b.capture(100)
+    b.replay()
+    torch.cuda.synchronize()
+    b.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps // 100):
+        b.replay()
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t0
+    spikes = int(b.spike_count.sum())
+    events = spikes * 80          # every neuron has 64 + 16 targets
+    b2 = BatchedCOBA([spec], V_init=net.V0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b2.run(steps)
+    torch.cuda.synchronize()
+    t_eager = time.perf_counter() - t0
+    net.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    net.run(steps, record=False)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    idx, ptr = combined_csr(csrs, 3200, 800)
+    t0 = time.perf_counter()
+    cpu_spikes, _ = orc.coba_net_run(3200, 800, ptr, idx, 0.6, 6.7, 20.0, net.V0, steps,
+                                     (-60.0, -50.0, -60.0, 20.0, 5.0, 1.0), raster=False)
+    t_cpu = time.perf_counter() - t0
+    r = {"workload": "coba_4000_p0.02_1000steps", "steps": steps, "spikes": spikes, "cpu_spikes": int(cpu_spikes),
+         "synaptic_events": events, "gpu_graph_s": t_graph, "gpu_eager_s": t_eager, "gpu_plain_opbyop_s": t_plain,
+         "cpu_1thread_s": t_cpu, "gpu_steps_per_s": steps / t_graph, "gpu_events_per_s": events / t_graph,
+         "cpu_events_per_s": events / t_cpu, "mean_rate_hz": spikes / 4000 / (steps * 1e-4)}
+    print(f"# coba C1 graph {t_graph*1e3:.2f} ms eager {t_eager*1e3:.2f} ms plain {t_plain*1e3:.2f} ms "
+          f"cpu {t_cpu*1e3:.1f} ms spikes {spikes} (cpu {cpu_spikes})", file=sys.stderr, flush=True)
+    return r
+
+
+def _c4_specs(n_nets=256, seed=2024):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = np.exp(rng.uniform(np.log(1000), np.log(64000), n_nets)).astype(np.int64)
+    specs = []
+    for k, N in enumerate(sizes):
+        ne = int(round(N * 0.8))
+        specs.append(dict(n_exc=ne, n_inh=int(N) - ne, seed=10_000 + k, w_exc=float(rng.uniform(0.3, 0.9)),
+                          w_inh=float(rng.uniform(4, 9)), I_ext=float(rng.uniform(15, 25)), in_degree=80))
+    return specs
+
+
+def bench_batched(_lib, torch, steps=2000, n_nets=256, plain_sample=8, plain_steps=100):
+    """C4: 256 independent E/I networks (4:1), N log-uniform in [1k, 64k],
+    per-network seed, we ~ U(0.3, 0.9), wi ~ U(4, 9), input ~ U(15, 25) mV,
+    jitconn in-degree 80, BF16 weights, FP32 state, 2000 steps, one launch
+    per step for all networks, captured in HIP graphs of 100 steps.
+    Baseline: the plain per-network FP32 op-by-op path, timed on a sample of
+    networks (spread over the size range) and scaled to all 256 x 2000 steps."""
+    from brainpy_amd.dyn.batched import BatchedCOBA, PlainCOBA, bf16_round
+    specs = _c4_specs(n_nets)
+    b = BatchedCOBA(specs, bf16=True)
+    b.capture(100)
+    b.replay()
+    torch.cuda.synchronize()
+    b.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps // 100):
+        b.replay()
+    torch.cuda.synchronize()
+    t_b = time.perf_counter() - t0
+    events = int(b.events.sum())
+    spikes = int(b.spike_count.sum())
+    total_neurons = b.total
+    # plain FP32 baseline on a size-stratified sample
+    order = np.argsort([s["n_exc"] + s["n_inh"] for s in specs])
+    pick = order[np.linspace(0, len(order) - 1, plain_sample).astype(int)]
+    per_step = []
+    for k in pick:
+        p = PlainCOBA(specs[k])
+        p.run(5)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p.run(plain_steps)
+        torch.cuda.synchronize()
+        per_step.append((time.perf_counter() - t0) / plain_steps)
+    t_plain_est = float(np.mean(per_step)) * steps * n_nets
+    # BF16 weight error against the FP32/FP64 weights (homogeneous weights:
+    # the per-step current error is the weight's rounding error)
+    rel = []
+    for s in specs:
+        for key in ("w_exc", "w_inh"):
+            w = np.float64(np.float32(s[key]))
+            rel.append(abs(bf16_round(s[key]) - w) / w)
+    r = {"workload": "batched_coba_256nets_1k-64k_jit80_bf16_2000steps", "networks": n_nets,
+         "total_neurons": total_neurons, "steps": steps, "batched_s": t_b, "sims_per_s": n_nets / t_b,
+         "synaptic_events": events, "spikes": spikes, "events_per_s": events / t_b, "us_per_step": t_b / steps * 1e6,
+         "plain_fp32_est_s": t_plain_est, "plain_sims_per_s": n_nets / t_plain_est,
+         "plain_events_per_s_est": events / t_plain_est, "speedup_vs_plain": t_plain_est / t_b,
+         "plain_sample_networks": int(plain_sample), "plain_sample_steps": plain_steps,
+         "bf16_weight_rel_err_max": float(max(rel)), "bf16_weight_rel_err_mean": float(np.mean(rel))}
+    print(f"# batched C4 {t_b:.3f} s ({t_b/steps*1e6:.1f} us/step) {events/t_b:.3e} ev/s, plain est "
+          f"{t_plain_est:.1f} s -> x{t_plain_est/t_b:.1f}", file=sys.stderr, flush=True)
+    return r
+
+
+def bench_served(_lib, torch, n_requests=80, rate=50.0, seed=7):
+    """C5: Poisson arrivals at 50 req/s of COBA requests, N log-uniform in
+    [4k, 1M], durations U[100, 5000] steps, jitconn in-degree 80, BF16
+    weights, FP32 state, admitted against a 200 GB HBM budget; continuous
+    batching versus a FIFO queue running one request at a time (same
+    kernel, same arrival trace)."""
+    from brainpy_amd.dyn.serve import Request, SimServer
+    rng = np.random.Generator(np.random.PCG64(seed))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    sizes = np.exp(rng.uniform(np.log(4000), np.log(1_000_000), n_requests)).astype(np.int64)
+    durs = rng.integers(100, 5001, n_requests)
+    params = [(int(sizes[k]), 50_000 + k, float(rng.uniform(0.3, 0.9)), float(rng.uniform(4, 9)),
+               float(rng.uniform(15, 25)), int(durs[k]), float(arrivals[k])) for k in range(n_requests)]
+
+    def run(max_active):
+        srv = SimServer(capacity=1 << 25, hbm_budget=200 * (1 << 30), bf16=True, max_active=max_active)
+        reqs = [Request(k, n, s, we, wi, I, st, arrival=a) for k, (n, s, we, wi, I, st, a) in enumerate(params)]
+        done = srv.serve(reqs)
+        lat = [r.completed - r.arrival for r in done]
+        ev = sum(r.events for r in done)
+        return {"completed": len(done), "wall_s": srv.wall, "p50_latency_s": _percentile(lat, 50),
+                "p99_latency_s": _percentile(lat, 99), "events": ev, "events_per_s": ev / srv.wall,
+                "sims_per_s": len(done) / srv.wall, "steps_launched": srv.step_idx}
+
+    served = run(None)
+    fifo = run(1)
+    r = {"workload": f"served_coba_poisson{rate:g}_{n_requests}req_4k-1M_100-5000steps_jit80_bf16",
+         "requests": n_requests, "continuous_batching": served, "fifo_queue": fifo,
+         "throughput_ratio": served["events_per_s"] / fifo["events_per_s"] if fifo["events_per_s"] else None}
+    print(f"# served C5 batching {served['events_per_s']:.3e} ev/s p50 {served['p50_latency_s']:.3f} s "
+          f"p99 {served['p99_latency_s']:.3f} s | fifo {fifo['events_per_s']:.3e} ev/s p50 "
+          f"{fifo['p50_latency_s']:.3f} s p99 {fifo['p99_latency_s']:.3f} s", file=sys.stderr, flush=True)
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +393,9 @@ def main():
     extra = {}
     if not args.no_extra:
         extra["jitconn"] = bench_jitconn(L, _lib, torch, max(10, args.steps // 10), 3)
+        extra["coba"] = bench_coba(_lib, torch)
+        extra["batched"] = bench_batched(_lib, torch)
+        extra["served"] = bench_served(_lib, torch)
 
     # CPU baseline: the oracle's serial scatter (the reference CPU loop
     # order, event/_csr_matvec.py:321-340), one thread, on a bounded number

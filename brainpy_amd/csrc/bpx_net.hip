@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kNetThreads) void k_net_step(NetArgs a) {
   const int j = start + (int)threadIdx.x;
   const int64_t total = a.total;
   const int64_t step = a.d_step ? *a.d_step : a.step;
-  const float t = (float)step * a.dt;
+  const float t = (float)(step - d.t0_step) * a.dt;      // network-local time
   uint32_t* cur = a.counts + (step & 1) * 2 * total;
   uint32_t* nxt = a.counts + ((step + 1) & 1) * 2 * total;
   if (j < N) {

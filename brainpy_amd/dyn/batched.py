@@ -30,7 +30,7 @@ class NetDesc(ctypes.Structure):
     _fields_ = [("off", ctypes.c_int64), ("n_exc", ctypes.c_int32), ("n_inh", ctypes.c_int32),
                 ("seed", ctypes.c_uint32 * 4), ("clen", ctypes.c_int32 * 4), ("div", (ctypes.c_uint32 * 4) * 4),
                 ("w_exc", ctypes.c_float), ("w_inh", ctypes.c_float), ("I_ext", ctypes.c_float),
-                ("reserved", ctypes.c_int32), ("indptr", ctypes.c_void_p), ("indices", ctypes.c_void_p)]
+                ("t0_step", ctypes.c_int32), ("indptr", ctypes.c_void_p), ("indices", ctypes.c_void_p)]
 
 
 class NetParams(ctypes.Structure):
@@ -153,18 +153,29 @@ class BatchedCOBA:
         self.reset()
 
     def reset(self):
+        """Initial state (in place, so a captured graph stays valid)."""
         d = self.dev
         T = self.total
-        self.V = torch.from_numpy(self.V0.copy()).to(d)
-        self.t_last = torch.full((T,), -1e7, dtype=torch.float32, device=d)
-        self.g_exc = torch.zeros(T, dtype=torch.float32, device=d)
-        self.g_inh = torch.zeros(T, dtype=torch.float32, device=d)
-        self.counts = torch.zeros(4 * T, dtype=torch.int32, device=d)
-        self.spikes = torch.zeros(T, dtype=torch.uint8, device=d)
-        self.events = torch.zeros(len(self.specs), dtype=torch.int64, device=d)
-        self.spike_count = torch.zeros(len(self.specs), dtype=torch.int32, device=d)
-        self.d_step = torch.zeros(1, dtype=torch.int64, device=d)
-        self.graph = None
+        if getattr(self, "V", None) is None:
+            self.V = torch.empty(T, dtype=torch.float32, device=d)
+            self.t_last = torch.empty(T, dtype=torch.float32, device=d)
+            self.g_exc = torch.empty(T, dtype=torch.float32, device=d)
+            self.g_inh = torch.empty(T, dtype=torch.float32, device=d)
+            self.counts = torch.empty(4 * T, dtype=torch.int32, device=d)
+            self.spikes = torch.empty(T, dtype=torch.uint8, device=d)
+            self.events = torch.empty(len(self.specs), dtype=torch.int64, device=d)
+            self.spike_count = torch.empty(len(self.specs), dtype=torch.int32, device=d)
+            self.d_step = torch.empty(1, dtype=torch.int64, device=d)
+            self.graph = None
+        self.V.copy_(torch.from_numpy(self.V0))
+        self.t_last.fill_(-1e7)
+        self.g_exc.zero_()
+        self.g_inh.zero_()
+        self.counts.zero_()
+        self.spikes.zero_()
+        self.events.zero_()
+        self.spike_count.zero_()
+        self.d_step.zero_()
         self.i = 0
 
     def step(self, spikes_out=None, count=True, device_step=False):
@@ -221,3 +232,75 @@ class BatchedCOBA:
     def net_slice(self, k):
         off = int(self.descs[k].off)
         return slice(off, off + self.sizes[k])
+
+
+class PlainCOBA:
+    """One jitconn COBA network stepped operator by operator, the way the
+    reference's step executes: four ``bm.jitconn.event_mv_prob_homo``
+    projections (outdim_parallel=False) into fresh outputs, ``g += out``,
+    then the E and I population updates -- ~14 launches per step.  FP32
+    weights unless ``bf16``.  Used as the per-network baseline of the
+    batched path; its raster equals the batched one."""
+
+    def __init__(self, spec, V_init=None, bf16=False, dt=0.1, ltc=True, V_rest=-60.0, V_th=-50.0, V_reset=-60.0,
+                 tau=20.0, tau_ref=5.0, R=1.0, tau_e=5.0, tau_i=10.0, E_e=0.0, E_i=-80.0):
+        self.dev = _lib.device()
+        ne, ni = int(spec["n_exc"]), int(spec["n_inh"])
+        self.ne, self.ni = ne, ni
+        N = ne + ni
+        w_e, w_i = float(spec["w_exc"]), float(spec["w_inh"])
+        if bf16:
+            w_e, w_i = bf16_round(w_e), bf16_round(w_i)
+        self.clen = conn_len(min(float(spec["in_degree"]) / N, 1.0))
+        base = int(spec["seed"])
+        self.projs = [(0, "E", "E", 0, w_e), (1, "E", "I", 0, w_e), (2, "I", "E", 1, w_i), (3, "I", "I", 1, w_i)]
+        self.seeds = [(base * 4 + q) & 0xFFFFFFFF for q in range(4)]
+        self.I_ext = float(spec["I_ext"])
+        if V_init is None:
+            rng = np.random.Generator(np.random.PCG64(base + 7))
+            V_init = rng.uniform(V_rest, V_th, N).astype(np.float32)
+        V_init = np.asarray(V_init, np.float32)
+        d = self.dev
+        self.size = {"E": ne, "I": ni}
+        self.V = {"E": torch.from_numpy(V_init[:ne].copy()).to(d), "I": torch.from_numpy(V_init[ne:].copy()).to(d)}
+        self.t_last = {p: torch.full((n,), -1e7, dtype=torch.float32, device=d) for p, n in self.size.items()}
+        self.spike = {p: torch.zeros(n, dtype=torch.uint8, device=d) for p, n in self.size.items()}
+        self.g = {p: [torch.zeros(n, dtype=torch.float32, device=d) for _ in range(2)] for p, n in self.size.items()}
+        self.out = {p: torch.empty(n, dtype=torch.float32, device=d) for p, n in self.size.items()}
+        self.tau_g = (ctypes.c_float * 2)(tau_e, tau_i)
+        self.E_g = (ctypes.c_float * 2)(E_e, E_i)
+        self.c_g = (ctypes.c_float * 2)(float(expon_coef(tau_e, dt)), float(expon_coef(tau_i, dt)))
+        self.params = (V_rest, V_th, V_reset, tau, tau_ref, R)
+        self.dt = np.float32(dt)
+        self.ltc = int(ltc)
+        self.i = 0
+
+    def step(self):
+        L = _lib.lib()
+        s = _lib.stream_ptr()
+        for q, pre, post, kind, w in self.projs:
+            out = self.out[post]
+            rc = L.bpx_jitconn_mv(s, _lib.ptr(self.spike[pre]), _lib.BPX_BOOL, 1, _lib.DIST_HOMO, w, 0.0, self.clen,
+                                  self.seeds[q], self.size[post], self.size[pre], 0, _lib.LAYOUT_LANE32,
+                                  _lib.ptr(out))
+            _lib.check(rc, "bpx_jitconn_mv")
+            self.g[post][kind].add_(out)
+        t = np.float32(self.i) * self.dt
+        for p in ("E", "I"):
+            gp = (ctypes.c_void_p * 2)(self.g[p][0].data_ptr(), self.g[p][1].data_ptr())
+            rc = L.bpx_lif_coba_step(s, self.size[p], self.ltc, 2, _lib.ptr(self.V[p]), _lib.ptr(self.t_last[p]),
+                                     _lib.ptr(self.spike[p]), ctypes.cast(gp, ctypes.POINTER(ctypes.c_void_p)),
+                                     ctypes.cast(self.tau_g, ctypes.c_void_p), ctypes.cast(self.E_g, ctypes.c_void_p),
+                                     ctypes.cast(self.c_g, ctypes.c_void_p), self.I_ext, None, *self.params, t,
+                                     self.dt)
+            _lib.check(rc, "bpx_lif_coba_step")
+        self.i += 1
+
+    def run(self, steps, raster=False):
+        out = torch.empty((steps, self.ne + self.ni), dtype=torch.uint8, device=self.dev) if raster else None
+        for k in range(steps):
+            self.step()
+            if raster:
+                out[k, :self.ne] = self.spike["E"]
+                out[k, self.ne:] = self.spike["I"]
+        return out

@@ -217,8 +217,9 @@ float bpx_expon_coef(float tau, float dt);
  * jitconn/_event_matvec_taichi.py:92-123) and the population updates
  * (dyn/neurons/lif.py:461-500) in the reference order.
  * State pools are indexed by off + local neuron id (E first, then I);
- * counts is [2][2][total] uint32, zero at start.  The step index (time
- * t = step*dt, count buffer parity) is `step`, or *d_step when d_step is a
+ * counts is [2][2][total] uint32, zero at start (and zero in a network's
+ * range when it joins).  The step index (count buffer parity; network time
+ * t = (step - t0_step) * dt) is `step`, or *d_step when d_step is a
  * device counter -- the form for HIP graphs, where bpx_net_advance(d_step)
  * after each step makes every replay continue in time.
  * tiles: device array of (network, first local neuron) int32 pairs, one per
@@ -239,7 +240,7 @@ typedef struct {
   uint32_t div[4][4];        /* filled by bpx_net_prepare */
   float w_exc, w_inh;        /* homogeneous weights */
   float I_ext;               /* constant input */
-  int32_t reserved;
+  int32_t t0_step;           /* step index at which the network started (t = (step - t0_step) * dt) */
   const int32_t* indptr;     /* CSR networks: [N+1] */
   const int32_t* indices;    /* CSR networks: local post ids */
 } bpx_net_desc;

@@ -31,6 +31,12 @@
 #include <stdlib.h>
 #include <string.h>
 
+float orc_expon_coef(float tau, float dt);
+int orc_lif_coba_step(int64_t n, int ltc, int n_g, float* V, float* t_last, uint8_t* spike, float** g,
+                      const float* tau_g, const float* E_g, const float* c_g, float I_ext, const float* I_vec,
+                      float V_rest, float V_th, float V_reset, float tau, float tau_ref, float R, float t,
+                      float dt);
+
 /* dtype codes shared with include/bpx.h */
 enum {
   DT_BOOL = 0, DT_F32 = 1, DT_F64 = 2, DT_I32 = 3, DT_I64 = 4, DT_BF16 = 5,
@@ -671,4 +677,51 @@ int orc_lif_coba_step_f64(int64_t n, int ltc, int n_g,
     spike[i] = (uint8_t)sp;
   }
   return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Whole E/I COBA network run (f32), the CPU restatement of the reference    */
+/* step loop (examples/dynamics_simulation/COBA.py:58-109 with DSRunner,     */
+/* runners.py:619-643): per step, every projection's serial event csrmv      */
+/* scatter into a zeroed output (event/_csr_matvec.py:321-340), g += out,   */
+/* then the E and I neuron updates.  The CSR is the per-neuron combined      */
+/* form (row p lists network-local targets; targets of E neurons feed g_exc, */
+/* of I neurons g_inh).  Homogeneous weights: the scatter output of a target */
+/* receiving c spikes is ((0 + w) + w) + ... (c terms), computed as such.    */
+/* raster (nullable) receives steps x N spike flags; returns total spikes.   */
+/* ------------------------------------------------------------------------ */
+int64_t orc_coba_net_run(int32_t n_exc, int32_t n_inh, const int32_t* indptr, const int32_t* indices,
+                         float w_exc, float w_inh, float I_ext, const float* V0, int64_t steps, int ltc,
+                         float V_rest, float V_th, float V_reset, float tau, float tau_ref, float R,
+                         float tau_e, float tau_i, float E_e, float E_i, float dt, uint8_t* raster) {
+  const int32_t N = n_exc + n_inh;
+  float* V = (float*)malloc(sizeof(float) * N);
+  float* tl = (float*)malloc(sizeof(float) * N);
+  float* ge = (float*)calloc(N, sizeof(float));
+  float* gi = (float*)calloc(N, sizeof(float));
+  float* oute = (float*)calloc(N, sizeof(float));
+  float* outi = (float*)calloc(N, sizeof(float));
+  uint8_t* sp = (uint8_t*)calloc(N, 1);
+  int64_t total = 0;
+  const float tau_g[2] = {tau_e, tau_i}, E_g[2] = {E_e, E_i};
+  const float c_g[2] = {orc_expon_coef(tau_e, dt), orc_expon_coef(tau_i, dt)};
+  for (int32_t j = 0; j < N; ++j) { V[j] = V0[j]; tl[j] = -1e7f; }
+  for (int64_t i = 0; i < steps; ++i) {
+    const float t = (float)i * dt;
+    for (int32_t j = 0; j < N; ++j) { oute[j] = 0.f; outi[j] = 0.f; }
+    for (int32_t p = 0; p < N; ++p) {
+      if (!sp[p]) continue;
+      float* o = p < n_exc ? oute : outi;
+      const float w = p < n_exc ? w_exc : w_inh;
+      for (int32_t q = indptr[p]; q < indptr[p + 1]; ++q) o[indices[q]] += w;
+    }
+    for (int32_t j = 0; j < N; ++j) { ge[j] = ge[j] + oute[j]; gi[j] = gi[j] + outi[j]; }
+    float* gp[2] = {ge, gi};
+    orc_lif_coba_step(N, ltc, 2, V, tl, sp, gp, tau_g, E_g, c_g, I_ext, NULL, V_rest, V_th, V_reset, tau, tau_ref,
+                      R, t, dt);
+    for (int32_t j = 0; j < N; ++j) total += sp[j];
+    if (raster) memcpy(raster + i * N, sp, N);
+  }
+  free(V); free(tl); free(ge); free(gi); free(oute); free(outi); free(sp);
+  return total;
 }

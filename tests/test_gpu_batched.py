@@ -109,3 +109,38 @@ def test_csr_network_vs_eicoba_oracle():
     ref = net.run(100).cpu().numpy()
     assert ref.sum() > 0
     np.testing.assert_array_equal(raster, ref)
+
+
+def test_plain_path_equals_batched():
+    from brainpy_amd.dyn.batched import PlainCOBA
+    specs = _jit_specs([1500, 3000], seed0=11)
+    b = BatchedCOBA(specs, bf16=True)
+    rb = b.run(80, raster=True).cpu().numpy()
+    for k in range(2):
+        p = PlainCOBA(specs[k], V_init=b.V0[b.net_slice(k)], bf16=True)
+        rp = p.run(80, raster=True).cpu().numpy()
+        np.testing.assert_array_equal(rb[:, b.net_slice(k)], rp)
+
+
+def test_served_alone_equals_in_mix():
+    from brainpy_amd.dyn.serve import Request, SimServer
+    rng = np.random.default_rng(5)
+
+    def make(rid, arrival):
+        n = int(np.exp(rng.uniform(np.log(4000), np.log(40000))))
+        return Request(rid, n, seed=1000 + rid, w_exc=rng.uniform(0.3, 0.9), w_inh=rng.uniform(4, 9),
+                       I_ext=rng.uniform(15, 25), steps=int(rng.integers(50, 300)), arrival=arrival,
+                       keep_state=True)
+
+    reqs = [make(k, 0.002 * k) for k in range(12)]
+    specs = [(r.n, r.seed, r.w_exc, r.w_inh, r.I_ext, r.steps) for r in reqs]
+    srv = SimServer(capacity=1 << 17)   # smaller than the sum of the requests: some queue
+    done = {r.rid: r for r in srv.serve(reqs)}
+    assert len(done) == 12 and sum(r.spikes for r in done.values()) > 0
+    # admission respected the budget, so some requests waited
+    for k in (0, 5, 11):
+        n, seed, we, wi, I, steps = specs[k]
+        alone = SimServer(capacity=1 << 16)
+        (r,) = alone.serve([Request(k, n, seed, we, wi, I, steps, keep_state=True)])
+        assert r.spikes == done[k].spikes and r.events == done[k].events
+        np.testing.assert_array_equal(r.V_final, done[k].V_final)
