@@ -270,6 +270,19 @@ def bench_served(_lib, torch, n_requests=80, rate=50.0, seed=7):
     return r
 
 
+def measured_traffic(name="event_scatter_100k_p0.01_rate10_heter_f32.md"):
+    """HBM bytes per launch of the headline kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,6 +292,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the jitconn and network legs")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run 3 plain device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs on a single MI355X and "
@@ -367,6 +382,13 @@ def main():
 
     global TILES
     TILES = plans["heter_f32"].plan.tiles
+    if args.copy_mib:
+        src = torch.empty(args.copy_mib << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            L.bpx_copy(sp, _lib.ptr(src), _lib.ptr(dst), src.numel())
+        torch.cuda.synchronize()
+        del src, dst
     # the tiled path must agree with the plain CSR path
     for kind in ("homo", "heter_f32"):
         call_csr(kind, HEADLINE[1])
@@ -446,7 +468,7 @@ def main():
                    "path": "tiled CSR plan (built once before timing)",
                    "active_rows": head["active_rows"], "events_per_step": head["events_per_call"]},
         "roofline": {"bound": "hbm", "achieved": head["kernel_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": head["kernel_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": head["kernel_GBs"] / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_tiled_scatter (heter f32, bool events)", "kernel_us": head["kernel_us"],
                      "algorithmic_bytes_per_launch": head["algorithmic_bytes"]},
         "plain_csr_path": {"events_per_s": plain["events_per_s"], "ms_per_step": plain["ms_per_step"],
