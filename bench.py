@@ -69,13 +69,17 @@ def bench_jitconn(L, _lib, torch, steps, warmup):
     out = torch.empty(n, dtype=torch.float32, device=dev)
     clen = conn_len(1e-4)
     sp = _lib.stream_ptr()
+    wsb = int(L.bpx_jitconn_workspace(n, n))
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
     res = []
     for od in (1, 0):
         # exact connection counts of this layout: M @ ones (all streams) and
         # M @ events (streams of active columns, for the scatter form)
-        L.bpx_jitconn_mv(sp, _lib.ptr(ones), _lib.BPX_F32, 0, 0, 1.0, 0.0, clen, 42, n, n, od, 0, _lib.ptr(out))
+        L.bpx_jitconn_mv(sp, _lib.ptr(ones), _lib.BPX_F32, 0, 0, 1.0, 0.0, clen, 42, n, n, od, 0, _lib.ptr(out),
+                         None, 0)
         conns_all = float(out.double().sum())
-        L.bpx_jitconn_mv(sp, _lib.ptr(ev), _lib.BPX_BOOL, 1, 0, 1.0, 0.0, clen, 42, n, n, od, 0, _lib.ptr(out))
+        L.bpx_jitconn_mv(sp, _lib.ptr(ev), _lib.BPX_BOOL, 1, 0, 1.0, 0.0, clen, 42, n, n, od, 0, _lib.ptr(out),
+                         _lib.ptr(ws), wsb)
         events = float(out.double().sum())       # synaptic events of this call
         n_active = int(ev.sum())
         streams = n * 32 if od else n_active * 32
@@ -83,7 +87,7 @@ def bench_jitconn(L, _lib, torch, steps, warmup):
         for dist, (w0, w1), extra in ((0, (1.0, 0.0), 0), (1, (0.0, 1.0), 1), (2, (0.5, 0.1), 2)):
             def call():
                 rc = L.bpx_jitconn_mv(sp, _lib.ptr(ev), _lib.BPX_BOOL, 1, dist, w0, w1, clen, 42, n, n, od, 0,
-                                      _lib.ptr(out))
+                                      _lib.ptr(out), _lib.ptr(ws), wsb)
                 if rc != 0:
                     raise RuntimeError(f"bpx_jitconn_mv failed ({rc})")
             for _ in range(warmup):
@@ -398,6 +402,15 @@ def main():
         err = float(((out - ref).abs() / ref.abs().clamp_min(1.0)).max())
         if err > 1e-5:
             raise RuntimeError(f"tiled path disagrees with the CSR path ({kind}: {err})")
+    # BF16-stored heterogeneous weights: relative error of the scattered
+    # current against the FP32-weight result (itself within ~1e-7 of FP64)
+    call_tiled("heter_f32", HEADLINE[1])
+    ref32 = out.double().clone()
+    call_tiled("heter_bf16", HEADLINE[1])
+    torch.cuda.synchronize()
+    big = ref32.abs() > 1.0
+    bf16_err = float(((out.double() - ref32).abs()[big] / ref32.abs()[big]).max())
+    bf16_err_mean = float(((out.double() - ref32).abs()[big] / ref32.abs()[big]).mean())
     sweep = []
     if not args.no_sweep:
         for path in ("csr", "tiled"):
@@ -471,6 +484,8 @@ def main():
                      "frac": head["kernel_GBs"] / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_tiled_scatter (heter f32, bool events)", "kernel_us": head["kernel_us"],
                      "algorithmic_bytes_per_launch": head["algorithmic_bytes"]},
+        "bf16_weights_rel_err": {"max": bf16_err, "mean": bf16_err_mean,
+                                  "against": "FP32-weight result, outputs with |y| > 1"},
         "plain_csr_path": {"events_per_s": plain["events_per_s"], "ms_per_step": plain["ms_per_step"],
                            "kernel_us": plain["kernel_us"], "kernel_GBs": plain["kernel_GBs"],
                            "kernel": "k_csr_scatter (stateless bm.event.csrmv)"},

@@ -50,8 +50,9 @@ SIGNATURES = {
     "bpx_event_csrmv_fixed_order": ([_vp, _vp, _i32, _i64, _vp, _vp, _i64, _vp, _i32, _i32, _i64, _i64, _i32, _vp],
                                     _i32),
     "bpx_event_csrmv_add": ([_vp, _vp, _i32, _i64, _vp, _vp, _i32, _i64, _vp, _i32, _i32, _i64, _i64, _vp], _i32),
-    "bpx_jitconn_mv": ([_vp, _vp, _i32, _i32, _i32, _f32, _f32, ctypes.c_int32, _u32, _i64, _i64, _i32, _i32, _vp],
-                       _i32),
+    "bpx_jitconn_mv": ([_vp, _vp, _i32, _i32, _i32, _f32, _f32, ctypes.c_int32, _u32, _i64, _i64, _i32, _i32, _vp,
+                        _vp, _sz], _i32),
+    "bpx_jitconn_workspace": ([_i64, _i64], _sz),
     "bpx_lif_coba_step": ([_vp, _i64, _i32, _i32, _vp, _vp, _vp, ctypes.POINTER(_vp), _vp, _vp, _vp, _f32, _vp,
                            _f32, _f32, _f32, _f32, _f32, _f32, _f32, _f32], _i32),
     "bpx_copy": ([_vp, _vp, _vp, _i64], _i32),

@@ -23,6 +23,7 @@
 #include "bpx_common.h"
 #include "bpx_rng.h"
 
+
 namespace bpx {
 
 enum { VM_DENSE = 0, VM_BOOL = 1, VM_FLOAT = 2 };
@@ -141,6 +142,86 @@ __global__ __launch_bounds__(256) void k_jit_lane32_scatter(const void* __restri
 }
 
 // ---------------------------------------------------------------------------
+// lane32, outdim_parallel, bool events, event bitmask in LDS.
+//
+// Events are packed once into a bitmask (k_pack_bits); every block of this
+// persistent kernel copies the bitmask into LDS (1M columns = 125 KB of the
+// 160 KB gfx950 gives a block) so the random event test of each generated
+// column is an LDS read instead of a scattered global byte load.  Each
+// thread runs the streams of KS rows for one lane, one after another, which
+// evens out the per-lane draw counts a wave waits on; the 32 lanes of each
+// row are then summed with a 32-wide shuffle reduction.
+// ---------------------------------------------------------------------------
+constexpr int kJitThreads = 1024;
+constexpr int kJitKS = 4;                       // rows per thread (per lane)
+constexpr int kRowsPerPass = kJitThreads / 32 * kJitKS;
+
+__global__ __launch_bounds__(256) void k_pack_bits(const uint8_t* __restrict__ ev, int64_t n,
+                                                   uint32_t* __restrict__ bits) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool on = i < n && ev[i] != 0;
+  const uint64_t mask = __ballot(on);
+  const int64_t w = i >> 5;
+  if ((threadIdx.x & 63) == 0 && (w << 5) < n) bits[w] = (uint32_t)mask;
+  if ((threadIdx.x & 63) == 32 && (w << 5) < n) bits[w] = (uint32_t)(mask >> 32);
+}
+
+__device__ __forceinline__ float repeat_sum_f(uint32_t c, float w) {
+  float r = 0.f;
+  for (uint32_t k = 0; k < c; ++k) r += w;
+  return r;
+}
+
+template <int DIST, bool Q24>
+__global__ __launch_bounds__(kJitThreads) void k_jit_lane32_outdim_bits(const uint32_t* __restrict__ gbits,
+                                                                        int64_t nwords, float w0, float w1, DivU32 dv,
+                                                                        uint32_t seed, int64_t num_row,
+                                                                        int64_t num_col, uint32_t step,
+                                                                        float* __restrict__ out) {
+  extern __shared__ uint32_t s_bits[];
+  for (int64_t k = threadIdx.x; k < nwords; k += kJitThreads) s_bits[k] = gbits[k];
+  __syncthreads();
+  const uint32_t l = threadIdx.x & 31;
+  const int half = threadIdx.x >> 5;            // 32 halves of 32 lanes
+  uint32_t c0 = step * l - 1u;
+  uint32_t end = c0 + step;
+  if ((int64_t)end > num_col) end = (uint32_t)num_col;
+  for (int64_t base = (int64_t)blockIdx.x * kRowsPerPass; base < num_row; base += (int64_t)gridDim.x * kRowsPerPass) {
+    float acc[kJitKS];
+#pragma unroll
+    for (int k = 0; k < kJitKS; ++k) {
+      const int64_t row = base + (int64_t)half * kJitKS + k;
+      acc[k] = 0.f;
+      uint32_t hits = 0;
+      if (row < num_row) {
+        Lfsr88 key(seed + (uint32_t)(row * 32) + l);
+        uint32_t c = c0 + rand_inc_u<Q24>(key, dv);
+        while (c < end) {
+          const uint32_t bit = (s_bits[c >> 5] >> (c & 31)) & 1u;
+          if constexpr (DIST == BPX_DIST_HOMO) {
+            hits += bit;
+          } else {
+            // r += w * e with e in {0, 1}: w * 1 = w and w * 0 adds +0
+            const float w = draw_weight<DIST>(key, w0, w1);
+            acc[k] += bit ? w : 0.f;
+          }
+          c += rand_inc_u<Q24>(key, dv);
+        }
+      }
+      // the reference lane adds weight0 * event per connection: a run of
+      // `hits` additions of weight0 (zero terms leave the sum unchanged)
+      if constexpr (DIST == BPX_DIST_HOMO) acc[k] = repeat_sum_f(hits, w0);
+    }
+#pragma unroll
+    for (int k = 0; k < kJitKS; ++k) {
+      const float v = group_sum<32>(acc[k]);
+      const int64_t row = base + (int64_t)half * kJitKS + k;
+      if (l == 0 && row < num_row) out[row] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // row layout
 // ---------------------------------------------------------------------------
 template <int DIST, int VM>
@@ -199,13 +280,48 @@ __global__ __launch_bounds__(256) void k_zero_f32(float* __restrict__ out, int64
   for (; i < n; i += stride) out[i] = 0.f;
 }
 
+static int jit_cus() {
+  static int c = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) return 256;
+    return v;
+  }();
+  return c;
+}
+
+static bool bits_fit(int64_t nwords) {
+  static int shm = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 65536;
+    return v;
+  }();
+  return nwords * 4 <= shm;
+}
+
 template <int DIST, int VM>
 static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t clen, uint32_t seed, int64_t num_row,
-                   int64_t num_col, int outdim, int layout, float* out) {
+                   int64_t num_col, int outdim, int layout, float* out, void* ws, size_t ws_bytes) {
   const DivU32 dv = DivU32::make((uint32_t)clen);
   const uint32_t step = (uint32_t)(((num_row + 1) >> 5) > 1 ? ((num_row + 1) >> 5) : 1);
   if (outdim) {
-    if (layout == BPX_LAYOUT_LANE32) {
+    const int64_t nwords = (num_col + 31) / 32;
+    if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && DIST != BPX_DIST_NORMAL && bits_fit(nwords) && ws &&
+        ws_bytes >= (size_t)nwords * 4) {
+      uint32_t* bits = (uint32_t*)ws;
+      hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((num_col + 255) / 256)), dim3(256), 0, s, (const uint8_t*)vec,
+                         num_col, bits);
+      int64_t blocks = (num_row + kRowsPerPass - 1) / kRowsPerPass;
+      const int64_t maxb = (int64_t)jit_cus() * 1;
+      if (blocks > maxb) blocks = maxb;
+      if (dv.small_quotient())
+        BPX_LAUNCH(KID_JIT_OUTDIM, (k_jit_lane32_outdim_bits<DIST, true>), dim3((unsigned)blocks), dim3(kJitThreads),
+                   (size_t)nwords * 4, s, (const uint32_t*)bits, nwords, w0, w1, dv, seed, num_row, num_col, step, out);
+      else
+        BPX_LAUNCH(KID_JIT_OUTDIM, (k_jit_lane32_outdim_bits<DIST, false>), dim3((unsigned)blocks), dim3(kJitThreads),
+                   (size_t)nwords * 4, s, (const uint32_t*)bits, nwords, w0, w1, dv, seed, num_row, num_col, step, out);
+    } else if (layout == BPX_LAYOUT_LANE32) {
       const int64_t blocks = (num_row * 32 + 255) / 256;
       BPX_LAUNCH(KID_JIT_OUTDIM, (k_jit_lane32_outdim<DIST, VM>), dim3((unsigned)blocks), dim3(256), 0, s, vec, w0,
                  w1, dv, seed, num_row, num_col, step, out);
@@ -234,19 +350,28 @@ static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t c
 
 template <int DIST>
 static int run_jit_vm(int vm, hipStream_t s, const void* vec, float w0, float w1, int32_t clen, uint32_t seed,
-                      int64_t num_row, int64_t num_col, int outdim, int layout, float* out) {
-  if (vm == VM_DENSE) return run_jit<DIST, VM_DENSE>(s, vec, w0, w1, clen, seed, num_row, num_col, outdim, layout, out);
-  if (vm == VM_BOOL) return run_jit<DIST, VM_BOOL>(s, vec, w0, w1, clen, seed, num_row, num_col, outdim, layout, out);
-  return run_jit<DIST, VM_FLOAT>(s, vec, w0, w1, clen, seed, num_row, num_col, outdim, layout, out);
+                      int64_t num_row, int64_t num_col, int outdim, int layout, float* out, void* ws,
+                      size_t wsb) {
+  if (vm == VM_DENSE)
+    return run_jit<DIST, VM_DENSE>(s, vec, w0, w1, clen, seed, num_row, num_col, outdim, layout, out, ws, wsb);
+  if (vm == VM_BOOL)
+    return run_jit<DIST, VM_BOOL>(s, vec, w0, w1, clen, seed, num_row, num_col, outdim, layout, out, ws, wsb);
+  return run_jit<DIST, VM_FLOAT>(s, vec, w0, w1, clen, seed, num_row, num_col, outdim, layout, out, ws, wsb);
 }
 
 }  // namespace bpx
 
 using namespace bpx;
 
+extern "C" size_t bpx_jitconn_workspace(int64_t num_row, int64_t num_col) {
+  (void)num_row;
+  if (num_col <= 0) return 0;
+  return (size_t)((num_col + 31) / 32) * 4;
+}
+
 extern "C" int bpx_jitconn_mv(void* stream, const void* vec, int vec_dtype, int event, int dist, float w0, float w1,
                               int32_t clen, uint32_t seed, int64_t num_row, int64_t num_col, int outdim_parallel,
-                              int layout, float* out) {
+                              int layout, float* out, void* workspace, size_t workspace_bytes) {
   if (num_row < 0 || num_col < 0) return BPX_ERR_SHAPE;
   if (num_row > INT32_MAX || num_col > INT32_MAX || num_row * 32 > (int64_t)UINT32_MAX ||
       num_col * 32 > (int64_t)UINT32_MAX)
@@ -269,8 +394,8 @@ extern "C" int bpx_jitconn_mv(void* stream, const void* vec, int vec_dtype, int 
   if (num_row == 0) return BPX_OK;
   const int o = outdim_parallel ? 1 : 0;
   switch (dist) {
-    case BPX_DIST_HOMO: return run_jit_vm<BPX_DIST_HOMO>(vm, s, vec, w0, w1, clen, seed, num_row, num_col, o, layout, out);
-    case BPX_DIST_UNIFORM: return run_jit_vm<BPX_DIST_UNIFORM>(vm, s, vec, w0, w1, clen, seed, num_row, num_col, o, layout, out);
-    default: return run_jit_vm<BPX_DIST_NORMAL>(vm, s, vec, w0, w1, clen, seed, num_row, num_col, o, layout, out);
+    case BPX_DIST_HOMO: return run_jit_vm<BPX_DIST_HOMO>(vm, s, vec, w0, w1, clen, seed, num_row, num_col, o, layout, out, workspace, workspace_bytes);
+    case BPX_DIST_UNIFORM: return run_jit_vm<BPX_DIST_UNIFORM>(vm, s, vec, w0, w1, clen, seed, num_row, num_col, o, layout, out, workspace, workspace_bytes);
+    default: return run_jit_vm<BPX_DIST_NORMAL>(vm, s, vec, w0, w1, clen, seed, num_row, num_col, o, layout, out, workspace, workspace_bytes);
   }
 }

@@ -59,6 +59,16 @@ struct DivU32 {
     return (t + ((n - t) >> s1)) >> s2;
   }
   BPX_HD uint32_t mod(uint32_t n) const { return n - div(n) * d; }
+  // same result when 256 < d < 2^24 (then q = n / d < 2^24 and q * d fits the
+  // full-rate 24-bit multiplier)
+  BPX_HD uint32_t mod24(uint32_t n) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return n - __umul24(div(n), d);
+#else
+    return n - div(n) * d;
+#endif
+  }
+  BPX_HD bool small_quotient() const { return d > 256u && d < (1u << 24); }
 };
 
 // lfsr88_random_integers(key, 0, clen-1) and (key, 1, clen) (tifunc.py:136-147):
@@ -66,5 +76,9 @@ struct DivU32 {
 // kernels, so both reduce to bits % clen plus low.
 BPX_HD int32_t rand_first(Lfsr88& k, const DivU32& dv) { return (int32_t)dv.mod(k.next()); }
 BPX_HD int32_t rand_inc(Lfsr88& k, const DivU32& dv) { return (int32_t)(dv.mod(k.next()) + 1u); }
+template <bool Q24>
+BPX_HD uint32_t rand_inc_u(Lfsr88& k, const DivU32& dv) {
+  return (Q24 ? dv.mod24(k.next()) : dv.mod(k.next())) + 1u;
+}
 
 }  // namespace bpx

@@ -282,7 +282,7 @@ class PlainCOBA:
             out = self.out[post]
             rc = L.bpx_jitconn_mv(s, _lib.ptr(self.spike[pre]), _lib.BPX_BOOL, 1, _lib.DIST_HOMO, w, 0.0, self.clen,
                                   self.seeds[q], self.size[post], self.size[pre], 0, _lib.LAYOUT_LANE32,
-                                  _lib.ptr(out))
+                                  _lib.ptr(out), None, 0)
             _lib.check(rc, "bpx_jitconn_mv")
             self.g[post][kind].add_(out)
         t = np.float32(self.i) * self.dt

@@ -96,9 +96,12 @@ def _run(vector, event, dist, w0, w1, conn_prob, seed, shape, transpose, outdim_
         raise ValueError(f'layout must be one of {tuple(_LAYOUTS)}, but we got {layout}.')
     clen = conn_len(conn_prob)
     out = torch.empty(num_row, dtype=torch.float32, device=v.device)
-    rc = _lib.lib().bpx_jitconn_mv(_lib.stream_ptr(), _lib.ptr(v), _lib.bpx_dtype(v), int(event), dist,
-                                   np.float32(w0), np.float32(w1), clen, _seed(seed), num_row, num_col,
-                                   int(outdim_parallel), _LAYOUTS[layout], _lib.ptr(out))
+    L = _lib.lib()
+    wsb = int(L.bpx_jitconn_workspace(num_row, num_col)) if (event and outdim_parallel) else 0
+    ws = torch.empty(max(wsb, 4), dtype=torch.uint8, device=v.device) if wsb else None
+    rc = L.bpx_jitconn_mv(_lib.stream_ptr(), _lib.ptr(v), _lib.bpx_dtype(v), int(event), dist,
+                          np.float32(w0), np.float32(w1), clen, _seed(seed), num_row, num_col,
+                          int(outdim_parallel), _LAYOUTS[layout], _lib.ptr(out), _lib.ptr(ws), wsb)
     _lib.check(rc, "bpx_jitconn_mv")
     return out
 

@@ -176,10 +176,15 @@ int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* e
  *   dist: BPX_DIST_*; (w0, w1) = weight | (low, high) | (mu, sigma).
  *   clen = ceil(1/p)*2-1; seed as u32.
  * out: f32 [num_row].
+ * workspace (optional, bpx_jitconn_workspace bytes): with bool events and
+ * outdim_parallel the events are packed into it as a bitmask that every
+ * block keeps in LDS; NULL falls back to reading the event bytes.
  */
+size_t bpx_jitconn_workspace(int64_t num_row, int64_t num_col);
 int bpx_jitconn_mv(void* stream, const void* vec, int vec_dtype, int event,
                    int dist, float w0, float w1, int32_t clen, uint32_t seed,
-                   int64_t num_row, int64_t num_col, int outdim_parallel, int layout, float* out);
+                   int64_t num_row, int64_t num_col, int outdim_parallel, int layout, float* out,
+                   void* workspace, size_t workspace_bytes);
 
 /*
  * Fused LIF + Expon + COBA exponential-Euler step of one population (f32).

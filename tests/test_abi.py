@@ -52,10 +52,10 @@ def test_argument_validation_without_launch():
     f = lib.bpx_jitconn_mv
     f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                   ctypes.c_float, ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
-                  ctypes.c_int, ctypes.c_void_p]
-    assert f(None, None, 1, 1, 0, 1.0, 0.0, 0, 1, 10, 10, 1, 0, None) == -5       # clen < 1
-    assert f(None, None, 1, 1, 7, 1.0, 0.0, 3, 1, 10, 10, 1, 0, None) == -5       # bad dist
-    assert f(None, None, 2, 1, 0, 1.0, 0.0, 3, 1, 10, 10, 1, 0, None) == -1       # f64 events
+                  ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    assert f(None, None, 1, 1, 0, 1.0, 0.0, 0, 1, 10, 10, 1, 0, None, None, 0) == -5       # clen < 1
+    assert f(None, None, 1, 1, 7, 1.0, 0.0, 3, 1, 10, 10, 1, 0, None, None, 0) == -5       # bad dist
+    assert f(None, None, 2, 1, 0, 1.0, 0.0, 3, 1, 10, 10, 1, 0, None, None, 0) == -1       # f64 events
     g = lib.bpx_event_csrmv
     g.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                   ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
