@@ -159,7 +159,7 @@ __device__ __forceinline__ float tiled_weight(const void* __restrict__ wptr, int
   else return hw;
 }
 
-template <int WK, typename E, int G>
+template <int WK, typename E, int G, int R>
 __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* __restrict__ rowtile,
                                                                  const uint16_t* __restrict__ lidx,
                                                                  const void* __restrict__ wptr,
@@ -222,28 +222,39 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int e = g; e < cnt; e += 2 * NGRP) {
-      const int2 sa = seg[e];
-      const int2 sb = (e + NGRP < cnt) ? seg[e + NGRP] : make_int2(0, 0);
-      for (int32_t k = l8; sa.x + k < sa.y || sb.x + k < sb.y; k += 32) {
-        uint16_t la[U], lb[U];
-        float wa[U], wb[U];
+    // each group takes R listed rows per round: R*U loads of indices and of
+    // weights per lane are in flight before the LDS atomics
+    for (int e = g; e < cnt; e += R * NGRP) {
+      int2 sg[R];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int32_t ja = sa.x + k + G * u, jb2 = sb.x + k + G * u;
-          la[u] = 0xFFFF;
-          lb[u] = 0xFFFF;
-          if (ja < sa.y) { la[u] = lidx[ja]; wa[u] = tiled_weight<WK>(wptr, ja, hw); }
-          if (jb2 < sb.y) { lb[u] = lidx[jb2]; wb[u] = tiled_weight<WK>(wptr, jb2, hw); }
+      for (int q = 0; q < R; ++q) sg[q] = (e + q * NGRP < cnt) ? seg[e + q * NGRP] : make_int2(0, 0);
+      int32_t len = 0;
+#pragma unroll
+      for (int q = 0; q < R; ++q) len = max(len, sg[q].y - sg[q].x);
+      for (int32_t k = l8; k < len; k += 32) {
+        uint16_t li[R][U];
+        float wv[R][U];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int32_t j = sg[q].x + k + G * u;
+            li[q][u] = 0xFFFF;
+            if (j < sg[q].y) {
+              li[q][u] = lidx[j];
+              wv[q][u] = tiled_weight<WK>(wptr, j, hw);
+            }
+          }
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (la[u] < tlen)
-            __hip_atomic_fetch_add(s_acc + la[u], (unsigned long long)(WK == 0 ? hfx : fixed_of(wa[u], fx_scale)),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          if (lb[u] < tlen)
-            __hip_atomic_fetch_add(s_acc + lb[u], (unsigned long long)(WK == 0 ? hfx : fixed_of(wb[u], fx_scale)),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        for (int q = 0; q < R; ++q) {
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            if (li[q][u] < tlen)
+              __hip_atomic_fetch_add(s_acc + li[q][u],
+                                     (unsigned long long)(WK == 0 ? hfx : fixed_of(wv[q][u], fx_scale)),
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
         }
       }
     }
@@ -285,6 +296,16 @@ static int tiled_group_width() {
     return (v == 8 || v == 16 || v == 32) ? v : 16;
   }();
   return g;
+}
+
+// rows a lane group streams per round (BPX_TILED_ROWS overrides for tuning)
+static int tiled_rows_per_group() {
+  static int r = [] {
+    const char* e = getenv("BPX_TILED_ROWS");
+    int v = e ? atoi(e) : 2;
+    return (v == 1 || v == 2 || v == 4) ? v : 2;
+  }();
+  return r;
 }
 
 struct TiledShape {
@@ -468,12 +489,19 @@ int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* e
   const int P = plan->tiles, T = plan->tile_len;
   const int cpx = tiled_copies_per_xcd();
   const int grp = tiled_group_width();
+  const int rows = tiled_rows_per_group();
   const unsigned blocks = (unsigned)(8 * cpx * P);
   const size_t shm = (size_t)T * sizeof(unsigned long long);
   const double fx_scale = ldexp(1.0, plan->fx_shift), fx_inv = ldexp(1.0, -plan->fx_shift);
-#define TLG(WK, E, G)                                                                                         \
-  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E, G>), dim3(blocks), dim3(kTiledThreads), shm, s, rowtile, lidx, w, \
-             hw, (const E*)events, ev_rule, m, n, P, T, cpx, fx_scale, fx_inv, out)
+#define TLGR(WK, E, G, R)                                                                                     \
+  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E, G, R>), dim3(blocks), dim3(kTiledThreads), shm, s, rowtile, lidx, \
+             w, hw, (const E*)events, ev_rule, m, n, P, T, cpx, fx_scale, fx_inv, out)
+#define TLG(WK, E, G)                          \
+  do {                                         \
+    if (rows == 4) TLGR(WK, E, G, 4);          \
+    else if (rows == 1) TLGR(WK, E, G, 1);     \
+    else TLGR(WK, E, G, 2);                    \
+  } while (0)
 #define TL(WK, E)                       \
   do {                                  \
     if (grp == 16) TLG(WK, E, 16);      \
@@ -492,6 +520,7 @@ int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* e
 #undef TL_E
 #undef TL
 #undef TLG
+#undef TLGR
   BPX_CHECK_LAUNCH();
   return BPX_OK;
 }
