@@ -11,9 +11,10 @@
 //   * the rows are dealt to the 8C copies in 64-row chunks (chunk q -> copy
 //     q % 8C); inside a block each wave walks chunks of its copy, lists the
 //     active rows with their tile-p segment (row-tile offsets precomputed),
-//     and groups of 16 lanes stream the segments, accumulating at 16-bit
-//     tile-local column indices with LDS integer atomics in 64-bit fixed
-//     point (gfx950's LDS float atomics run ~20x slower than integer ones);
+//     and groups of 4 lanes stream the segments with 16-byte vector loads
+//     (segments are padded to 8 entries), accumulating at 16-bit tile-local
+//     column indices with LDS integer atomics in 64-bit fixed point (gfx950's
+//     LDS float atomics run ~20x slower than integer ones);
 //   * at the end each block adds its non-zero LDS entries into the output
 //     with coalesced global atomics: 8C copies x n/16 segments per call
 //     instead of one segment per synaptic event.
@@ -46,10 +47,16 @@ static int device_cus() {
 // build: per row, count elements per tile -> row-tile offsets; then place
 // tile-local indices (u16) and weights in tile-grouped order (stable)
 // ---------------------------------------------------------------------------
+// Segments are padded to a multiple of kVec elements so the exec kernel can
+// fetch 8 tile-local indices (16 B) and 8 weights with one vector load each;
+// padding entries carry the index 0xFFFF, which no tile reaches.
+constexpr int kVec = 8;
+
 template <typename I>
 __global__ __launch_bounds__(kBuildThreads) void k_tiled_count(const I* __restrict__ indices,
                                                                const I* __restrict__ indptr, int64_t nnz, int64_t m,
-                                                               int64_t n, int P, DivU32 dT, int32_t* __restrict__ rowtile) {
+                                                               int64_t n, int P, DivU32 dT, int32_t* __restrict__ rowtile,
+                                                               int32_t* __restrict__ rowsize) {
   __shared__ int32_t s_cnt[kBuildThreads * kMaxTiles];
   int32_t* cnt = s_cnt + threadIdx.x * kMaxTiles;
   const int64_t r = (int64_t)blockIdx.x * kBuildThreads + threadIdx.x;
@@ -63,10 +70,54 @@ __global__ __launch_bounds__(kBuildThreads) void k_tiled_count(const I* __restri
     if ((uint64_t)c < (uint64_t)n) cnt[dT.div((uint32_t)c)] += 1;
   }
   int32_t* rt = rowtile + r * (P + 1);
-  int32_t acc = (int32_t)(s > t ? t : s);
+  int32_t tot = 0;
   for (int p = 0; p < P; ++p) {
+    const int32_t padded = (cnt[p] + kVec - 1) & ~(kVec - 1);
+    rt[p] = padded;           // segment size for now; offsets after the row scan
+    tot += padded;
+  }
+  rowsize[r] = tot;
+}
+
+// exclusive scan of the padded row sizes (one block), total in rowsize[m]
+__global__ __launch_bounds__(1024) void k_scan_rows(int32_t* __restrict__ rowsize, int64_t m) {
+  __shared__ int s_w[16];
+  __shared__ int s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  const uint32_t w = threadIdx.x >> 6;
+  for (int64_t base = 0; base < m; base += 1024) {
+    const int64_t i = base + threadIdx.x;
+    const int x = i < m ? rowsize[i] : 0;
+    int v = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(v, o, 64);
+      if (lane_id() >= o) v += y;
+    }
+    if (lane_id() == 63) s_w[w] = v;
+    __syncthreads();
+    int wo = 0;
+    for (uint32_t k = 0; k < w; ++k) wo += s_w[k];
+    const int carry = s_carry;
+    if (i < m) rowsize[i] = carry + wo + v - x;
+    __syncthreads();
+    if (threadIdx.x == 1023) s_carry = carry + wo + v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rowsize[m] = s_carry;
+}
+
+__global__ __launch_bounds__(256) void k_tiled_offsets(const int32_t* __restrict__ rowbase, int64_t m, int P,
+                                                       int32_t* __restrict__ rowtile) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= m) return;
+  int32_t* rt = rowtile + r * (P + 1);
+  int32_t acc = rowbase[r];
+  for (int p = 0; p < P; ++p) {
+    const int32_t sz = rt[p];
     rt[p] = acc;
-    acc += cnt[p];
+    acc += sz;
   }
   rt[P] = acc;
 }
@@ -102,6 +153,16 @@ __global__ __launch_bounds__(kBuildThreads) void k_tiled_fill(const V* __restric
         w[pos].bits = (uint16_t)(u >> 16);
       } else {
         w[pos] = v;
+      }
+    }
+  }
+  // pad every segment to its vector boundary
+  for (int p = 0; p < P; ++p) {
+    for (int32_t pos = cur[p]; pos < rt[p + 1]; ++pos) {
+      lidx[pos] = 0xFFFF;
+      if (heter) {
+        if constexpr (sizeof(W) == 2) w[pos].bits = 0;
+        else w[pos] = 0.f;
       }
     }
   }
@@ -152,14 +213,8 @@ constexpr int kWavesPerBlock = kTiledThreads / 64;
 // weight -> 64-bit fixed point (w * 2^shift, exact for |w| >= 2^(23-shift))
 __device__ __forceinline__ long long fixed_of(float w, double scale) { return __double2ll_rn((double)w * scale); }
 
-template <int WK>
-__device__ __forceinline__ float tiled_weight(const void* __restrict__ wptr, int32_t j, float hw) {
-  if constexpr (WK == 1) return ((const float*)wptr)[j];
-  else if constexpr (WK == 2) return to_f32(((const bf16_t*)wptr)[j]);
-  else return hw;
-}
 
-template <int WK, typename E, int G, int R>
+template <int WK, typename E, int R>
 __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* __restrict__ rowtile,
                                                                  const uint16_t* __restrict__ lidx,
                                                                  const void* __restrict__ wptr,
@@ -184,9 +239,9 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
   const long long hfx = fixed_of(hw, fx_scale);
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  constexpr int NGRP = 64 / G;          // lane groups per wave
-  constexpr int U = 32 / G;             // elements per lane and row per pass
-  const int g = lane / G, l8 = lane % G;
+  constexpr int GV = 4;                 // lanes per row segment, kVec elements each
+  constexpr int NGRP = 64 / GV;         // lane groups per wave
+  const int g = lane / GV, lv = lane % GV;
   int2* seg = s_seg[wave];
   const int64_t nchunks = (m + 63) >> 6;
   const int64_t cstep = (int64_t)ncopy * kWavesPerBlock;    // chunk stride of one wave
@@ -222,8 +277,8 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // each group takes R listed rows per round: R*U loads of indices and of
-    // weights per lane are in flight before the LDS atomics
+    // each group of 4 lanes takes R listed rows per round; a lane fetches
+    // 8 indices (one 16-byte load) and 8 weights per row and round
     for (int e = g; e < cnt; e += R * NGRP) {
       int2 sg[R];
 #pragma unroll
@@ -231,29 +286,42 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
       int32_t len = 0;
 #pragma unroll
       for (int q = 0; q < R; ++q) len = max(len, sg[q].y - sg[q].x);
-      for (int32_t k = l8; k < len; k += 32) {
-        uint16_t li[R][U];
-        float wv[R][U];
+      for (int32_t k = kVec * lv; k < len; k += kVec * GV) {
+        uint4 li[R];
+        float wv[R][kVec];
 #pragma unroll
         for (int q = 0; q < R; ++q) {
+          const int32_t j = sg[q].x + k;
+          li[q] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+          if (j < sg[q].y) {
+            li[q] = *reinterpret_cast<const uint4*>(lidx + j);
+            if constexpr (WK == 1) {
+              const float4 a = *reinterpret_cast<const float4*>((const float*)wptr + j);
+              const float4 b = *reinterpret_cast<const float4*>((const float*)wptr + j + 4);
+              wv[q][0] = a.x; wv[q][1] = a.y; wv[q][2] = a.z; wv[q][3] = a.w;
+              wv[q][4] = b.x; wv[q][5] = b.y; wv[q][6] = b.z; wv[q][7] = b.w;
+            } else if constexpr (WK == 2) {
+              const uint4 h = *reinterpret_cast<const uint4*>((const bf16_t*)wptr + j);
+              const uint32_t hw4[4] = {h.x, h.y, h.z, h.w};
 #pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const int32_t j = sg[q].x + k + G * u;
-            li[q][u] = 0xFFFF;
-            if (j < sg[q].y) {
-              li[q][u] = lidx[j];
-              wv[q][u] = tiled_weight<WK>(wptr, j, hw);
+              for (int u = 0; u < 4; ++u) {
+                wv[q][2 * u] = __uint_as_float(hw4[u] << 16);
+                wv[q][2 * u + 1] = __uint_as_float(hw4[u] & 0xFFFF0000u);
+              }
             }
           }
         }
 #pragma unroll
         for (int q = 0; q < R; ++q) {
+          const uint32_t l4[4] = {li[q].x, li[q].y, li[q].z, li[q].w};
 #pragma unroll
-          for (int u = 0; u < U; ++u) {
-            if (li[q][u] < tlen)
-              __hip_atomic_fetch_add(s_acc + li[q][u],
-                                     (unsigned long long)(WK == 0 ? hfx : fixed_of(wv[q][u], fx_scale)),
-                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          for (int u = 0; u < kVec; ++u) {
+            const uint32_t c = (l4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
+            if (c < (uint32_t)tlen) {
+              const long long fx = WK == 0 ? hfx : fixed_of(wv[q][u], fx_scale);
+              __hip_atomic_fetch_add(s_acc + c, (unsigned long long)fx, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
           }
         }
       }
@@ -288,22 +356,12 @@ static int tiled_copies_per_xcd() {
   return c;
 }
 
-// lanes cooperating on one row segment (16 by default; BPX_TILED_GROUP overrides for tuning)
-static int tiled_group_width() {
-  static int g = [] {
-    const char* e = getenv("BPX_TILED_GROUP");
-    int v = e ? atoi(e) : 16;
-    return (v == 8 || v == 16 || v == 32) ? v : 16;
-  }();
-  return g;
-}
-
 // rows a lane group streams per round (BPX_TILED_ROWS overrides for tuning)
 static int tiled_rows_per_group() {
   static int r = [] {
     const char* e = getenv("BPX_TILED_ROWS");
-    int v = e ? atoi(e) : 2;
-    return (v == 1 || v == 2 || v == 4) ? v : 2;
+    int v = e ? atoi(e) : 1;
+    return (v == 1 || v == 2 || v == 4) ? v : 1;
   }();
   return r;
 }
@@ -362,11 +420,17 @@ int bpx_tiled_csr_layout(int64_t m, int64_t n, int64_t nnz, int weight_dtype, bp
     off += (b + 255) & ~(size_t)255;
     return o;
   };
+  // every (row, tile) segment may gain up to kVec-1 padding entries
+  const int64_t cap = nnz + (int64_t)(kVec - 1) * m * ts.P;
+  if (cap > INT32_MAX) return BPX_ERR_SHAPE;
+  plan->capacity = cap;
   size_t o_rt = take((size_t)m * (ts.P + 1) * sizeof(int32_t));
-  size_t o_li = take((size_t)nnz * sizeof(uint16_t));
+  size_t o_aux = take((size_t)(m + 1) * sizeof(int32_t));
+  size_t o_li = take((size_t)cap * sizeof(uint16_t));
   size_t wb = weight_dtype == BPX_F32 ? 4 : weight_dtype == BPX_BF16 ? 2 : 0;
-  size_t o_w = take((size_t)nnz * wb);
+  size_t o_w = take((size_t)cap * wb);
   size_t o_h = take(2 * sizeof(float));
+  plan->aux_offset = (int64_t)o_aux;
   plan->rowtile_offset = (int64_t)o_rt;
   plan->lidx_offset = (int64_t)o_li;
   plan->weights_offset = (int64_t)o_w;
@@ -392,6 +456,7 @@ int bpx_tiled_csr_build(void* stream, const void* data, int data_dtype, int64_t 
   hipStream_t s = (hipStream_t)stream;
   char* base = (char*)mem;
   int32_t* rowtile = (int32_t*)(base + plan->rowtile_offset);
+  int32_t* aux = (int32_t*)(base + plan->aux_offset);
   uint16_t* lidx = (uint16_t*)(base + plan->lidx_offset);
   void* w = base + plan->weights_offset;
   float* hw = (float*)(base + plan->homo_offset);
@@ -429,7 +494,10 @@ int bpx_tiled_csr_build(void* stream, const void* data, int data_dtype, int64_t 
 #define BUILD_IDX(I)                                                                                           \
   do {                                                                                                         \
     hipLaunchKernelGGL(k_tiled_count<I>, dim3(blocks), dim3(kBuildThreads), 0, s, (const I*)indices,          \
-                       (const I*)indptr, nnz, m, n, P, dT, rowtile);                                           \
+                       (const I*)indptr, nnz, m, n, P, dT, rowtile, aux);                                      \
+    hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(1024), 0, s, aux, m);                                        \
+    hipLaunchKernelGGL(k_tiled_offsets, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, (const int32_t*)aux, \
+                       m, P, rowtile);                                                                         \
     if (homo || plan->weight_dtype == BPX_F32) {                                                               \
       if (data_dtype == BPX_F64)                                                                               \
         hipLaunchKernelGGL((k_tiled_fill<I, double, float>), dim3(blocks), dim3(kBuildThreads), 0, s,          \
@@ -488,25 +556,18 @@ int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* e
   const float* hw = (const float*)(base + plan->homo_offset);
   const int P = plan->tiles, T = plan->tile_len;
   const int cpx = tiled_copies_per_xcd();
-  const int grp = tiled_group_width();
   const int rows = tiled_rows_per_group();
   const unsigned blocks = (unsigned)(8 * cpx * P);
   const size_t shm = (size_t)T * sizeof(unsigned long long);
   const double fx_scale = ldexp(1.0, plan->fx_shift), fx_inv = ldexp(1.0, -plan->fx_shift);
-#define TLGR(WK, E, G, R)                                                                                     \
-  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E, G, R>), dim3(blocks), dim3(kTiledThreads), shm, s, rowtile, lidx, \
+#define TLR(WK, E, R)                                                                                     \
+  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E, R>), dim3(blocks), dim3(kTiledThreads), shm, s, rowtile, lidx, \
              w, hw, (const E*)events, ev_rule, m, n, P, T, cpx, fx_scale, fx_inv, out)
-#define TLG(WK, E, G)                          \
-  do {                                         \
-    if (rows == 4) TLGR(WK, E, G, 4);          \
-    else if (rows == 1) TLGR(WK, E, G, 1);     \
-    else TLGR(WK, E, G, 2);                    \
-  } while (0)
-#define TL(WK, E)                       \
-  do {                                  \
-    if (grp == 16) TLG(WK, E, 16);      \
-    else if (grp == 32) TLG(WK, E, 32); \
-    else TLG(WK, E, 8);                 \
+#define TL(WK, E)                          \
+  do {                                     \
+    if (rows == 4) TLR(WK, E, 4);          \
+    else if (rows == 1) TLR(WK, E, 1);     \
+    else TLR(WK, E, 2);                    \
   } while (0)
 #define TL_E(WK)                                     \
   do {                                               \
@@ -519,8 +580,7 @@ int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* e
   else TL_E(2);
 #undef TL_E
 #undef TL
-#undef TLG
-#undef TLGR
+#undef TLR
   BPX_CHECK_LAUNCH();
   return BPX_OK;
 }

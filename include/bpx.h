@@ -130,7 +130,8 @@ int bpx_event_csrmv_add(void* stream, const void* data, int data_dtype, int64_t 
  * atomic per synaptic event: the output is cut into one tile per CU of an
  * XCD (tile_len floats, 16-bit tile-local column indices), each XCD holds
  * one copy of the output in the LDS of its CUs, and rows carry per-tile
- * segment offsets.  No reference counterpart (the reference re-reads plain
+ * segment offsets; segments are padded to multiples of 8 entries (index
+ * 0xFFFF) so indices and weights stream as 16-byte vectors.  No reference counterpart (the reference re-reads plain
  * CSR each call, event/_csr_matvec.py:453-500); it serves the same operator.
  * Usage: bpx_tiled_csr_layout (sizes, fills the plan's shape fields) ->
  * caller allocates `bytes` of device memory -> bpx_tiled_csr_build ->
@@ -152,7 +153,8 @@ typedef struct {
   int32_t tile_len;     /* T: floats per tile */
   int32_t weight_dtype; /* BPX_F32, BPX_BF16 or BPX_TILED_HOMO */
   int32_t fx_shift;     /* LDS accumulators hold sum(w) * 2^fx_shift as int64 */
-  int64_t rowtile_offset, lidx_offset, weights_offset, homo_offset; /* byte offsets in mem */
+  int64_t capacity;     /* element slots incl. padding (segments padded to 8) */
+  int64_t rowtile_offset, aux_offset, lidx_offset, weights_offset, homo_offset; /* byte offsets in mem */
   void* mem;            /* device memory holding the plan (set by build) */
 } bpx_tiled_csr;
 
