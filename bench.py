@@ -296,6 +296,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the jitconn and network legs")
+    ap.add_argument("--only", choices=["jitconn", "batched", "coba", "served"], default=None,
+                    help="run only this leg (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run 3 plain device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -310,6 +312,12 @@ def main():
 
     dev = _lib.device()
     L = _lib.lib()
+    if args.only:
+        fn = {"jitconn": lambda: bench_jitconn(L, _lib, torch, max(10, args.steps // 10), 3),
+              "batched": lambda: bench_batched(_lib, torch, plain_sample=2, plain_steps=20),
+              "coba": lambda: bench_coba(_lib, torch), "served": lambda: bench_served(_lib, torch)}[args.only]
+        print(json.dumps({"only": args.only, "result": fn()}), flush=True)
+        return
     stream = torch.cuda.current_stream()
     sp = _lib.stream_ptr()
     print(f"# building {M}x{N} CSR, {K_ROW}/row, nnz={M * K_ROW:.3g}", file=sys.stderr, flush=True)

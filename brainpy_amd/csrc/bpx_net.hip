@@ -50,13 +50,9 @@ struct NetArgs {
   LifParams p;
   float tau_e, tau_i, E_e, E_i, c_e, c_i;
   float dt;
-  int64_t step;                // used when d_step is null
-  const int64_t* d_step;       // device step counter (graph replays)
+  int64_t step;                // step index, or offset from *d_step
+  const int64_t* d_step;       // device base step (graph replays) or null
 };
-
-__global__ void k_net_advance(int64_t* d_step) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *d_step += 1;
-}
 
 template <int CONN>
 __global__ __launch_bounds__(kNetThreads) void k_net_step(NetArgs a) {
@@ -74,7 +70,7 @@ __global__ __launch_bounds__(kNetThreads) void k_net_step(NetArgs a) {
   __syncthreads();
   const int j = start + (int)threadIdx.x;
   const int64_t total = a.total;
-  const int64_t step = a.d_step ? *a.d_step : a.step;
+  const int64_t step = a.d_step ? *a.d_step + a.step : a.step;
   const float t = (float)(step - d.t0_step) * a.dt;      // network-local time
   uint32_t* cur = a.counts + (step & 1) * 2 * total;
   uint32_t* nxt = a.counts + ((step + 1) & 1) * 2 * total;
@@ -102,7 +98,6 @@ __global__ __launch_bounds__(kNetThreads) void k_net_step(NetArgs a) {
   }
   __syncthreads();
   const int nspk = s_nspk;
-  if (nspk == 0) return;
   // deliver: 8 groups of 32 lanes, one spiking neuron per group
   const int grp = threadIdx.x >> 5;
   const uint32_t l = threadIdx.x & 31;
@@ -144,7 +139,7 @@ __global__ __launch_bounds__(kNetThreads) void k_net_step(NetArgs a) {
       }
     }
   }
-  if (a.events || a.spike_count) {
+  if (nspk && (a.events || a.spike_count)) {
     // block totals, one global atomic each
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) ev += __shfl_xor(ev, o, 64);
@@ -155,6 +150,10 @@ __global__ __launch_bounds__(kNetThreads) void k_net_step(NetArgs a) {
       if (a.spike_count) atomicAdd(a.spike_count + net, (unsigned int)nspk);
     }
   }
+}
+
+__global__ void k_net_advance(int64_t* d_step, int64_t delta) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *d_step += delta;
 }
 
 }  // namespace bpx
@@ -221,9 +220,9 @@ int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, i
   return BPX_OK;
 }
 
-int bpx_net_advance(void* stream, int64_t* d_step) {
+int bpx_net_advance(void* stream, int64_t* d_step, int64_t delta) {
   if (!d_step) return BPX_ERR_ARG;
-  hipLaunchKernelGGL(k_net_advance, dim3(1), dim3(64), 0, (hipStream_t)stream, d_step);
+  hipLaunchKernelGGL(k_net_advance, dim3(1), dim3(64), 0, (hipStream_t)stream, d_step, delta);
   BPX_CHECK_LAUNCH();
   return BPX_OK;
 }

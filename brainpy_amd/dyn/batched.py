@@ -49,7 +49,7 @@ def _register():
                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.POINTER(NetParams), ctypes.c_int64, ctypes.c_void_p]
         L.bpx_net_step.restype = ctypes.c_int
-        L.bpx_net_advance.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.bpx_net_advance.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
         L.bpx_net_advance.restype = ctypes.c_int
         L._bpx_net_registered = True
     return L
@@ -165,7 +165,7 @@ class BatchedCOBA:
             self.spikes = torch.empty(T, dtype=torch.uint8, device=d)
             self.events = torch.empty(len(self.specs), dtype=torch.int64, device=d)
             self.spike_count = torch.empty(len(self.specs), dtype=torch.int32, device=d)
-            self.d_step = torch.empty(1, dtype=torch.int64, device=d)
+            self.d_step = torch.empty(1, dtype=torch.int64, device=d)      # base step of graph replays
             self.graph = None
         self.V.copy_(torch.from_numpy(self.V0))
         self.t_last.fill_(-1e7)
@@ -178,21 +178,21 @@ class BatchedCOBA:
         self.d_step.zero_()
         self.i = 0
 
-    def step(self, spikes_out=None, count=True, device_step=False):
+    def step(self, spikes_out=None, count=True, device_offset=None):
         """One step of every network.  `spikes_out` (uint8 [total]) receives
-        this step's spike flags.  With `device_step` the step index is read
-        from (and advanced in) the device counter, as graph replays need."""
+        this step's spike flags.  With `device_offset` k the step index is the
+        device base step + k, as graph replays need."""
         L = _lib.lib()
         sp = spikes_out if spikes_out is not None else (self.spikes if self.record else None)
         s = _lib.stream_ptr()
+        dev = device_offset is not None
         rc = L.bpx_net_step(s, _lib.ptr(self.d_descs), _lib.ptr(self.d_tiles), self.n_tiles, self.kind,
                             self.total, _lib.ptr(self.V), _lib.ptr(self.t_last), _lib.ptr(self.g_exc),
                             _lib.ptr(self.g_inh), _lib.ptr(self.counts), _lib.ptr(sp),
                             _lib.ptr(self.events) if count else None, _lib.ptr(self.spike_count) if count else None,
-                            ctypes.byref(self.params), self.i, _lib.ptr(self.d_step) if device_step else None)
+                            ctypes.byref(self.params), device_offset if dev else self.i,
+                            _lib.ptr(self.d_step) if dev else None)
         _lib.check(rc, "bpx_net_step")
-        if device_step:
-            _lib.check(L.bpx_net_advance(s, _lib.ptr(self.d_step)), "bpx_net_advance")
         self.i += 1
 
     def run(self, steps, raster=False):
@@ -203,9 +203,10 @@ class BatchedCOBA:
         return out
 
     def capture(self, steps):
-        """Capture `steps` steps in one HIP graph (each step one batched
-        launch plus the device step-counter increment); replay() advances the
-        batch by `steps` steps per call, continuing in time."""
+        """Capture `steps` steps in one HIP graph (one batched launch per step
+        at offsets 0..steps-1 from the device base step, then one launch that
+        advances the base by `steps`); replay() advances the batch by `steps`
+        steps per call, continuing in time."""
         self.d_step.fill_(self.i)
         torch.cuda.synchronize()
         i0 = self.i
@@ -214,8 +215,10 @@ class BatchedCOBA:
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
             with torch.cuda.graph(g, stream=s):
-                for _ in range(steps):
-                    self.step(device_step=True)
+                for k in range(steps):
+                    self.step(device_offset=k)
+                _lib.check(_lib.lib().bpx_net_advance(_lib.stream_ptr(), _lib.ptr(self.d_step), steps),
+                           "bpx_net_advance")
         torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
         # capture records the launches without running them

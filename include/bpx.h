@@ -226,9 +226,10 @@ float bpx_expon_coef(float tau, float dt);
  * State pools are indexed by off + local neuron id (E first, then I);
  * counts is [2][2][total] uint32, zero at start (and zero in a network's
  * range when it joins).  The step index (count buffer parity; network time
- * t = (step - t0_step) * dt) is `step`, or *d_step when d_step is a
- * device counter -- the form for HIP graphs, where bpx_net_advance(d_step)
- * after each step makes every replay continue in time.
+ * t = (step - t0_step) * dt) is `step`, or *d_step + step when d_step points
+ * to a device int64 base -- the form for HIP graphs: capture K launches with
+ * step = 0..K-1 followed by bpx_net_advance(d_step, K), and every replay
+ * continues in time.
  * tiles: device array of (network, first local neuron) int32 pairs, one per
  * 256-neuron block.  nets: device array of bpx_net_desc prepared with
  * bpx_net_prepare (host).  jitconn projections use seed[k], clen[k] for
@@ -265,7 +266,7 @@ int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, i
                  int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts,
                  uint8_t* spikes, unsigned long long* events, unsigned int* spike_count,
                  const bpx_net_params* params, int64_t step, const int64_t* d_step);
-int bpx_net_advance(void* stream, int64_t* d_step);
+int bpx_net_advance(void* stream, int64_t* d_step, int64_t delta);
 
 /* Plain device copy (bandwidth calibration of the HBM counters). */
 int bpx_copy(void* stream, const void* src, void* dst, int64_t bytes);
