@@ -17,6 +17,7 @@ the launch stream) against the 8 TB/s MI355X peak.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -184,7 +185,51 @@ def _c4_specs(n_nets=256, seed=2024):
     return specs
 
 
-def bench_batched(_lib, torch, steps=2000, n_nets=256, plain_sample=8, plain_steps=100):
+def _current_error_f64(torch, specs, bf16, steps, stored_weights=False):
+    """Per-step synaptic current of the batched FP32 run against an FP64 twin
+    fed with the run's own spike counts and the unrounded weights (with
+    `stored_weights`, the weights the run stores: arithmetic error only).
+
+    Each step, before the launch, the twin reads the counts the launch will
+    consume and the pre-step potentials V; its conductances follow
+    g64 = (g64 + count * w64) * exp(-dt / tau) exactly, and the currents are
+    I = g_e (E_e - V) + g_i (E_i - V) from the run's g (FP32) and from g64.
+    Conditioning on the run's spikes is the only comparison a chaotic network
+    allows.  Returns, per network, max over steps of
+    max|I32 - I64| / max|I64| (networks with no current yet are skipped)."""
+    from brainpy_amd.dyn.batched import BatchedCOBA, bf16_round
+    b = BatchedCOBA(specs, bf16=bf16)
+    d, T = b.dev, b.total
+    nid = torch.repeat_interleave(torch.arange(len(specs), device=d), torch.tensor(b.sizes, device=d))
+    rnd = bf16_round if (bf16 and stored_weights) else (lambda x: float(np.float32(x)))
+    we = torch.tensor([rnd(s["w_exc"]) for s in specs], dtype=torch.float64, device=d)[nid]
+    wi = torch.tensor([rnd(s["w_inh"]) for s in specs], dtype=torch.float64, device=d)[nid]
+    P = b.params
+    de = math.exp(-float(np.float32(P.dt)) / float(np.float32(P.tau_e)))
+    di = math.exp(-float(np.float32(P.dt)) / float(np.float32(P.tau_i)))
+    Ee, Ei = float(np.float32(P.E_e)), float(np.float32(P.E_i))
+    ge = torch.zeros(T, dtype=torch.float64, device=d)
+    gi = torch.zeros(T, dtype=torch.float64, device=d)
+    worst = torch.zeros(len(specs), dtype=torch.float64, device=d)
+    for _ in range(steps):
+        base = (b.i & 1) * 2 * T
+        ce = b.counts[base:base + T].double()
+        ci = b.counts[base + T:base + 2 * T].double()
+        V = b.V.double()
+        b.step()
+        ge = (ge + ce * we) * de
+        gi = (gi + ci * wi) * di
+        I64 = ge * (Ee - V) + gi * (Ei - V)
+        I32 = b.g_exc.double() * (Ee - V) + b.g_inh.double() * (Ei - V)
+        num = torch.zeros_like(worst).scatter_reduce(0, nid, (I32 - I64).abs(), "amax")
+        den = torch.zeros_like(worst).scatter_reduce(0, nid, I64.abs(), "amax")
+        worst = torch.maximum(worst, torch.where(den > 0, num / den.clamp_min(1e-300), torch.zeros_like(num)))
+    w = worst.cpu().numpy()
+    del b
+    return {"max": float(w.max()), "mean": float(w.mean()), "steps": steps}
+
+
+def bench_batched(_lib, torch, steps=2000, n_nets=256, plain_sample=8, plain_steps=100, err_steps=400):
     """C4: 256 independent E/I networks (4:1), N log-uniform in [1k, 64k],
     per-network seed, we ~ U(0.3, 0.9), wi ~ U(4, 9), input ~ U(15, 25) mV,
     jitconn in-degree 80, BF16 weights, FP32 state, 2000 steps, one launch
@@ -207,6 +252,21 @@ def bench_batched(_lib, torch, steps=2000, n_nets=256, plain_sample=8, plain_ste
     events = int(b.events.sum())
     spikes = int(b.spike_count.sum())
     total_neurons = b.total
+    # the same batch with FP32 homogeneous weights (scalar kernel arguments:
+    # same bytes, same kernel)
+    b32 = BatchedCOBA(specs, bf16=False)
+    b32.capture(100)
+    b32.replay()
+    torch.cuda.synchronize()
+    b32.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps // 100):
+        b32.replay()
+    torch.cuda.synchronize()
+    t_b32 = time.perf_counter() - t0
+    events32 = int(b32.events.sum())
+    del b32
     # plain FP32 baseline on a size-stratified sample
     order = np.argsort([s["n_exc"] + s["n_inh"] for s in specs])
     pick = order[np.linspace(0, len(order) - 1, plain_sample).astype(int)]
@@ -220,8 +280,11 @@ def bench_batched(_lib, torch, steps=2000, n_nets=256, plain_sample=8, plain_ste
         torch.cuda.synchronize()
         per_step.append((time.perf_counter() - t0) / plain_steps)
     t_plain_est = float(np.mean(per_step)) * steps * n_nets
-    # BF16 weight error against the FP32/FP64 weights (homogeneous weights:
-    # the per-step current error is the weight's rounding error)
+    # per-step synaptic current against an FP64 twin (same spikes, unrounded
+    # weights): BF16-rounded homogeneous weights and FP32 weights
+    cur_err = {"bf16_weights": _current_error_f64(torch, specs, True, err_steps),
+               "bf16_weights_vs_stored": _current_error_f64(torch, specs, True, err_steps, stored_weights=True),
+               "f32_weights": _current_error_f64(torch, specs, False, err_steps)}
     rel = []
     for s in specs:
         for key in ("w_exc", "w_inh"):
@@ -233,9 +296,13 @@ def bench_batched(_lib, torch, steps=2000, n_nets=256, plain_sample=8, plain_ste
          "plain_fp32_est_s": t_plain_est, "plain_sims_per_s": n_nets / t_plain_est,
          "plain_events_per_s_est": events / t_plain_est, "speedup_vs_plain": t_plain_est / t_b,
          "plain_sample_networks": int(plain_sample), "plain_sample_steps": plain_steps,
-         "bf16_weight_rel_err_max": float(max(rel)), "bf16_weight_rel_err_mean": float(np.mean(rel))}
+         "bf16_weight_rel_err_max": float(max(rel)), "bf16_weight_rel_err_mean": float(np.mean(rel)),
+         "current_rel_err_vs_f64": cur_err, "f32_weights_s": t_b32, "f32_weights_events_per_s": events32 / t_b32}
     print(f"# batched C4 {t_b:.3f} s ({t_b/steps*1e6:.1f} us/step) {events/t_b:.3e} ev/s, plain est "
-          f"{t_plain_est:.1f} s -> x{t_plain_est/t_b:.1f}", file=sys.stderr, flush=True)
+          f"{t_plain_est:.1f} s -> x{t_plain_est/t_b:.1f}; current err vs f64: bf16 max "
+          f"{cur_err['bf16_weights']['max']:.2e} mean {cur_err['bf16_weights']['mean']:.2e} (vs stored bf16 "
+          f"{cur_err['bf16_weights_vs_stored']['max']:.2e}), f32 max {cur_err['f32_weights']['max']:.2e}",
+          file=sys.stderr, flush=True)
     return r
 
 
@@ -410,15 +477,24 @@ def main():
         err = float(((out - ref).abs() / ref.abs().clamp_min(1.0)).max())
         if err > 1e-5:
             raise RuntimeError(f"tiled path disagrees with the CSR path ({kind}: {err})")
-    # BF16-stored heterogeneous weights: relative error of the scattered
-    # current against the FP32-weight result (itself within ~1e-7 of FP64)
-    call_tiled("heter_f32", HEADLINE[1])
-    ref32 = out.double().clone()
-    call_tiled("heter_bf16", HEADLINE[1])
-    torch.cuda.synchronize()
-    big = ref32.abs() > 1.0
-    bf16_err = float(((out.double() - ref32).abs()[big] / ref32.abs()[big]).max())
-    bf16_err_mean = float(((out.double() - ref32).abs()[big] / ref32.abs()[big]).mean())
+    # FP32 and BF16-stored heterogeneous weights: relative error of the
+    # scattered current against an FP64 sum of the same (FP32) weights
+    ev = spikes[HEADLINE[1]]
+    rows = torch.nonzero(ev).flatten()
+    starts = indptr[rows].long()
+    lens = (indptr[rows + 1] - indptr[rows]).long()
+    first = torch.cumsum(lens, 0) - lens
+    ent = torch.repeat_interleave(starts - first, lens) + torch.arange(int(lens.sum()), device=dev)
+    ref64 = torch.zeros(N, dtype=torch.float64, device=dev)
+    ref64.index_add_(0, indices[ent].long(), w_f32[ent].double())
+    del ent
+    big = ref64.abs() > 1.0
+    f64_err = {}
+    for kind in ("heter_f32", "heter_bf16"):
+        call_tiled(kind, HEADLINE[1])
+        torch.cuda.synchronize()
+        e = (out.double() - ref64).abs()[big] / ref64.abs()[big]
+        f64_err[kind] = {"max": float(e.max()), "mean": float(e.mean())}
     sweep = []
     if not args.no_sweep:
         for path in ("csr", "tiled"):
@@ -432,6 +508,9 @@ def main():
     head = measure(HEADLINE[0], HEADLINE[1], args.steps, args.warmup, "tiled")
     plain = measure(HEADLINE[0], HEADLINE[1], args.steps, args.warmup, "csr")
     print(f"# headline {head}", file=sys.stderr, flush=True)
+    print(f"# current error vs f64 (10% heter): f32 max {f64_err['heter_f32']['max']:.2e} mean "
+          f"{f64_err['heter_f32']['mean']:.2e}, bf16 max {f64_err['heter_bf16']['max']:.2e} mean "
+          f"{f64_err['heter_bf16']['mean']:.2e}", file=sys.stderr, flush=True)
 
     extra = {}
     if not args.no_extra:
@@ -492,8 +571,7 @@ def main():
                      "frac": head["kernel_GBs"] / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_tiled_scatter (heter f32, bool events)", "kernel_us": head["kernel_us"],
                      "algorithmic_bytes_per_launch": head["algorithmic_bytes"]},
-        "bf16_weights_rel_err": {"max": bf16_err, "mean": bf16_err_mean,
-                                  "against": "FP32-weight result, outputs with |y| > 1"},
+        "current_rel_err_vs_f64": dict(f64_err, against="FP64 sum of the FP32 weights, outputs with |y| > 1"),
         "plain_csr_path": {"events_per_s": plain["events_per_s"], "ms_per_step": plain["ms_per_step"],
                            "kernel_us": plain["kernel_us"], "kernel_GBs": plain["kernel_GBs"],
                            "kernel": "k_csr_scatter (stateless bm.event.csrmv)"},

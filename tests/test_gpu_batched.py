@@ -144,3 +144,68 @@ def test_served_alone_equals_in_mix():
         (r,) = alone.serve([Request(k, n, seed, we, wi, I, steps, keep_state=True)])
         assert r.spikes == done[k].spikes and r.events == done[k].events
         np.testing.assert_array_equal(r.V_final, done[k].V_final)
+
+
+def _twin_error(specs, bf16, steps, stored_weights=False):
+    """FP64 twin fed with the run's spike counts:
+    g64 = (g64 + count * w) * exp(-dt / tau), w the unrounded weights (or the
+    weights the run stores, with `stored_weights`).  Returns the max over
+    networks and steps of max|I32 - I64| / max|I64| (synaptic current) and
+    the per-network max relative conductance error, E and I."""
+    import math
+    from brainpy_amd.dyn.batched import bf16_round
+    b = BatchedCOBA(specs, bf16=bf16)
+    d, T = b.dev, b.total
+
+    def wcol(key):
+        vals = [bf16_round(s[key]) if (bf16 and stored_weights) else float(np.float32(s[key])) for s in specs]
+        return torch.cat([torch.full((n,), v, dtype=torch.float64) for v, n in zip(vals, b.sizes)]).to(d)
+
+    we, wi = wcol("w_exc"), wcol("w_inh")
+    de, di = math.exp(-0.1 / 5.0), math.exp(-0.1 / 10.0)
+    ge = torch.zeros(T, dtype=torch.float64, device=d)
+    gi = torch.zeros(T, dtype=torch.float64, device=d)
+    worst = 0.0
+    g_err = np.zeros((len(specs), 2))
+    for _ in range(steps):
+        base = (b.i & 1) * 2 * T
+        ce = b.counts[base:base + T].double()
+        ci = b.counts[base + T:base + 2 * T].double()
+        V = b.V.double()
+        b.step()
+        ge = (ge + ce * we) * de
+        gi = (gi + ci * wi) * di
+        I64 = ge * (0.0 - V) + gi * (-80.0 - V)
+        I32 = b.g_exc.double() * (0.0 - V) + b.g_inh.double() * (-80.0 - V)
+        for k in range(len(specs)):
+            sl = b.net_slice(k)
+            den = float(I64[sl].abs().max())
+            if den > 0:
+                worst = max(worst, float((I32[sl] - I64[sl]).abs().max()) / den)
+            for j, (g32, g64) in enumerate(((b.g_exc, ge), (b.g_inh, gi))):
+                m = g64[sl] > 0
+                if bool(m.any()):
+                    r = float(((g32[sl].double() - g64[sl]).abs()[m] / g64[sl][m]).max())
+                    g_err[k, j] = max(g_err[k, j], r)
+    return worst, g_err, b
+
+
+@pytest.mark.gpu
+def test_current_error_vs_f64_twin():
+    """FP32 arithmetic: per-step synaptic currents within 1e-5 relative of the
+    FP64 twin (FP32 weights, and BF16-stored weights against a twin using the
+    same stored weights).  Against the unrounded weights the BF16 run's
+    conductances are off by exactly each network's weight rounding (to 1e-5)."""
+    from brainpy_amd.dyn.batched import bf16_round
+    specs = _jit_specs((1500, 4000, 9000, 2500))
+    e32, _, b = _twin_error(specs, False, 200)
+    assert int(b.spike_count.sum()) > 0
+    assert e32 <= 1e-5, e32
+    e16s, _, _ = _twin_error(specs, True, 200, stored_weights=True)
+    assert e16s <= 1e-5, e16s
+    _, g_err, _ = _twin_error(specs, True, 200)
+    for k, s in enumerate(specs):
+        for j, key in enumerate(("w_exc", "w_inh")):
+            w = float(np.float32(s[key]))
+            delta = abs(bf16_round(s[key]) - w) / w
+            assert abs(g_err[k, j] - delta) <= 1e-5, (k, key, g_err[k, j], delta)
