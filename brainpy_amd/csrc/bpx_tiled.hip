@@ -201,10 +201,12 @@ __global__ void k_store_scalar(const void* src, int src_dtype, float* dst) {
 //
 // Each wave walks its chunks of 64 rows four at a time: the next four
 // chunks' events are loaded before the current ones are processed.  Active
-// rows of a batch (ballot + mbcnt) are listed in LDS together with their
-// tile-p segment [start, end); then 8 groups of 8 lanes take two listed rows
-// each per round and issue 4+4 element loads per lane before their LDS
-// atomics.
+// rows of a batch with a non-empty tile-p segment (ballot + mbcnt) are
+// listed in LDS with the segment [start, end); then 16 groups of 4 lanes
+// take one listed row each per round, a lane loading 8 indices (16 bytes)
+// and 8 weights per pass before its LDS atomics.  Every load of a phase is
+// issued without a branch around it, so the four chunks' loads are in
+// flight together.
 // ---------------------------------------------------------------------------
 constexpr int kChunksPerBatch = 4;
 constexpr int kListCap = 64 * kChunksPerBatch;
@@ -247,27 +249,40 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
   const int64_t cstep = (int64_t)ncopy * kWavesPerBlock;    // chunk stride of one wave
   int64_t q0 = copy + (int64_t)ncopy * wave;                // first chunk of this wave
 
+  // loads are issued for every lane (rows past the end read row m-1 and are
+  // masked), so the four chunks' loads are in flight together
   auto load_batch = [&](int64_t qb, bool* act) {
+    E v[kChunksPerBatch];
 #pragma unroll
     for (int u = 0; u < kChunksPerBatch; ++u) {
-      const int64_t q = qb + cstep * u;
-      const int64_t r = (q << 6) + lane;
-      act[u] = q < nchunks && r < m && ev_true(ev[r], rule);
+      const int64_t r = ((qb + cstep * u) << 6) + lane;
+      v[u] = ev[r < m ? r : m - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < kChunksPerBatch; ++u) {
+      const int64_t r = ((qb + cstep * u) << 6) + lane;
+      act[u] = r < m && ev_true(v[u], rule);
     }
   };
   bool act[kChunksPerBatch];
   load_batch(q0, act);
   while (q0 < nchunks) {
     // list the batch's active rows with their tile-p segments
+    // (inactive lanes read row 0's pair: one shared line, no branch, so the
+    // four chunks' loads are in flight together)
+    int2 sgl[kChunksPerBatch];
+#pragma unroll
+    for (int u = 0; u < kChunksPerBatch; ++u) {
+      const int64_t r = act[u] ? ((q0 + cstep * u) << 6) + lane : 0;
+      const int32_t* rt = rowtile + r * (P + 1) + p;
+      sgl[u] = make_int2(rt[0], rt[1]);
+    }
     int cnt = 0;
 #pragma unroll
     for (int u = 0; u < kChunksPerBatch; ++u) {
-      const uint64_t mask = __ballot(act[u]);
-      if (act[u]) {
-        const int64_t r = ((q0 + cstep * u) << 6) + lane;
-        const int32_t* rt = rowtile + r * (P + 1) + p;
-        seg[cnt + lanes_below(mask)] = make_int2(rt[0], rt[1]);
-      }
+      const bool nz = act[u] && sgl[u].y > sgl[u].x;
+      const uint64_t mask = __ballot(nz);
+      if (nz) seg[cnt + lanes_below(mask)] = sgl[u];
       cnt += __popcll(mask);
     }
     // prefetch the next batch's events
@@ -361,7 +376,7 @@ static int tiled_rows_per_group() {
   static int r = [] {
     const char* e = getenv("BPX_TILED_ROWS");
     int v = e ? atoi(e) : 1;
-    return (v == 1 || v == 2 || v == 4) ? v : 1;
+    return (v == 1 || v == 2) ? v : 1;
   }();
   return r;
 }
@@ -565,8 +580,7 @@ int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* e
              w, hw, (const E*)events, ev_rule, m, n, P, T, cpx, fx_scale, fx_inv, out)
 #define TL(WK, E)                          \
   do {                                     \
-    if (rows == 4) TLR(WK, E, 4);          \
-    else if (rows == 1) TLR(WK, E, 1);     \
+    if (rows == 1) TLR(WK, E, 1);          \
     else TLR(WK, E, 2);                    \
   } while (0)
 #define TL_E(WK)                                     \
