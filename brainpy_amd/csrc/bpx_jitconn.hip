@@ -11,9 +11,12 @@
 // The u32 modulo by clen is an exact multiply-high (DivU32).
 //
 // Launch shapes:
-//   outdim_parallel, lane32: one thread per (row, lane): a wave covers two
-//     rows; the 32 lane partials are combined with a 32-wide shuffle sum and
-//     written once (no atomics, no zero-fill).
+//   outdim_parallel, lane32, bool events, homo/uniform weights: one thread
+//     per row walking the row's 32 streams in one flattened loop, event
+//     bitmask in LDS (k_jit_rows_bits).
+//   outdim_parallel, lane32, otherwise: one thread per (row, lane): a wave
+//     covers two rows; the 32 lane partials are combined with a 32-wide
+//     shuffle sum and written once (no atomics, no zero-fill).
 //   scatter (outdim_parallel=False), lane32: 256-thread blocks own 256
 //     columns; events are balloted into an LDS list of active columns, then 8
 //     groups of 32 lanes regenerate the rows of one active column each and
@@ -22,6 +25,9 @@
 //   row layout: one thread per row (outdim_parallel) or per active column.
 #include "bpx_common.h"
 #include "bpx_rng.h"
+
+#include <math.h>
+#include <stdlib.h>
 
 
 namespace bpx {
@@ -222,6 +228,81 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_lane32_outdim_bits(const ui
 }
 
 // ---------------------------------------------------------------------------
+// lane32, outdim_parallel, bool events, one thread per row.
+//
+// A thread walks the 32 streams of its row one after another in a single
+// flattened loop: each iteration either tests the current column (and, for
+// uniform/normal weights, draws the connection's weight) or, when the stream
+// has passed its end, folds the stream's partial into the row sum and starts
+// the next stream; then every lane takes the one shared LFSR88 increment
+// draw.  A row is ~32 x (1 + connections per stream) draws, so the lanes of a
+// wave stay busy together (a wave waits for its longest row, not for its
+// longest 3-draw stream), and the row sum is formed in ascending stream
+// order, `out[row] = (((0 + p_0) + p_1) + ... + p_31)` with
+// `p_l = sum of w * e over stream l's connections in order` (the Taichi
+// kernel's `r += weight0 * events[i_col]` per stream, then `out[i_row] += r`;
+// jitconn/_event_matvec_taichi.py:140-155).  Rows are dealt to the grid's
+// threads in turn; the event bitmask sits in LDS as above.
+// ---------------------------------------------------------------------------
+// FIN: the weights are finite, so `w * e` with e in {0, 1} equals
+// `e ? w : +0` and adding +0 leaves the partial unchanged (the partial starts
+// at +0 and never becomes -0); non-finite weights take the multiply.
+template <int DIST, bool Q24, bool FIN>
+__global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* __restrict__ gbits, int64_t nwords,
+                                                                float w0, float w1, DivU32 dv, uint32_t seed,
+                                                                int64_t num_row, int64_t num_col, uint32_t step,
+                                                                float* __restrict__ out) {
+  extern __shared__ uint32_t s_bits[];
+  for (int64_t k = threadIdx.x; k < nwords; k += kJitThreads) s_bits[k] = gbits[k];
+  __syncthreads();
+  const int64_t nthreads = (int64_t)gridDim.x * kJitThreads;
+  const uint32_t ncol = (uint32_t)num_col;       // num_col < 2^32 (checked by the launcher)
+  int64_t row = (int64_t)blockIdx.x * kJitThreads + threadIdx.x;
+  bool live = row < num_row;
+  uint32_t l = 0;
+  uint32_t kseed = seed + (uint32_t)(row * 32);  // stream key seed + row*32 + l
+  Lfsr88 key(kseed);
+  uint32_t c0 = 0xFFFFFFFFu;                      // step * l - 1
+  uint32_t c = c0;
+  uint32_t end = min(c0 + step, ncol);
+  float acc = 0.f, racc = 0.f;
+  c += rand_inc_u<Q24>(key, dv);
+  while (__ballot(live) != 0ull) {
+    const bool in = c < end;
+    const uint32_t word = s_bits[(in ? c : 0u) >> 5];   // in flight during the draws
+    const uint32_t cb = c & 31u;
+    float w = w0;
+    if constexpr (DIST != BPX_DIST_HOMO) {
+      if (in) w = draw_weight<DIST>(key, w0, w1);
+    }
+    if (!in && live) {
+      racc += acc;
+      acc = 0.f;
+      ++kseed;
+      c0 += step;
+      if (++l == 32u) {
+        out[row] = racc;
+        racc = 0.f;
+        l = 0;
+        row += nthreads;
+        live = row < num_row;
+        kseed = seed + (uint32_t)(row * 32);
+        c0 = 0xFFFFFFFFu;
+      }
+      key = Lfsr88(kseed);
+      c = c0;
+      end = min(c0 + step, ncol);
+    }
+    c += rand_inc_u<Q24>(key, dv);
+    if (in) {
+      const bool on = (word >> cb) & 1u;
+      if constexpr (FIN) acc += on ? w : 0.f;
+      else acc += w * (on ? 1.f : 0.f);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // row layout
 // ---------------------------------------------------------------------------
 template <int DIST, int VM>
@@ -290,6 +371,16 @@ static int jit_cus() {
   return c;
 }
 
+// one-thread-per-row kernel for bool events (BPX_JIT_ROWS=0 selects the
+// thread-per-(row, lane) bitmask kernel, for comparison)
+static bool jit_rows_kernel() {
+  static int v = [] {
+    const char* e = getenv("BPX_JIT_ROWS");
+    return e ? atoi(e) : 1;
+  }();
+  return v != 0;
+}
+
 static bool bits_fit(int64_t nwords) {
   static int shm = [] {
     int dev = 0, v = 0;
@@ -307,7 +398,28 @@ static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t c
   const uint32_t step = (uint32_t)(((num_row + 1) >> 5) > 1 ? ((num_row + 1) >> 5) : 1);
   if (outdim) {
     const int64_t nwords = (num_col + 31) / 32;
-    if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && DIST != BPX_DIST_NORMAL && bits_fit(nwords) && ws &&
+    if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && DIST != BPX_DIST_NORMAL && jit_rows_kernel() &&
+        bits_fit(nwords) && ws &&
+        ws_bytes >= (size_t)nwords * 4 && num_col < 0xFFFFFFFFll) {
+      uint32_t* bits = (uint32_t*)ws;
+      hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((num_col + 255) / 256)), dim3(256), 0, s, (const uint8_t*)vec,
+                         num_col, bits);
+      int64_t blocks = (num_row + kJitThreads - 1) / kJitThreads;
+      const int64_t maxb = (int64_t)jit_cus();
+      if (blocks > maxb) blocks = maxb;
+      const bool fin = isfinite(w0) && (DIST == BPX_DIST_HOMO || isfinite(w1));
+#define JRB(Q, F)                                                                                             \
+  BPX_LAUNCH(KID_JIT_OUTDIM, (k_jit_rows_bits<DIST, Q, F>), dim3((unsigned)blocks), dim3(kJitThreads),          \
+             (size_t)nwords * 4, s, (const uint32_t*)bits, nwords, w0, w1, dv, seed, num_row, num_col, step, out)
+      if (dv.small_quotient()) {
+        if (fin) JRB(true, true);
+        else JRB(true, false);
+      } else {
+        if (fin) JRB(false, true);
+        else JRB(false, false);
+      }
+#undef JRB
+    } else if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && DIST != BPX_DIST_NORMAL && bits_fit(nwords) && ws &&
         ws_bytes >= (size_t)nwords * 4) {
       uint32_t* bits = (uint32_t*)ws;
       hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((num_col + 255) / 256)), dim3(256), 0, s, (const uint8_t*)vec,
