@@ -49,3 +49,34 @@ def fixed_prob_csr_device(m, n, prob, seed, device):
         indices[r0 * k:r1 * k] = (draw + ar).to(torch.int32).reshape(-1)
     indptr = torch.arange(m + 1, device=device, dtype=torch.int64).mul_(k).to(torch.int32)
     return indices, indptr
+
+
+class CSRConn:
+    """A two-end connector holding a given CSR (the part of
+    connect/base.py's ``TwoEndConnector`` the CSR layers use: ``pre_num``,
+    ``post_num`` and ``require('csr')``)."""
+
+    def __init__(self, indices, indptr, pre_num, post_num):
+        self.indices, self.indptr = indices, indptr
+        self.pre_num, self.post_num = int(pre_num), int(post_num)
+        if len(indptr) != self.pre_num + 1:
+            raise ValueError(f"indptr must have pre_num + 1 = {self.pre_num + 1} entries, got {len(indptr)}.")
+
+    def require(self, *structures):
+        if structures != ("csr",):
+            raise ValueError(f"only 'csr' is provided, got {structures}")
+        return self.indices, self.indptr
+
+
+class FixedProb(CSRConn):
+    """``FixedProb(prob, seed=...)`` applied to ``(pre_num, post_num)``
+    (connect/random_conn.py:85-128 with ``pre_ratio=1``, no self-connection
+    filter): every pre-synaptic neuron gets ``int(post_num * prob)`` distinct
+    targets, drawn here from a seeded PCG64 stream."""
+
+    def __init__(self, prob, pre_num, post_num, seed=0):
+        if not 0.0 <= prob <= 1.0:
+            raise ValueError(f"prob must be in [0, 1], got {prob}")
+        self.prob, self.seed = float(prob), int(seed)
+        indices, indptr = fixed_prob_csr(int(pre_num), int(post_num), prob, seed)
+        super().__init__(indices, indptr, pre_num, post_num)

@@ -19,9 +19,16 @@ _METHODS = ("cusparse", "scalar", "vector", "adaptive")
 
 
 def _as_array(x):
+    """Arrays pass through; python scalars and sequences take JAX's default
+    dtypes without x64 (float -> float32, int -> int32)."""
     if isinstance(x, (torch.Tensor, np.ndarray)):
         return x
-    return np.asarray(x)
+    a = np.asarray(x)
+    if a.dtype == np.float64:
+        a = a.astype(np.float32)
+    elif a.dtype == np.int64:
+        a = a.astype(np.int32)
+    return a
 
 
 def _run(data, indices, indptr, vector, shape, transpose):
