@@ -244,29 +244,45 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_lane32_outdim_bits(const ui
 // jitconn/_event_matvec_taichi.py:140-155).  Rows are dealt to the grid's
 // threads in turn; the event bitmask sits in LDS as above.
 // ---------------------------------------------------------------------------
+// Modulo variants of the increment draw: 0 exact multiply-high (DivU32),
+// 1 the same with a 24-bit product, 2 the short form (DivShort) with one
+// signed 24-bit multiply-add for the remainder.
+enum { MOD_GM = 0, MOD_GM24 = 1, MOD_SHORT24 = 2 };
+
+template <int MK>
+__device__ __forceinline__ uint32_t mod_clen(uint32_t x, const DivU32& dv, const DivShort& ds) {
+  if constexpr (MK == MOD_SHORT24) return ds.mod24(x);
+  else if constexpr (MK == MOD_GM24) return dv.mod24(x);
+  else return dv.mod(x);
+}
+
 // FIN: the weights are finite, so `w * e` with e in {0, 1} equals
-// `e ? w : +0` and adding +0 leaves the partial unchanged (the partial starts
-// at +0 and never becomes -0); non-finite weights take the multiply.
-template <int DIST, bool Q24, bool FIN>
+// `e ? w : +0`, adding +0 leaves a partial unchanged (it starts at +0 and
+// never becomes -0), and a homogeneous stream's partial is the sum of its
+// `hits` active terms, ((0 + w) + w) + ...; non-finite weights take the
+// multiply.
+template <int DIST, int MK, bool FIN>
 __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* __restrict__ gbits, int64_t nwords,
-                                                                float w0, float w1, DivU32 dv, uint32_t seed,
-                                                                int64_t num_row, int64_t num_col, uint32_t step,
-                                                                float* __restrict__ out) {
+                                                                float w0, float w1, DivU32 dv, DivShort ds,
+                                                                uint32_t seed, int64_t num_row, int64_t num_col,
+                                                                uint32_t step, float* __restrict__ out) {
   extern __shared__ uint32_t s_bits[];
   for (int64_t k = threadIdx.x; k < nwords; k += kJitThreads) s_bits[k] = gbits[k];
   __syncthreads();
+  constexpr bool HITS = DIST == BPX_DIST_HOMO && FIN;
   const int64_t nthreads = (int64_t)gridDim.x * kJitThreads;
   const uint32_t ncol = (uint32_t)num_col;       // num_col < 2^32 (checked by the launcher)
+  const uint32_t c0_last = step * 32u - 1u;      // step * l - 1 at l = 32: the row is done
   int64_t row = (int64_t)blockIdx.x * kJitThreads + threadIdx.x;
   bool live = row < num_row;
-  uint32_t l = 0;
   uint32_t kseed = seed + (uint32_t)(row * 32);  // stream key seed + row*32 + l
   Lfsr88 key(kseed);
   uint32_t c0 = 0xFFFFFFFFu;                      // step * l - 1
   uint32_t c = c0;
   uint32_t end = min(c0 + step, ncol);
   float acc = 0.f, racc = 0.f;
-  c += rand_inc_u<Q24>(key, dv);
+  uint32_t hits = 0;
+  c = c + mod_clen<MK>(key.next(), dv, ds) + 1u;
   while (__ballot(live) != 0ull) {
     const bool in = c < end;
     const uint32_t word = s_bits[(in ? c : 0u) >> 5];   // in flight during the draws
@@ -276,14 +292,20 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
       if (in) w = draw_weight<DIST>(key, w0, w1);
     }
     if (!in && live) {
-      racc += acc;
-      acc = 0.f;
+      if constexpr (HITS) {
+        // a plain loop (no memory access, so no wait on the event word
+        // read in flight): 0 or 1 terms for nearly every stream
+        racc += repeat_sum_f(hits, w0);
+        hits = 0;
+      } else {
+        racc += acc;
+        acc = 0.f;
+      }
       ++kseed;
       c0 += step;
-      if (++l == 32u) {
+      if (c0 == c0_last) {
         out[row] = racc;
         racc = 0.f;
-        l = 0;
         row += nthreads;
         live = row < num_row;
         kseed = seed + (uint32_t)(row * 32);
@@ -293,8 +315,10 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
       c = c0;
       end = min(c0 + step, ncol);
     }
-    c += rand_inc_u<Q24>(key, dv);
-    if (in) {
+    c = c + mod_clen<MK>(key.next(), dv, ds) + 1u;
+    if constexpr (HITS) {
+      hits += in ? ((word >> cb) & 1u) : 0u;
+    } else if (in) {
       const bool on = (word >> cb) & 1u;
       if constexpr (FIN) acc += on ? w : 0.f;
       else acc += w * (on ? 1.f : 0.f);
@@ -408,16 +432,21 @@ static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t c
       const int64_t maxb = (int64_t)jit_cus();
       if (blocks > maxb) blocks = maxb;
       const bool fin = isfinite(w0) && (DIST == BPX_DIST_HOMO || isfinite(w1));
-#define JRB(Q, F)                                                                                             \
-  BPX_LAUNCH(KID_JIT_OUTDIM, (k_jit_rows_bits<DIST, Q, F>), dim3((unsigned)blocks), dim3(kJitThreads),          \
-             (size_t)nwords * 4, s, (const uint32_t*)bits, nwords, w0, w1, dv, seed, num_row, num_col, step, out)
-      if (dv.small_quotient()) {
-        if (fin) JRB(true, true);
-        else JRB(true, false);
-      } else {
-        if (fin) JRB(false, true);
-        else JRB(false, false);
-      }
+      DivShort ds = {1u, 0u, 0u};
+      const bool shrt = DivShort::make((uint32_t)clen, &ds) && DivShort::mad24_ok((uint32_t)clen);
+#define JRB(MKV, F)                                                                                           \
+  BPX_LAUNCH(KID_JIT_OUTDIM, (k_jit_rows_bits<DIST, MKV, F>), dim3((unsigned)blocks), dim3(kJitThreads),        \
+             (size_t)nwords * 4, s, (const uint32_t*)bits, nwords, w0, w1, dv, ds, seed, num_row, num_col, step, \
+             out)
+#define JRB_F(MKV)             \
+  do {                         \
+    if (fin) JRB(MKV, true);   \
+    else JRB(MKV, false);      \
+  } while (0)
+      if (shrt) JRB_F(MOD_SHORT24);
+      else if (dv.small_quotient()) JRB_F(MOD_GM24);
+      else JRB_F(MOD_GM);
+#undef JRB_F
 #undef JRB
     } else if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && DIST != BPX_DIST_NORMAL && bits_fit(nwords) && ws &&
         ws_bytes >= (size_t)nwords * 4) {

@@ -71,6 +71,50 @@ struct DivU32 {
   BPX_HD bool small_quotient() const { return d > 256u && d < (1u << 24); }
 };
 
+// Short form of the same division, for divisors that admit it (about 3 in 4
+// below 2^18, 19999 among them): with m = ceil(2^(32+s) / d) < 2^32 and
+// m*d - 2^(32+s) <= 2^s, q = mulhi(n, m) >> s equals n / d for every u32 n
+// (Granlund & Montgomery 1994, Theorem 4.2 with N = 32).  When also
+// 512 < d < 2^23, the quotient is below 2^23 and the remainder is one signed
+// 24-bit multiply-add, n + q * (-d).
+struct DivShort {
+  uint32_t d, m, s;
+
+  // false when no such m exists
+  static bool make(uint32_t d, DivShort* out) {
+    if (d < 2) return false;
+    for (uint32_t sh = 0; sh < 32; ++sh) {
+      const unsigned __int128 two_p = (unsigned __int128)1 << (32 + sh);
+      const unsigned __int128 m = (two_p + d - 1) / d;
+      if (m >= ((unsigned __int128)1 << 32)) return false;
+      if (m * d - two_p <= ((unsigned __int128)1 << sh)) {
+        out->d = d;
+        out->m = (uint32_t)m;
+        out->s = sh;
+        return true;
+      }
+    }
+    return false;
+  }
+  static bool mad24_ok(uint32_t d) { return d > 512u && d < (1u << 23); }
+
+  BPX_HD uint32_t div(uint32_t n) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(m, n) >> s;
+#else
+    return (uint32_t)(((uint64_t)m * n) >> 32) >> s;
+#endif
+  }
+  // n % d, for divisors with mad24_ok(d)
+  BPX_HD uint32_t mod24(uint32_t n) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return n + (uint32_t)__mul24((int)div(n), -(int)d);
+#else
+    return n - div(n) * d;
+#endif
+  }
+};
+
 // lfsr88_random_integers(key, 0, clen-1) and (key, 1, clen) (tifunc.py:136-147):
 // the divisor (high + 1 - low) equals clen at every call site of the jitconn
 // kernels, so both reduce to bits % clen plus low.

@@ -94,3 +94,48 @@ int main(){
         subprocess.check_call(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "brainpy_amd", "csrc"), c, "-o", exe])
         out = subprocess.run([exe], capture_output=True, text=True)
         assert out.returncode == 0 and out.stdout.strip() == "0", out.stdout
+
+
+def test_short_divisor_exact():
+    # the short-form division q = mulhi(n, m) >> s (bpx_rng.h DivShort) is
+    # exact for every divisor it accepts: random numerators, both ends of the
+    # u32 range and the neighbours of multiples of d, for every d in 2..3000
+    # and a spread of larger ones (19999 = conn_len(1e-4) among them)
+    import subprocess
+    import tempfile
+    src = r'''
+#include <stdio.h>
+#include <stdint.h>
+#include "bpx_rng.h"
+int main(){
+  uint64_t x = 88172645463325252ull; long bad = 0, accepted = 0, tested = 0;
+  for (uint32_t d = 2; d < 3000000u; d = d < 3000u ? d + 1 : d * 3 / 2 + 7) {
+    bpx::DivShort ds;
+    uint32_t list[] = {d, 19999u, 99999u};
+    for (uint32_t dd : list) {
+      if (!bpx::DivShort::make(dd, &ds)) continue;
+      ++accepted;
+      for (int i = 0; i < 4000; ++i) {
+        x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+        uint32_t n = (uint32_t)x;
+        if (i < 32) n = i;
+        else if (i < 64) n = 0xFFFFFFFFu - (i - 32);
+        else if (i < 128) { uint64_t k = (x >> 32) % (0xFFFFFFFFull / dd); n = (uint32_t)(k * dd + (i & 1 ? 0 : dd - 1)); }
+        ++tested;
+        uint32_t q = ds.div(n);
+        uint32_t r = n + (uint32_t)((int64_t)q * -(int64_t)dd);   // the device's 24-bit multiply-add
+        if (q != n / dd) ++bad;
+        if (bpx::DivShort::mad24_ok(dd) && r != n % dd) ++bad;
+        if (ds.mod24(n) != n % dd) ++bad;
+      }
+    }
+  }
+  printf("%ld %ld %ld\n", bad, accepted, tested); return bad != 0 || accepted < 1000;
+}'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.cpp")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "t")
+        subprocess.check_call(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "brainpy_amd", "csrc"), c, "-o", exe])
+        out = subprocess.run([exe], capture_output=True, text=True)
+        assert out.returncode == 0 and out.stdout.split()[0] == "0", out.stdout

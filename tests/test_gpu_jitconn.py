@@ -121,3 +121,23 @@ def test_full_size_1m(od):
     # deterministic across calls (outdim_parallel writes each row once)
     y2 = bm.jitconn.event_mv_prob_homo(ev, 1.0, 1e-4, 42, shape=(n, n), outdim_parallel=od).cpu().numpy()
     np.testing.assert_array_equal(y, y2)
+
+
+@pytest.mark.parametrize("p", [1e-4, 2e-4, 5e-4])
+@pytest.mark.parametrize("dist,w0,w1", [("homo", 0.37, 0.0), ("homo", -1.25, 0.0), ("uniform", -0.5, 2.0)])
+def test_rows_kernel_bit_exact(p, dist, w0, w1):
+    # bool events, outdim_parallel, lane32: the row kernel sums each row's 32
+    # stream partials in stream order, as the oracle does, so non-integer
+    # homogeneous and uniform weights match bit-for-bit; these conn_len values
+    # (19999, 9999, 3999) take the short-form modulo when they admit it.
+    # Square: a stream spans (num_row + 1) >> 5 columns, taken from num_row.
+    shape = (40000, 40000)
+    ev = np.random.default_rng(int(p * 1e6)).random(shape[1]) < 0.05
+    if dist == "homo":
+        y = bm.jitconn.event_mv_prob_homo(ev, w0, p, 77, shape=shape, outdim_parallel=True).cpu().numpy()
+    else:
+        y = bm.jitconn.event_mv_prob_uniform(ev, w0, w1, p, 77, shape=shape, outdim_parallel=True).cpu().numpy()
+    ref = orc.jitconn_mv(ev, dist, 1, True, orc.LAYOUT_LANE32, shape[0], shape[1], np.float32(w0), np.float32(w1),
+                         np_ref.conn_len(p), 77)
+    assert np.count_nonzero(ref) > 100
+    np.testing.assert_array_equal(y, ref)
