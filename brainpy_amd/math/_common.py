@@ -52,3 +52,28 @@ def data_tensor(data):
 
 def acc_dtype(data_dtype):
     return torch.float64 if data_dtype == torch.float64 else torch.float32
+
+
+def needs_ad(*xs):
+    """True when an input takes part in differentiation or a torch.func
+    transform (it requires grad, or it is a functorch-wrapped tensor): the
+    operator then runs through its autograd Function; otherwise it launches
+    directly."""
+    wrapped = torch._C._functorch.is_functorch_wrapped_tensor
+    for x in xs:
+        if isinstance(x, torch.Tensor) and ((x.requires_grad and torch.is_grad_enabled()) or wrapped(x)):
+            return True
+    return False
+
+
+def batch_rows(info, in_dims, args, fn):
+    """vmap rule shared by the operators: run the batch one row at a time
+    (the reference's batching rule for these primitives is a sequential scan
+    over the batch, op_register/utils.py:16-37) and stack along dim 0."""
+    outs = []
+    for b in range(info.batch_size):
+        row = [a.select(d, b) if (d is not None and isinstance(a, torch.Tensor)) else a for a, d in zip(args, in_dims)]
+        outs.append(fn(*row))
+    if isinstance(outs[0], tuple):
+        return tuple(torch.stack(o) for o in zip(*outs)), tuple(0 for _ in outs[0])
+    return torch.stack(outs), 0

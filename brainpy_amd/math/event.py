@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from .. import _lib
-from ._common import acc_dtype, data_tensor, index_tensor, ndim
+from ._common import acc_dtype, batch_rows, data_tensor, index_tensor, ndim, needs_ad
 
 __all__ = ["csrmv", "csrmv_taichi", "info", "csrmv_fixed_order", "csr_to_csc", "csrmv_add", "TiledCSR"]
 
@@ -54,9 +54,16 @@ _TAICHI_INDEX = _PUBLIC_INDEX | {torch.int8, torch.int16, torch.uint8, np.dtype(
 
 
 def _as_array(x):
+    """Arrays pass through; python scalars and sequences take JAX's default
+    dtypes without x64 (float -> float32, int -> int32)."""
     if isinstance(x, (torch.Tensor, np.ndarray)):
         return x
-    return np.asarray(x)
+    a = np.asarray(x)
+    if a.dtype == np.float64:
+        a = a.astype(np.float32)
+    elif a.dtype == np.int64:
+        a = a.astype(np.int32)
+    return a
 
 
 def _events_tensor(events):
@@ -64,6 +71,61 @@ def _events_tensor(events):
     if ev.dtype == torch.bool or ev.dtype in (torch.float32, torch.float64):
         return ev
     return ev.to(torch.float32)
+
+
+def _launch(d, idx, ptr, ev, shape, transpose, rule):
+    # raw pointers go to the kernel: views are made dense first
+    d, idx, ptr, ev = d.contiguous(), idx.contiguous(), ptr.contiguous(), ev.contiguous()
+    m, n = int(shape[0]), int(shape[1])
+    if ptr.shape[0] != m + 1:
+        raise ValueError(f'indptr should have shape ({m + 1},), but we got {tuple(ptr.shape)}.')
+    out = torch.empty(n if transpose else m, dtype=acc_dtype(d.dtype), device=ev.device)
+    rc = _lib.lib().bpx_event_csrmv(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), d.shape[0], _lib.ptr(idx),
+                                    _lib.ptr(ptr), _lib.bpx_dtype(idx), idx.shape[0], _lib.ptr(ev),
+                                    _lib.bpx_dtype(ev), rule, m, n, int(bool(transpose)), _lib.ptr(out))
+    _lib.check(rc, "bpx_event_csrmv")
+    return out
+
+
+class _EventCSRMV(torch.autograd.Function):
+    """Differentiation rules of event/_csr_matvec.py:522-546: the tangent in
+    `data` is the same operator on the tangent; the cotangent of a scalar
+    weight is <ct, csrmv(1)>, of heterogeneous weights events[row] * ct[col]
+    (transpose) or events[col] * ct[row]; float events take the plain sparse
+    matvec of `data` with the cotangent and `transpose` flipped."""
+
+    @staticmethod
+    def forward(d, ev, idx, ptr, shape, transpose, rule):
+        return _launch(d, idx, ptr, ev, shape, transpose, rule)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        d, ev, idx, ptr, shape, transpose, rule = inputs
+        ctx.save_for_backward(d, ev, idx, ptr)
+        ctx.shape, ctx.transpose, ctx.rule = shape, transpose, rule
+
+    @staticmethod
+    def backward(ctx, ct):
+        from .sparse import _CSRMV, csr_to_coo
+        d, ev, idx, ptr = ctx.saved_tensors
+        gd = gev = None
+        if ctx.needs_input_grad[0]:
+            if d.shape[0] == 1:
+                ones = torch.ones(1, dtype=d.dtype, device=d.device)
+                base = _EventCSRMV.apply(ones, ev, idx, ptr, ctx.shape, ctx.transpose, ctx.rule)
+                gd = torch.dot(ct.to(base.dtype), base).reshape(1)
+            else:
+                row, col = csr_to_coo(idx, ptr)
+                e = ev.to(ct.dtype)
+                gd = e[row.long()] * ct[col.long()] if ctx.transpose else e[col.long()] * ct[row.long()]
+            gd = gd.to(d.dtype)
+        if ctx.needs_input_grad[1]:
+            gev = _CSRMV.apply(d, ct.to(acc_dtype(d.dtype)), idx, ptr, ctx.shape, not ctx.transpose).to(ev.dtype)
+        return gd, gev, None, None, None, None, None
+
+    @staticmethod
+    def vmap(info, in_dims, d, ev, idx, ptr, shape, transpose, rule):
+        return batch_rows(info, in_dims, (d, ev, idx, ptr, shape, transpose, rule), _EventCSRMV.apply)
 
 
 def _run(data, indices, indptr, events, shape, transpose, rule):
@@ -75,15 +137,9 @@ def _run(data, indices, indptr, events, shape, transpose, rule):
     ev = _events_tensor(events)
     if d.dtype == torch.float16:
         d = d.to(torch.float32)
-    m, n = int(shape[0]), int(shape[1])
-    if ptr.shape[0] != m + 1:
-        raise ValueError(f'indptr should have shape ({m + 1},), but we got {tuple(ptr.shape)}.')
-    out = torch.empty(n if transpose else m, dtype=acc_dtype(d.dtype), device=ev.device)
-    rc = _lib.lib().bpx_event_csrmv(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), d.shape[0], _lib.ptr(idx),
-                                    _lib.ptr(ptr), _lib.bpx_dtype(idx), idx.shape[0], _lib.ptr(ev),
-                                    _lib.bpx_dtype(ev), rule, m, n, int(bool(transpose)), _lib.ptr(out))
-    _lib.check(rc, "bpx_event_csrmv")
-    return out
+    if needs_ad(d, ev):
+        return _EventCSRMV.apply(d, ev, idx, ptr, (int(shape[0]), int(shape[1])), bool(transpose), rule)
+    return _launch(d, idx, ptr, ev, shape, transpose, rule)
 
 
 def csrmv(data, indices, indptr, events, *, shape: Tuple[int, int], transpose: bool = False) -> torch.Tensor:

@@ -23,6 +23,7 @@ import numpy as np
 import torch
 
 from .. import _lib
+from ._common import batch_rows, needs_ad
 
 __all__ = [
     "event_mv_prob_homo", "event_mv_prob_uniform", "event_mv_prob_normal",
@@ -79,7 +80,78 @@ def _general_checking(vector, shape, outdim_parallel, transpose):
     return (int(shape[0]), int(shape[1]))
 
 
-def _run(vector, event, dist, w0, w1, conn_prob, seed, shape, transpose, outdim_parallel, layout):
+def _launch(v, event, dist, w0, w1, clen, seed, num_row, num_col, outdim_parallel, layout):
+    v = v.contiguous()         # raw pointer to the kernel: views are made dense first
+    out = torch.empty(num_row, dtype=torch.float32, device=v.device)
+    L = _lib.lib()
+    wsb = int(L.bpx_jitconn_workspace(num_row, num_col)) if (event and outdim_parallel) else 0
+    ws = torch.empty(max(wsb, 4), dtype=torch.uint8, device=v.device) if wsb else None
+    rc = L.bpx_jitconn_mv(_lib.stream_ptr(), _lib.ptr(v), _lib.bpx_dtype(v), int(event), dist,
+                          np.float32(w0), np.float32(w1), clen, seed, num_row, num_col,
+                          int(outdim_parallel), _LAYOUTS[layout], _lib.ptr(out), _lib.ptr(ws), wsb)
+    _lib.check(rc, "bpx_jitconn_mv")
+    return out
+
+
+class _JitMV(torch.autograd.Function):
+    """Differentiation rules of the Taichi primitives
+    (jitconn/_matvec_taichi.py:138-172, 480-496; _event_matvec_taichi.py:
+    284-300): the cotangent of the vector (or of float events) is the
+    non-event operator on the cotangent with the matrix shape reversed and
+    `outdim_parallel` flipped; the cotangent of a homogeneous weight is
+    sum(op(ct, 1) * vector) with the same flags (the reference's rule, which
+    reads the cotangent as an input of the same operator, so it needs a
+    square matrix).  Uniform and normal weight parameters are not
+    differentiable, as in the reference.  `w` is the homogeneous weight as a
+    tensor when it takes part in differentiation, else None."""
+
+    @staticmethod
+    def forward(v, w, event, dist, w0, w1, clen, seed, num_row, num_col, outdim_parallel, layout):
+        if w is not None:
+            w0 = float(w.detach().reshape(-1)[0])
+        return _launch(v, event, dist, w0, w1, clen, seed, num_row, num_col, outdim_parallel, layout)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        v, w, event, dist, w0, w1, clen, seed, num_row, num_col, outdim_parallel, layout = inputs
+        ctx.save_for_backward(v, w)
+        ctx.args = (dist, w0, w1, clen, seed, num_row, num_col, outdim_parallel, layout)
+
+    @staticmethod
+    def backward(ctx, ct):
+        v, w = ctx.saved_tensors
+        dist, w0, w1, clen, seed, num_row, num_col, outdim_parallel, layout = ctx.args
+        gv = gw = None
+        ct = ct.to(torch.float32)
+        if ctx.needs_input_grad[0]:
+            gv = _JitMV.apply(ct, w, False, dist, w0, w1, clen, seed, num_col, num_row, not outdim_parallel,
+                              layout).to(v.dtype)
+        if ctx.needs_input_grad[1]:
+            if num_row != num_col:
+                raise ValueError(f'Shape mismatch, mat {(num_row, num_col)} @ vec ({num_row},).')
+            row = _launch(ct, False, dist, 1.0, w1, clen, seed, num_row, num_col, outdim_parallel, layout)
+            gw = torch.sum(row * v.to(torch.float32)).reshape(w.shape).to(w.dtype)
+        return gv, gw, None, None, None, None, None, None, None, None, None, None
+
+    @staticmethod
+    def vmap(info, in_dims, *args):
+        return batch_rows(info, in_dims, args, _JitMV.apply)
+
+
+def _weight_arg(x, name, differentiable):
+    """A weight parameter as (float value, tensor or None): the tensor is
+    kept only when it takes part in differentiation."""
+    if isinstance(x, torch.Tensor) and needs_ad(x):
+        if not differentiable:
+            raise AssertionError(f'Cannot differentiate through {name}.')
+        t = x.reshape(-1)
+        if t.numel() != 1:
+            raise ValueError('weight must be a 1D scalar.')
+        return 0.0, t
+    return _scalar(x), None
+
+
+def _run(vector, event, dist, w0, w1, conn_prob, seed, shape, transpose, outdim_parallel, layout, w_t=None):
     v = _lib.to_device(vector)
     if event:
         if v.dtype not in (torch.bool, torch.float16, torch.float32, torch.float64):
@@ -95,30 +167,30 @@ def _run(vector, event, dist, w0, w1, conn_prob, seed, shape, transpose, outdim_
     if layout not in _LAYOUTS:
         raise ValueError(f'layout must be one of {tuple(_LAYOUTS)}, but we got {layout}.')
     clen = conn_len(conn_prob)
-    out = torch.empty(num_row, dtype=torch.float32, device=v.device)
-    L = _lib.lib()
-    wsb = int(L.bpx_jitconn_workspace(num_row, num_col)) if (event and outdim_parallel) else 0
-    ws = torch.empty(max(wsb, 4), dtype=torch.uint8, device=v.device) if wsb else None
-    rc = L.bpx_jitconn_mv(_lib.stream_ptr(), _lib.ptr(v), _lib.bpx_dtype(v), int(event), dist,
-                          np.float32(w0), np.float32(w1), clen, _seed(seed), num_row, num_col,
-                          int(outdim_parallel), _LAYOUTS[layout], _lib.ptr(out), _lib.ptr(ws), wsb)
-    _lib.check(rc, "bpx_jitconn_mv")
-    return out
+    sd = _seed(seed)
+    if w_t is not None or needs_ad(v):
+        if w_t is not None:
+            w_t = w_t.to(device=v.device, dtype=torch.float32)
+        return _JitMV.apply(v, w_t, bool(event), dist, float(w0), float(w1), clen, sd, num_row, num_col,
+                            bool(outdim_parallel), layout)
+    return _launch(v, event, dist, w0, w1, clen, sd, num_row, num_col, outdim_parallel, layout)
 
 
 def event_mv_prob_homo(events, weight, conn_prob, seed: Optional[int] = None, *, shape: Tuple[int, int],
                        transpose: bool = False, outdim_parallel: bool = True, layout: str = "lane32"):
     """y = M @ events with a homogeneous weight at every generated connection.
     Reference: jitconn/_event_matvec.py:32-59, _event_matvec_taichi.py:340-408."""
-    return _run(events, True, _lib.DIST_HOMO, _scalar(weight), 0.0, conn_prob, seed, shape, transpose,
-                outdim_parallel, layout)
+    w, wt = _weight_arg(weight, 'weight', True)
+    return _run(events, True, _lib.DIST_HOMO, w, 0.0, conn_prob, seed, shape, transpose, outdim_parallel, layout,
+                wt)
 
 
 def event_mv_prob_uniform(events, w_low, w_high, conn_prob, seed: Optional[int] = None, *, shape: Tuple[int, int],
                           transpose: bool = False, outdim_parallel: bool = True, layout: str = "lane32"):
     """y = M @ events with U[w_low, w_high) weights.
     Reference: jitconn/_event_matvec.py:64-92, _event_matvec_taichi.py:751-824."""
-    return _run(events, True, _lib.DIST_UNIFORM, _scalar(w_low), _scalar(w_high), conn_prob, seed, shape,
+    return _run(events, True, _lib.DIST_UNIFORM, _weight_arg(w_low, 'w_low', False)[0],
+                _weight_arg(w_high, 'w_high', False)[0], conn_prob, seed, shape,
                 transpose, outdim_parallel, layout)
 
 
@@ -126,7 +198,8 @@ def event_mv_prob_normal(events, w_mu, w_sigma, conn_prob, seed: Optional[int] =
                          transpose: bool = False, outdim_parallel: bool = True, layout: str = "lane32"):
     """y = M @ events with N(w_mu, w_sigma) weights.
     Reference: jitconn/_event_matvec.py:97-126, _event_matvec_taichi.py:1168-1241."""
-    return _run(events, True, _lib.DIST_NORMAL, _scalar(w_mu), _scalar(w_sigma), conn_prob, seed, shape,
+    return _run(events, True, _lib.DIST_NORMAL, _weight_arg(w_mu, 'w_mu', False)[0],
+                _weight_arg(w_sigma, 'w_sigma', False)[0], conn_prob, seed, shape,
                 transpose, outdim_parallel, layout)
 
 
@@ -134,21 +207,24 @@ def mv_prob_homo(vector, weight, conn_prob, seed: Optional[int] = None, *, shape
                  transpose: bool = False, outdim_parallel: bool = True, layout: str = "lane32"):
     """y = M @ vector (dense float vector), homogeneous weight.
     Reference: jitconn/_matvec.py:27-100, _matvec_taichi.py:242-318."""
-    return _run(vector, False, _lib.DIST_HOMO, _scalar(weight), 0.0, conn_prob, seed, shape, transpose,
-                outdim_parallel, layout)
+    w, wt = _weight_arg(weight, 'weight', True)
+    return _run(vector, False, _lib.DIST_HOMO, w, 0.0, conn_prob, seed, shape, transpose, outdim_parallel, layout,
+                wt)
 
 
 def mv_prob_uniform(vector, w_low, w_high, conn_prob, seed: Optional[int] = None, *, shape: Tuple[int, int],
                     transpose: bool = False, outdim_parallel: bool = True, layout: str = "lane32"):
     """Reference: jitconn/_matvec.py:103-181, _matvec_taichi.py:527-600."""
-    return _run(vector, False, _lib.DIST_UNIFORM, _scalar(w_low), _scalar(w_high), conn_prob, seed, shape,
+    return _run(vector, False, _lib.DIST_UNIFORM, _weight_arg(w_low, 'w_low', False)[0],
+                _weight_arg(w_high, 'w_high', False)[0], conn_prob, seed, shape,
                 transpose, outdim_parallel, layout)
 
 
 def mv_prob_normal(vector, w_mu, w_sigma, conn_prob, seed: Optional[int] = None, *, shape: Tuple[int, int],
                    transpose: bool = False, outdim_parallel: bool = True, layout: str = "lane32"):
     """Reference: jitconn/_matvec.py:184-260, _matvec_taichi.py:814-887."""
-    return _run(vector, False, _lib.DIST_NORMAL, _scalar(w_mu), _scalar(w_sigma), conn_prob, seed, shape,
+    return _run(vector, False, _lib.DIST_NORMAL, _weight_arg(w_mu, 'w_mu', False)[0],
+                _weight_arg(w_sigma, 'w_sigma', False)[0], conn_prob, seed, shape,
                 transpose, outdim_parallel, layout)
 
 

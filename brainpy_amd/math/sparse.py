@@ -11,7 +11,7 @@ import numpy as np
 import torch
 
 from .. import _lib
-from ._common import acc_dtype, data_tensor, index_tensor, ndim
+from ._common import acc_dtype, batch_rows, data_tensor, index_tensor, ndim, needs_ad
 
 __all__ = ["csrmv", "csrmv_taichi", "csr_to_dense", "csr_to_coo", "coo_to_csr"]
 
@@ -31,6 +31,63 @@ def _as_array(x):
     return a
 
 
+def _launch(d, idx, ptr, v, shape, transpose):
+    # raw pointers go to the kernel: views (an expanded cotangent, a row of
+    # a batch) are made dense first
+    d, idx, ptr, v = d.contiguous(), idx.contiguous(), ptr.contiguous(), v.contiguous()
+    m, n = int(shape[0]), int(shape[1])
+    if v.shape[0] != (m if transpose else n):
+        raise ValueError(f'Shape mismatch, vector ({v.shape[0]},) with matrix {shape}, transpose={transpose}.')
+    if ptr.shape[0] != m + 1:
+        raise ValueError(f'indptr should have shape ({m + 1},), but we got {tuple(ptr.shape)}.')
+    out = torch.empty(n if transpose else m, dtype=acc_dtype(d.dtype), device=v.device)
+    rc = _lib.lib().bpx_csrmv(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), d.shape[0], _lib.ptr(idx),
+                              _lib.ptr(ptr), _lib.bpx_dtype(idx), idx.shape[0], _lib.ptr(v), _lib.bpx_dtype(v), m, n,
+                              int(bool(transpose)), _lib.ptr(out))
+    _lib.check(rc, "bpx_csrmv")
+    return out
+
+
+class _CSRMV(torch.autograd.Function):
+    """Differentiation rules of sparse/_csr_mv.py:211-244: tangents pass
+    through the same operator; the cotangent of a scalar weight is
+    <ct, csrmv(1, x)>, of heterogeneous weights x[row] * ct[col] (transpose)
+    or x[col] * ct[row]; the vector's is csrmv(data, ct) with `transpose`
+    flipped."""
+
+    @staticmethod
+    def forward(d, v, idx, ptr, shape, transpose):
+        return _launch(d, idx, ptr, v, shape, transpose)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        d, v, idx, ptr, shape, transpose = inputs
+        ctx.save_for_backward(d, v, idx, ptr)
+        ctx.shape, ctx.transpose = shape, transpose
+
+    @staticmethod
+    def backward(ctx, ct):
+        d, v, idx, ptr = ctx.saved_tensors
+        gd = gv = None
+        if ctx.needs_input_grad[0]:
+            if d.shape[0] == 1:
+                ones = torch.ones(1, dtype=d.dtype, device=d.device)
+                base = _CSRMV.apply(ones, v, idx, ptr, ctx.shape, ctx.transpose)
+                gd = torch.dot(ct.to(base.dtype), base).reshape(1)
+            else:
+                row, col = csr_to_coo(idx, ptr)
+                x = v.to(ct.dtype)
+                gd = x[row.long()] * ct[col.long()] if ctx.transpose else x[col.long()] * ct[row.long()]
+            gd = gd.to(d.dtype)
+        if ctx.needs_input_grad[1]:
+            gv = _CSRMV.apply(d, ct.to(v.dtype), idx, ptr, ctx.shape, not ctx.transpose)
+        return gd, gv, None, None, None, None
+
+    @staticmethod
+    def vmap(info, in_dims, d, v, idx, ptr, shape, transpose):
+        return batch_rows(info, in_dims, (d, v, idx, ptr, shape, transpose), _CSRMV.apply)
+
+
 def _run(data, indices, indptr, vector, shape, transpose):
     d = data_tensor(data)
     if d.dtype == torch.float16:
@@ -43,17 +100,9 @@ def _run(data, indices, indptr, vector, shape, transpose):
     acc = acc_dtype(d.dtype)
     if v.dtype != acc:
         v = v.to(acc)
-    m, n = int(shape[0]), int(shape[1])
-    if v.shape[0] != (m if transpose else n):
-        raise ValueError(f'Shape mismatch, vector ({v.shape[0]},) with matrix {shape}, transpose={transpose}.')
-    if ptr.shape[0] != m + 1:
-        raise ValueError(f'indptr should have shape ({m + 1},), but we got {tuple(ptr.shape)}.')
-    out = torch.empty(n if transpose else m, dtype=acc, device=v.device)
-    rc = _lib.lib().bpx_csrmv(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), d.shape[0], _lib.ptr(idx),
-                              _lib.ptr(ptr), _lib.bpx_dtype(idx), idx.shape[0], _lib.ptr(v), _lib.bpx_dtype(v), m, n,
-                              int(bool(transpose)), _lib.ptr(out))
-    _lib.check(rc, "bpx_csrmv")
-    return out
+    if needs_ad(d, v):
+        return _CSRMV.apply(d, v, idx, ptr, (int(shape[0]), int(shape[1])), bool(transpose))
+    return _launch(d, idx, ptr, v, shape, transpose)
 
 
 def csrmv(data, indices, indptr, vector, *, shape: Tuple[int, int], transpose: bool = False,
