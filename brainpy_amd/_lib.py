@@ -57,6 +57,12 @@ SIGNATURES = {
     "bpx_lif_coba_step": ([_vp, _i64, _i32, _i32, _vp, _vp, _vp, ctypes.POINTER(_vp), _vp, _vp, _vp, _f32, _vp,
                            _f32, _f32, _f32, _f32, _f32, _f32, _f32, _f32], _i32),
     "bpx_copy": ([_vp, _vp, _vp, _i64], _i32),
+    "bpx_csr_on_pre_update": ([_vp, _vp, _i32, _vp, _vp, _i32, _i64, _vp, _i32, _i64, _vp, _i64,
+                               ctypes.c_double, ctypes.c_double, _vp], _i32),
+    "bpx_csc_on_post_update": ([_vp, _vp, _i32, _vp, _vp, _vp, _i32, _i64, _vp, _i32, _i64, _vp, _i64,
+                                ctypes.c_double, ctypes.c_double, _vp], _i32),
+    "bpx_dense_on_pre": ([_vp, _vp, _i32, _vp, _i32, _i64, _i64, _vp, ctypes.c_double, ctypes.c_double, _vp], _i32),
+    "bpx_dense_on_post": ([_vp, _vp, _i32, _vp, _i32, _i64, _i64, _vp, ctypes.c_double, ctypes.c_double, _vp], _i32),
     "bpx_tiled_csr_layout": ([_i64, _i64, _i64, _i32, ctypes.POINTER(TiledCSRPlan), ctypes.POINTER(_sz)], _i32),
     "bpx_tiled_csr_build": ([_vp, _vp, _i32, _i64, _vp, _vp, _i32, ctypes.POINTER(TiledCSRPlan), _vp, _sz], _i32),
     "bpx_event_csrmv_tiled": ([_vp, ctypes.POINTER(TiledCSRPlan), _vp, _i32, _i32, _i32, _vp], _i32),

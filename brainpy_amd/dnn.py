@@ -7,7 +7,9 @@ with ``shape=(pre_num, post_num)``, and the just-in-time connectivity layers
 ``JitFP{Homo,Uniform,Normal}Linear`` (:805-1041) and
 ``EventJitFP{Homo,Uniform,Normal}Linear`` (:1044-1280) over
 ``bm.jitconn.(event_)mv_prob_*`` with ``shape=(num_out, num_in)`` and
-``outdim_parallel = not atomic``.  Inputs with leading batch dimensions are
+``outdim_parallel = not atomic``; the STDP weight updates of the same file
+(``csr_on_pre_update``, ``csc_on_post_update``, ``dense_on_pre/post``,
+``stdp_update``).  Inputs with leading batch dimensions are
 flattened to rows and each row goes through the operator (the reference maps
 the operator over rows with ``jax.vmap``, which its primitives run as a
 sequential scan).
@@ -51,6 +53,98 @@ def _rows(x, fn):
     return y.reshape(lead + (y.shape[-1],))
 
 
+def _bounds(w_min, w_max):
+    return (-np.inf if w_min is None else float(w_min)), (np.inf if w_max is None else float(w_max))
+
+
+def _plast_args(w, spike, trace, out):
+    w = _lib.to_device(w)
+    if w.dtype not in (torch.float32, torch.float64):
+        w = w.to(torch.float32)
+    sp = _lib.to_device(spike)
+    if sp.dtype not in (torch.bool, torch.float32, torch.float64):
+        sp = sp.to(torch.float32)
+    tr = _lib.to_device(trace).to(w.dtype).contiguous()
+    if out is None:
+        out = torch.empty_like(w)
+    elif out.dtype != w.dtype or out.shape != w.shape or not out.is_contiguous():
+        raise ValueError("out must be a contiguous tensor of the weights' shape and dtype")
+    return w, sp, tr, out
+
+
+def csr_on_pre_update(w, indices, indptr, spike, trace, w_min=None, w_max=None, *, out=None):
+    """STDP on pre-synaptic spikes over a CSR: for every spiking pre neuron i
+    and synapse k of row i, w[k] = min(max(w[k] + trace[indices[k]], w_min),
+    w_max) (dnn/linear.py:650-676).  Returns the new weights (`out`, which
+    may be `w` itself for an in-place update)."""
+    w, sp, tr, out = _plast_args(w, spike, trace, out)
+    idx = _lib.to_device(indices)
+    ptr = _lib.to_device(indptr).to(idx.dtype)
+    if idx.dtype not in (torch.int32, torch.int64):
+        idx, ptr = idx.to(torch.int32), ptr.to(torch.int32)
+    if w.numel() != idx.numel():
+        raise ValueError(f'The shape of weight should be the same as the shape of sparse weight {tuple(idx.shape)}.')
+    if ptr.numel() != sp.numel() + 1:
+        raise ValueError(f'indptr must have {sp.numel() + 1} entries (one per pre neuron + 1).')
+    lo, hi = _bounds(w_min, w_max)
+    rc = _lib.lib().bpx_csr_on_pre_update(_lib.stream_ptr(), _lib.ptr(w), _lib.bpx_dtype(w), _lib.ptr(idx),
+                                          _lib.ptr(ptr), _lib.bpx_dtype(idx), idx.numel(), _lib.ptr(sp),
+                                          _lib.bpx_dtype(sp), sp.numel(), _lib.ptr(tr), tr.numel(), lo, hi,
+                                          _lib.ptr(out))
+    _lib.check(rc, "bpx_csr_on_pre_update")
+    return out
+
+
+def csc_on_post_update(w, post_ids, indptr, w_ids, spike, trace, w_min=None, w_max=None, *, out=None):
+    """STDP on post-synaptic spikes over the CSC view of a CSR (from
+    ``brainpy_amd.math.event.csr_to_csc`` with data = arange(nnz)): for every
+    spiking post neuron j and entry k of column j, w[w_ids[k]] is moved by
+    trace[post_ids[k]] (the pre id) and clipped (dnn/linear.py:679-700)."""
+    w, sp, tr, out = _plast_args(w, spike, trace, out)
+    pre = _lib.to_device(post_ids)
+    if pre.dtype not in (torch.int32, torch.int64):
+        pre = pre.to(torch.int32)
+    cp = _lib.to_device(indptr).to(pre.dtype)
+    wid = _lib.to_device(w_ids).to(pre.dtype)
+    if w.numel() != pre.numel() or wid.numel() != pre.numel():
+        raise ValueError('w, post_ids and w_ids must have one entry per synapse.')
+    if cp.numel() != sp.numel() + 1:
+        raise ValueError(f'indptr must have {sp.numel() + 1} entries (one per post neuron + 1).')
+    lo, hi = _bounds(w_min, w_max)
+    rc = _lib.lib().bpx_csc_on_post_update(_lib.stream_ptr(), _lib.ptr(w), _lib.bpx_dtype(w), _lib.ptr(pre),
+                                           _lib.ptr(cp), _lib.ptr(wid), _lib.bpx_dtype(pre), pre.numel(),
+                                           _lib.ptr(sp), _lib.bpx_dtype(sp), sp.numel(), _lib.ptr(tr), tr.numel(),
+                                           lo, hi, _lib.ptr(out))
+    _lib.check(rc, "bpx_csc_on_post_update")
+    return out
+
+
+def _dense_on_spike(fn, weight, spike, trace, w_min, w_max, out, rows):
+    w, sp, tr, out = _plast_args(weight, spike, trace, out)
+    if w.ndim != 2:
+        raise ValueError('weight must be a 2D [num_pre, num_post] matrix.')
+    npre, npost = w.shape
+    if sp.numel() != (npre if rows else npost) or tr.numel() != (npost if rows else npre):
+        raise ValueError(f'spike / trace sizes do not match the weight shape {tuple(w.shape)}.')
+    lo, hi = _bounds(w_min, w_max)
+    rc = getattr(_lib.lib(), fn)(_lib.stream_ptr(), _lib.ptr(w), _lib.bpx_dtype(w), _lib.ptr(sp), _lib.bpx_dtype(sp),
+                                 npre, npost, _lib.ptr(tr), lo, hi, _lib.ptr(out))
+    _lib.check(rc, fn)
+    return out
+
+
+def dense_on_pre(weight, spike, trace, w_min, w_max, *, out=None):
+    """Rows of spiking pre neurons += trace (over post), clipped
+    (dnn/linear.py:238-254)."""
+    return _dense_on_spike("bpx_dense_on_pre", weight, spike, trace, w_min, w_max, out, True)
+
+
+def dense_on_post(weight, spike, trace, w_min, w_max, *, out=None):
+    """Columns of spiking post neurons += trace (over pre), clipped
+    (dnn/linear.py:257-275)."""
+    return _dense_on_spike("bpx_dense_on_post", weight, spike, trace, w_min, w_max, out, False)
+
+
 class _CSRLayer:
     def __init__(self, conn, weight, sharding=None, mode=None, name=None, transpose=True):
         if sharding is not None:
@@ -71,6 +165,29 @@ class _CSRLayer:
     @property
     def shape(self):
         return (self.conn.pre_num, self.conn.post_num)
+
+    def stdp_update(self, on_pre=None, on_post=None, w_min=None, w_max=None):
+        """Event-driven STDP on the layer's weights (dnn/linear.py:517-543):
+        `on_pre` / `on_post` = {'spike': ..., 'trace': ...}; the CSC view for
+        post-synaptic updates is built once."""
+        if not isinstance(self.weight, torch.Tensor):
+            raise ValueError('When using STDP to update synaptic weights, the weight cannot be a scalar.')
+        if tuple(self.weight.shape) != tuple(self.indices.shape):
+            raise ValueError(f'The shape of weight should be the same as the shape of sparse weight '
+                             f'{tuple(self.weight.shape)}.')
+        if on_pre is not None:
+            self.weight = csr_on_pre_update(self.weight, self.indices, self.indptr, on_pre['spike'],
+                                            on_pre['trace'], w_min, w_max, out=self.weight)
+        if on_post is not None:
+            if not hasattr(self, '_pre_ids'):
+                rows, colptr, order = _event.csr_to_csc(self.indices, self.indptr, self.shape,
+                                                        data=np.arange(self.indices.numel()))
+                dev = _lib.device()
+                self._pre_ids = torch.from_numpy(rows).to(dev)
+                self._post_indptr = torch.from_numpy(colptr).to(dev)
+                self.w_indices = torch.from_numpy(np.asarray(order, np.int32)).to(dev)
+            self.weight = csc_on_post_update(self.weight, self._pre_ids, self._post_indptr, self.w_indices,
+                                             on_post['spike'], on_post['trace'], w_min, w_max, out=self.weight)
 
     def __call__(self, x):
         return self.update(x)

@@ -268,6 +268,38 @@ int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, i
                  const bpx_net_params* params, int64_t step, const int64_t* d_step);
 int bpx_net_advance(void* stream, int64_t* d_step, int64_t delta);
 
+/* ---------------------------------------------------------------------------
+ * Event-driven STDP weight updates.
+ *
+ * bpx_csr_on_pre_update replaces csr_on_pre_update (dnn/linear.py:650-676):
+ *   out = w; for every pre neuron i with spike[i] != 0 and k in row i:
+ *   out[k] = min(max(out[k] + trace[indices[k]], w_min), w_max).
+ * bpx_csc_on_post_update replaces csc_on_post_update (dnn/linear.py:679-700)
+ * on the CSC view of the same synapses (csr2csc, connect/base.py:645):
+ *   for every post neuron j with spike[j] != 0 and k in column j:
+ *   out[w_ids[k]] = min(max(out[w_ids[k]] + trace[pre_ids[k]], w_min), w_max).
+ * bpx_dense_on_pre / bpx_dense_on_post replace dense_on_pre / dense_on_post
+ * (dnn/linear.py:238-275) on a row-major [num_pre, num_post] matrix:
+ *   rows of spiking pre neurons += trace[post], or columns of spiking post
+ *   neurons += trace[pre], clipped to [w_min, w_max].
+ * w, trace and out share w_dtype (F32 or F64); spike is BOOL, F32 or F64
+ * (non-zero = spike); indices, indptr (colptr), pre_ids and w_ids share
+ * index_dtype (I32 or I64).  out may equal w (in place); otherwise w is
+ * copied into out first.  Each weight is written at most once per call, so
+ * results do not depend on scheduling.
+ * ------------------------------------------------------------------------- */
+int bpx_csr_on_pre_update(void* stream, const void* w, int w_dtype, const void* indices, const void* indptr,
+                          int index_dtype, int64_t nnz, const void* spike, int spike_dtype, int64_t num_pre,
+                          const void* trace, int64_t num_post, double w_min, double w_max, void* out);
+int bpx_csc_on_post_update(void* stream, const void* w, int w_dtype, const void* pre_ids, const void* colptr,
+                           const void* w_ids, int index_dtype, int64_t nnz, const void* spike, int spike_dtype,
+                           int64_t num_post, const void* trace, int64_t num_pre, double w_min, double w_max,
+                           void* out);
+int bpx_dense_on_pre(void* stream, const void* w, int w_dtype, const void* spike, int spike_dtype, int64_t num_pre,
+                     int64_t num_post, const void* trace, double w_min, double w_max, void* out);
+int bpx_dense_on_post(void* stream, const void* w, int w_dtype, const void* spike, int spike_dtype, int64_t num_pre,
+                      int64_t num_post, const void* trace, double w_min, double w_max, void* out);
+
 /* Plain device copy (bandwidth calibration of the HBM counters). */
 int bpx_copy(void* stream, const void* src, void* dst, int64_t bytes);
 

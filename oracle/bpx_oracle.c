@@ -725,3 +725,56 @@ int64_t orc_coba_net_run(int32_t n_exc, int32_t n_inh, const int32_t* indptr, co
   free(V); free(tl); free(ge); free(gi); free(oute); free(outi); free(sp);
   return total;
 }
+
+/* ---------------------------------------------------------------------------
+ * STDP weight updates (dnn/linear.py:238-275, 650-700), f32.
+ * np.maximum / np.minimum propagate NaN (fmaxf/fminf would not).
+ * ------------------------------------------------------------------------- */
+static float np_max_f(float a, float b) { return (a != a || b != b) ? NAN : (a > b ? a : b); }
+static float np_min_f(float a, float b) { return (a != a || b != b) ? NAN : (a < b ? a : b); }
+
+/* _cpu_csr_on_pre_update (dnn/linear.py:650-660) */
+int orc_csr_on_pre_update(const float* w, const int32_t* indices, const int32_t* indptr, int64_t nnz,
+                          const uint8_t* spike, int64_t num_pre, const float* trace, int64_t num_post,
+                          float w_min, float w_max, float* out) {
+  for (int64_t k = 0; k < nnz; ++k) out[k] = w[k];
+  for (int64_t i = 0; i < num_pre; ++i) {
+    if (!spike[i]) continue;
+    for (int64_t k = indptr[i]; k < indptr[i + 1]; ++k) {
+      int64_t j = indices[k];
+      if (j < 0 || j >= num_post) return -1;
+      out[k] = np_min_f(np_max_f(out[k] + trace[j], w_min), w_max);
+    }
+  }
+  return 0;
+}
+
+/* _cpu_csc_on_pre_update (dnn/linear.py:679-689), used by csc_on_post_update */
+int orc_csc_on_post_update(const float* w, int64_t nnz, const int32_t* pre_ids, const int32_t* colptr,
+                           const int32_t* w_ids, const uint8_t* spike, int64_t num_post, const float* trace,
+                           int64_t num_pre, float w_min, float w_max, float* out) {
+  for (int64_t k = 0; k < nnz; ++k) out[k] = w[k];
+  for (int64_t i = 0; i < num_post; ++i) {
+    if (!spike[i]) continue;
+    for (int64_t k = colptr[i]; k < colptr[i + 1]; ++k) {
+      int64_t j = pre_ids[k], l = w_ids[k];
+      if (j < 0 || j >= num_pre || l < 0 || l >= nnz) return -1;
+      out[l] = np_min_f(np_max_f(out[l] + trace[j], w_min), w_max);
+    }
+  }
+  return 0;
+}
+
+/* _cpu_dense_on_pre / _cpu_dense_on_post (dnn/linear.py:238-244, 257-263):
+ * np.clip of a row (pre spike) or column (post spike) of [num_pre, num_post] */
+int orc_dense_on_spike(int rows, const float* w, const uint8_t* spike, int64_t num_pre, int64_t num_post,
+                       const float* trace, float w_min, float w_max, float* out) {
+  for (int64_t k = 0; k < num_pre * num_post; ++k) out[k] = w[k];
+  for (int64_t r = 0; r < num_pre; ++r)
+    for (int64_t c = 0; c < num_post; ++c) {
+      if (!spike[rows ? r : c]) continue;
+      float x = out[r * num_post + c] + trace[rows ? c : r];
+      out[r * num_post + c] = np_min_f(np_max_f(x, w_min), w_max);
+    }
+  return 0;
+}

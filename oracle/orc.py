@@ -65,6 +65,9 @@ def lib():
         d = ctypes.c_double
         L.orc_lif_coba_step_f64.argtypes = [i64, i32, i32, vp, vp, vp, fp, vp, vp, d, vp,
                                             d, d, d, d, d, d, d, d]
+        L.orc_csr_on_pre_update.argtypes = [vp, vp, vp, i64, vp, i64, vp, i64, f32, f32, vp]
+        L.orc_csc_on_post_update.argtypes = [vp, i64, vp, vp, vp, vp, i64, vp, i64, f32, f32, vp]
+        L.orc_dense_on_spike.argtypes = [i32, vp, vp, i64, i64, vp, f32, f32, vp]
         _lib = L
     return _lib
 
@@ -232,3 +235,42 @@ def coba_net_run(n_exc, n_inh, indptr, indices, w_exc, w_inh, I_ext, V0, steps, 
                                  np.float32(tau_e), np.float32(tau_i), np.float32(E_e), np.float32(E_i),
                                  np.float32(dt), _p(r))
     return tot, r
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def _u8(a):
+    return np.ascontiguousarray(np.asarray(a) != 0, np.uint8)
+
+
+def csr_on_pre_update(w, indices, indptr, spike, trace, w_min=-np.inf, w_max=np.inf):
+    w, ind, ptr, sp, tr = _f32(w), _i32(indices), _i32(indptr), _u8(spike), _f32(trace)
+    out = np.empty_like(w)
+    rc = lib().orc_csr_on_pre_update(_p(w), _p(ind), _p(ptr), w.size, _p(sp), sp.size, _p(tr), tr.size,
+                                     np.float32(w_min), np.float32(w_max), _p(out))
+    assert rc == 0
+    return out
+
+
+def csc_on_post_update(w, pre_ids, colptr, w_ids, spike, trace, w_min=-np.inf, w_max=np.inf):
+    w, pre, cp, wid, sp, tr = _f32(w), _i32(pre_ids), _i32(colptr), _i32(w_ids), _u8(spike), _f32(trace)
+    out = np.empty_like(w)
+    rc = lib().orc_csc_on_post_update(_p(w), w.size, _p(pre), _p(cp), _p(wid), _p(sp), sp.size, _p(tr), tr.size,
+                                      np.float32(w_min), np.float32(w_max), _p(out))
+    assert rc == 0
+    return out
+
+
+def dense_on_spike(rows, weight, spike, trace, w_min=-np.inf, w_max=np.inf):
+    w, sp, tr = _f32(weight), _u8(spike), _f32(trace)
+    out = np.empty_like(w)
+    rc = lib().orc_dense_on_spike(int(rows), _p(w), _p(sp), w.shape[0], w.shape[1], _p(tr), np.float32(w_min),
+                                  np.float32(w_max), _p(out))
+    assert rc == 0
+    return out
