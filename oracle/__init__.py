@@ -1,7 +1,8 @@
 """CPU oracle for the event-driven synaptic hot path -- TEST INFRASTRUCTURE.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may import this package, and only as the checker.  The product
+``bench.py`` may import this package, and only as the reference the GPU results are
+compared with (or, in the CPU baseline, timed as the CPU path).  The product
 path (``brainpy_amd``) never imports it and has no CPU fallback.
 
 * ``oracle.orc``   -- ctypes binding of ``liborc.so`` (the C restatement in

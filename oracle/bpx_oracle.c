@@ -1,7 +1,7 @@
 /*
  * bpx_oracle.c -- CPU restatement of BrainPy's event-driven synaptic hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity reference for the HIP
  * library in brainpy_amd/csrc.  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py load it; the product path never does.
  *

@@ -12,7 +12,7 @@ from tests.helpers import (bernoulli, bf16_bits_to_f32, dense_of, f32_to_bf16_bi
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(100, 200), (200, 100), (10, 1000), (1000, 10), (1, 1), (257, 129), (3000, 50)]
-RTOL = 1e-5   # per-call FP32 current vs FP64, relative
+RTOL = 1e-5   # per-call FP32 current vs FP64, relative (the stated per-step tolerance)
 
 
 def _tol_check(y, ref, scale):
