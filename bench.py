@@ -212,7 +212,7 @@ def _current_error_f64(torch, specs, bf16, steps, stored_weights=False):
     gi = torch.zeros(T, dtype=torch.float64, device=d)
     worst = torch.zeros(len(specs), dtype=torch.float64, device=d)
     for _ in range(steps):
-        base = (b.i & 1) * 2 * T
+        base = (b.i % b.n_buf) * 2 * T
         ce = b.counts[base:base + T].double()
         ci = b.counts[base + T:base + 2 * T].double()
         V = b.V.double()

@@ -11,16 +11,17 @@
 //      133-134); the counts are zeroed for reuse; then the fused LIF/Expon/
 //      COBA update (bpx_lif.h) in the reference order.
 //   2. deliver: every neuron that spiked in step i regenerates or reads its
-//      targets and adds 1 to their count for step i+1 (integer atomics into
-//      the other count buffer).  Connectivity is either just-in-time (the
+//      targets and adds 1 to their count for step i+D, D the projection's
+//      delay in steps (>= 1; integer atomics into that step's count buffer).  Connectivity is either just-in-time (the
 //      lane32 scatter form of jitconn/_event_matvec_taichi.py:92-123,
 //      outdim_parallel=False: 32 lanes per spiking neuron, stream key
 //      seed + pre*32 + lane) or a stored CSR per network.
 // Integer counts make every conductance update exact and independent of
 // scheduling, so a batched run, a network run alone, and the serial CPU
 // restatement give the same spike raster.
-// Double-buffered counts remove the grid-wide barrier between the two
-// phases: step i reads/zeroes buffer i%2 and fills buffer (i+1)%2.
+// A ring of n_buf count buffers removes the grid-wide barrier between the
+// two phases: step i reads/zeroes buffer i % n_buf and fills buffers
+// (i + D) % n_buf, D < n_buf.
 #include "bpx_common.h"
 #include "bpx_lif.h"
 #include "bpx_rng.h"
@@ -43,7 +44,8 @@ struct NetArgs {
   float* t_last;
   float* g_exc;
   float* g_inh;
-  uint32_t* counts;            // [2 buffers][2 kinds][total]
+  uint32_t* counts;            // [n_buf buffers][2 kinds][total]
+  int n_buf;                   // count buffers: step i reads buffer i % n_buf
   uint8_t* spikes;             // [total] spike flags of this step (nullable)
   unsigned long long* events;  // [n_nets] delivered synaptic events (nullable)
   unsigned int* spike_count;   // [n_nets] spikes (nullable)
@@ -72,8 +74,15 @@ __global__ __launch_bounds__(kNetThreads) void k_net_step(NetArgs a) {
   const int64_t total = a.total;
   const int64_t step = a.d_step ? *a.d_step + a.step : a.step;
   const float t = (float)(step - d.t0_step) * a.dt;      // network-local time
-  uint32_t* cur = a.counts + (step & 1) * 2 * total;
-  uint32_t* nxt = a.counts + ((step + 1) & 1) * 2 * total;
+  const int64_t nb = a.n_buf;
+  uint32_t* cur = a.counts + (step % nb) * 2 * total;
+  // spikes of projection k (E2E, E2I, I2E, I2I) with delay D_k steps are
+  // counted into the buffer step i + D_k reads (1 <= D_k < n_buf)
+  auto buf_of = [&](int k) -> uint32_t* {
+    int64_t dk = d.delay[k];
+    dk = dk < 1 ? 1 : (dk > nb - 1 ? nb - 1 : dk);
+    return a.counts + ((step + dk) % nb) * 2 * total;
+  };
   if (j < N) {
     const int64_t gid = d.off + j;
     const uint32_t ce = cur[gid], ci = cur[total + gid];
@@ -105,13 +114,14 @@ __global__ __launch_bounds__(kNetThreads) void k_net_step(NetArgs a) {
   for (int e = grp; e < nspk; e += kNetThreads / 32) {
     const int p = s_spk[e];
     const bool exc = p < d.n_exc;
-    uint32_t* cnt = nxt + (exc ? 0 : total) + d.off;     // post ids are network-local
+    const int64_t kind = exc ? 0 : total;                // excitatory or inhibitory conductance
     if constexpr (CONN == BPX_NET_JIT) {
       const int col = exc ? p : p - d.n_exc;
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         // projections from this population: to E (post offset 0), to I (offset n_exc)
         const int k = exc ? q : 2 + q;                   // E2E, E2I, I2E, I2I
+        uint32_t* cnt = buf_of(k) + kind + d.off;        // post ids are network-local
         const int64_t num_row = q == 0 ? d.n_exc : d.n_inh;
         const int post_off = q == 0 ? 0 : d.n_exc;
         if (num_row <= 0) continue;
@@ -129,11 +139,14 @@ __global__ __launch_bounds__(kNetThreads) void k_net_step(NetArgs a) {
         }
       }
     } else {
+      // targets in E belong to E2E / I2E, targets in I to E2I / I2I
+      uint32_t* cnt_e = buf_of(exc ? 0 : 2) + kind + d.off;
+      uint32_t* cnt_i = buf_of(exc ? 1 : 3) + kind + d.off;
       const int32_t s = d.indptr[p], t2 = d.indptr[p + 1];
       for (int32_t q = s + (int32_t)l; q < t2; q += 32) {
         const int32_t tgt = d.indices[q];
         if ((uint32_t)tgt < (uint32_t)N) {
-          atomicAdd(cnt + tgt, 1u);
+          atomicAdd((tgt < d.n_exc ? cnt_e : cnt_i) + tgt, 1u);
           ++ev;
         }
       }
@@ -175,10 +188,11 @@ void bpx_net_prepare(bpx_net_desc* d) {
 }
 
 int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, int n_tiles, int conn_kind,
-                 int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts,
+                 int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts, int n_buf,
                  uint8_t* spikes, unsigned long long* events, unsigned int* spike_count,
                  const bpx_net_params* prm, int64_t step, const int64_t* d_step) {
   if (!nets || !tiles || !prm) return BPX_ERR_ARG;
+  if (n_buf < 2 || (!d_step && step < 0)) return BPX_ERR_ARG;
   if (n_tiles < 0 || total < 0) return BPX_ERR_SHAPE;
   if (conn_kind != BPX_NET_JIT && conn_kind != BPX_NET_CSR) return BPX_ERR_ARG;
   if (n_tiles == 0) return BPX_OK;
@@ -191,6 +205,7 @@ int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, i
   a.g_exc = g_exc;
   a.g_inh = g_inh;
   a.counts = counts;
+  a.n_buf = n_buf;
   a.spikes = spikes;
   a.events = events;
   a.spike_count = spike_count;

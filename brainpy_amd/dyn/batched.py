@@ -3,16 +3,18 @@ launch per step (``bpx_net_step``).
 
 Each network is the ``EICOBA_PostAlign`` model of
 examples/dynamics_simulation/COBA.py:58-109 (LifRefLTC or LifRef neurons,
-Expon + COBA synapses, homogeneous weights, no delay) with either
-just-in-time connectivity (``bm.jitconn.event_mv_prob_homo`` with
+Expon + COBA synapses, homogeneous weights, optional projection delays)
+with either just-in-time connectivity (``bm.jitconn.event_mv_prob_homo`` with
 ``outdim_parallel=False``: every spiking neuron regenerates its targets from
 LFSR88 streams, nothing stored) or a stored CSR.  Per step the kernel
-updates all neurons (adding the conductance delivered by the previous step's
-spikes) and then delivers this step's spikes as integer counts, which makes
+updates all neurons (adding the conductance delivered by the spikes of the
+step one delay earlier) and then delivers this step's spikes as integer
+counts into the step they reach, which makes
 every network's trajectory exact and independent of what else runs in the
 batch.
 """
 import ctypes
+import math
 
 import numpy as np
 import torch
@@ -30,7 +32,8 @@ class NetDesc(ctypes.Structure):
     _fields_ = [("off", ctypes.c_int64), ("n_exc", ctypes.c_int32), ("n_inh", ctypes.c_int32),
                 ("seed", ctypes.c_uint32 * 4), ("clen", ctypes.c_int32 * 4), ("div", (ctypes.c_uint32 * 4) * 4),
                 ("w_exc", ctypes.c_float), ("w_inh", ctypes.c_float), ("I_ext", ctypes.c_float),
-                ("t0_step", ctypes.c_int32), ("indptr", ctypes.c_void_p), ("indices", ctypes.c_void_p)]
+                ("t0_step", ctypes.c_int32), ("delay", ctypes.c_int32 * 4), ("indptr", ctypes.c_void_p),
+                ("indices", ctypes.c_void_p)]
 
 
 class NetParams(ctypes.Structure):
@@ -46,7 +49,7 @@ def _register():
         L.bpx_net_prepare.restype = None
         L.bpx_net_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.POINTER(NetParams), ctypes.c_int64, ctypes.c_void_p]
         L.bpx_net_step.restype = ctypes.c_int
         L.bpx_net_advance.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
@@ -60,6 +63,28 @@ def bf16_round(x):
     u = np.array([x], np.float32).view(np.uint32).astype(np.uint64)
     r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
     return float(r.astype(np.uint32).view(np.float32)[0])
+
+
+def delay_steps(delay, dt):
+    """Delay in steps of the four projections (E2E, E2I, I2E, I2I) from a
+    delay in ms (one value or four; None for no delay): math.ceil(delay / dt)
+    as VarDelay.register_entry computes it (delay.py:263-266), at least 1 --
+    a projection without delay, or with delay 0, reads the pre's spikes of
+    the previous step, as delay 1 does."""
+    if delay is None:
+        return [1, 1, 1, 1]
+    vals = list(delay) if isinstance(delay, (list, tuple, np.ndarray)) else [delay] * 4
+    if len(vals) != 4:
+        raise ValueError("delay must be one value or four (E2E, E2I, I2E, I2I)")
+    out = []
+    for v in vals:
+        if v is None:
+            out.append(1)
+            continue
+        if v < 0:
+            raise ValueError(f"delay must be non-negative, got {v}")
+        out.append(max(int(math.ceil(float(v) / float(dt))), 1))
+    return out
 
 
 def combined_csr(csrs, n_exc, n_inh):
@@ -89,7 +114,8 @@ class BatchedCOBA:
     ``specs``: list of dicts with keys ``n_exc``, ``n_inh``, ``seed``,
     ``w_exc``, ``w_inh``, ``I_ext`` and either ``in_degree`` (just-in-time
     connectivity with p = in_degree / N for all four projections) or ``csr``
-    = {E2E, E2I, I2E, I2I: (indices, indptr)}.  All specs must use the same
+    = {E2E, E2I, I2E, I2I: (indices, indptr)}, and optionally ``delay`` (ms;
+    one value or four, see :func:`delay_steps`).  All specs must use the same
     kind.  ``bf16=True`` rounds the homogeneous weights to bfloat16.
     """
 
@@ -105,6 +131,7 @@ class BatchedCOBA:
         self._keep = []
         off = 0
         self.tiles = []
+        max_delay = 1
         for k, sp in enumerate(specs):
             d = self.descs[k]
             ne, ni = int(sp["n_exc"]), int(sp["n_inh"])
@@ -129,12 +156,16 @@ class BatchedCOBA:
                 d.indptr, d.indices = tp.data_ptr(), ti.data_ptr()
                 for q in range(4):
                     d.clen[q] = 1
+            for q, steps in enumerate(delay_steps(sp.get("delay"), dt)):
+                d.delay[q] = steps
+            max_delay = max(max_delay, *d.delay)
             L.bpx_net_prepare(ctypes.byref(d))
             for s in range(0, N, TILE):
                 self.tiles += [k, s]
             self.sizes.append(N)
             off += N
         self.total = off
+        self.n_buf = max_delay + 1             # count buffers: one per step of the longest delay, plus one
         self.d_descs = torch.frombuffer(bytearray(bytes(self.descs)), dtype=torch.uint8).to(self.dev)
         self.d_tiles = torch.tensor(self.tiles, dtype=torch.int32, device=self.dev)
         self.n_tiles = len(self.tiles) // 2
@@ -161,7 +192,7 @@ class BatchedCOBA:
             self.t_last = torch.empty(T, dtype=torch.float32, device=d)
             self.g_exc = torch.empty(T, dtype=torch.float32, device=d)
             self.g_inh = torch.empty(T, dtype=torch.float32, device=d)
-            self.counts = torch.empty(4 * T, dtype=torch.int32, device=d)
+            self.counts = torch.empty(self.n_buf * 2 * T, dtype=torch.int32, device=d)
             self.spikes = torch.empty(T, dtype=torch.uint8, device=d)
             self.events = torch.empty(len(self.specs), dtype=torch.int64, device=d)
             self.spike_count = torch.empty(len(self.specs), dtype=torch.int32, device=d)
@@ -188,7 +219,7 @@ class BatchedCOBA:
         dev = device_offset is not None
         rc = L.bpx_net_step(s, _lib.ptr(self.d_descs), _lib.ptr(self.d_tiles), self.n_tiles, self.kind,
                             self.total, _lib.ptr(self.V), _lib.ptr(self.t_last), _lib.ptr(self.g_exc),
-                            _lib.ptr(self.g_inh), _lib.ptr(self.counts), _lib.ptr(sp),
+                            _lib.ptr(self.g_inh), _lib.ptr(self.counts), self.n_buf, _lib.ptr(sp),
                             _lib.ptr(self.events) if count else None, _lib.ptr(self.spike_count) if count else None,
                             ctypes.byref(self.params), device_offset if dev else self.i,
                             _lib.ptr(self.d_step) if dev else None)

@@ -216,16 +216,20 @@ float bpx_expon_coef(float tau, float dt);
 /*
  * Batched E/I COBA network step.  Advances any number of independent
  * networks (LifRefLTC/LifRef neurons, Expon + COBA synapses, homogeneous
- * weights, delay None: the EICOBA_PostAlign model of
- * examples/dynamics_simulation/COBA.py:58-109) by one step in one launch.
+ * weights: the EICOBA_PostAlign model of
+ * examples/dynamics_simulation/COBA.py:58-109, with optional projection
+ * delays) by one step in one launch.
  * No single reference counterpart: it fuses, per step, the four event
  * projections (event csrmv, event/_csr_matvec.py:38-110, or jitconn
  * event_mv_prob_homo with outdim_parallel=False,
  * jitconn/_event_matvec_taichi.py:92-123) and the population updates
  * (dyn/neurons/lif.py:461-500) in the reference order.
  * State pools are indexed by off + local neuron id (E first, then I);
- * counts is [2][2][total] uint32, zero at start (and zero in a network's
- * range when it joins).  The step index (count buffer parity; network time
+ * counts is [n_buf][2][total] uint32, zero at start (and zero in a network's
+ * range when it joins), n_buf >= 2 and above every delay.  A projection with
+ * delay d steps reads at step i the spikes of step i - d, as the rotation
+ * update of VarDelay does (delay.py:305-349); delays 0 and 1 both read the
+ * previous step's spikes.  The step index (count buffer; network time
  * t = (step - t0_step) * dt) is `step`, or *d_step + step when d_step points
  * to a device int64 base -- the form for HIP graphs: capture K launches with
  * step = 0..K-1 followed by bpx_net_advance(d_step, K), and every replay
@@ -249,6 +253,7 @@ typedef struct {
   float w_exc, w_inh;        /* homogeneous weights */
   float I_ext;               /* constant input */
   int32_t t0_step;           /* step index at which the network started (t = (step - t0_step) * dt) */
+  int32_t delay[4];          /* delay in steps of E2E, E2I, I2E, I2I (0 or 1: the previous step's spikes) */
   const int32_t* indptr;     /* CSR networks: [N+1] */
   const int32_t* indices;    /* CSR networks: local post ids */
 } bpx_net_desc;
@@ -263,7 +268,7 @@ typedef struct {
 
 void bpx_net_prepare(bpx_net_desc* desc);
 int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, int n_tiles, int conn_kind,
-                 int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts,
+                 int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts, int n_buf,
                  uint8_t* spikes, unsigned long long* events, unsigned int* spike_count,
                  const bpx_net_params* params, int64_t step, const int64_t* d_step);
 int bpx_net_advance(void* stream, int64_t* d_step, int64_t delta);

@@ -693,8 +693,20 @@ int orc_lif_coba_step_f64(int64_t n, int ltc, int n_g,
 int64_t orc_coba_net_run(int32_t n_exc, int32_t n_inh, const int32_t* indptr, const int32_t* indices,
                          float w_exc, float w_inh, float I_ext, const float* V0, int64_t steps, int ltc,
                          float V_rest, float V_th, float V_reset, float tau, float tau_ref, float R,
-                         float tau_e, float tau_i, float E_e, float E_i, float dt, uint8_t* raster) {
+                         float tau_e, float tau_i, float E_e, float E_i, float dt, uint8_t* raster,
+                         const int32_t* delay4) {
+  /* delay4: steps of delay of the projections E2E, E2I, I2E, I2I (NULL: all
+   * 1).  The projection with delay d reads the spikes of step i - d at step
+   * i (delay.py:305-349, rotation update with the delay data written after
+   * the pre update); d = 0 reads the pre's current spikes, those of step
+   * i - 1, as d = 1 does. */
   const int32_t N = n_exc + n_inh;
+  int32_t D[4];
+  int32_t H = 2;
+  for (int k = 0; k < 4; ++k) {
+    D[k] = delay4 ? (delay4[k] < 1 ? 1 : delay4[k]) : 1;
+    if (D[k] + 1 > H) H = D[k] + 1;
+  }
   float* V = (float*)malloc(sizeof(float) * N);
   float* tl = (float*)malloc(sizeof(float) * N);
   float* ge = (float*)calloc(N, sizeof(float));
@@ -702,6 +714,7 @@ int64_t orc_coba_net_run(int32_t n_exc, int32_t n_inh, const int32_t* indptr, co
   float* oute = (float*)calloc(N, sizeof(float));
   float* outi = (float*)calloc(N, sizeof(float));
   uint8_t* sp = (uint8_t*)calloc(N, 1);
+  uint8_t* hist = (uint8_t*)calloc((size_t)H * N, 1);     /* spikes of step i at hist[i % H] */
   int64_t total = 0;
   const float tau_g[2] = {tau_e, tau_i}, E_g[2] = {E_e, E_i};
   const float c_g[2] = {orc_expon_coef(tau_e, dt), orc_expon_coef(tau_i, dt)};
@@ -710,19 +723,27 @@ int64_t orc_coba_net_run(int32_t n_exc, int32_t n_inh, const int32_t* indptr, co
     const float t = (float)i * dt;
     for (int32_t j = 0; j < N; ++j) { oute[j] = 0.f; outi[j] = 0.f; }
     for (int32_t p = 0; p < N; ++p) {
-      if (!sp[p]) continue;
-      float* o = p < n_exc ? oute : outi;
-      const float w = p < n_exc ? w_exc : w_inh;
-      for (int32_t q = indptr[p]; q < indptr[p + 1]; ++q) o[indices[q]] += w;
+      const int exc = p < n_exc;
+      const int ka = exc ? 0 : 2, kb = ka + 1;          /* projections to E and to I */
+      const int64_t sa = i - D[ka], sb = i - D[kb];
+      const int on_a = sa >= 0 && hist[(sa % H) * N + p], on_b = sb >= 0 && hist[(sb % H) * N + p];
+      if (!on_a && !on_b) continue;
+      float* o = exc ? oute : outi;
+      const float w = exc ? w_exc : w_inh;
+      for (int32_t q = indptr[p]; q < indptr[p + 1]; ++q) {
+        const int32_t post = indices[q];
+        if (post < n_exc ? on_a : on_b) o[post] += w;
+      }
     }
     for (int32_t j = 0; j < N; ++j) { ge[j] = ge[j] + oute[j]; gi[j] = gi[j] + outi[j]; }
     float* gp[2] = {ge, gi};
     orc_lif_coba_step(N, ltc, 2, V, tl, sp, gp, tau_g, E_g, c_g, I_ext, NULL, V_rest, V_th, V_reset, tau, tau_ref,
                       R, t, dt);
+    memcpy(hist + (i % H) * N, sp, N);
     for (int32_t j = 0; j < N; ++j) total += sp[j];
     if (raster) memcpy(raster + i * N, sp, N);
   }
-  free(V); free(tl); free(ge); free(gi); free(oute); free(outi); free(sp);
+  free(V); free(tl); free(ge); free(gi); free(oute); free(outi); free(sp); free(hist);
   return total;
 }
 

@@ -31,7 +31,7 @@ def _jit_specs(sizes, seed0=100):
     return specs
 
 
-def _oracle_jit(batch, k, steps):
+def _oracle_jit(batch, k, steps, delays=(1, 1, 1, 1)):
     sp = batch.specs[k]
     d = batch.descs[k]
     ne, ni = sp["n_exc"], sp["n_inh"]
@@ -49,17 +49,24 @@ def _oracle_jit(batch, k, steps):
     # (projection index, pre, post, conductance group, weight)
     projs = [(0, "E", "E", 0, d.w_exc), (1, "E", "I", 0, d.w_exc), (2, "I", "E", 1, d.w_inh),
              (3, "I", "I", 1, d.w_inh)]
+    hist = {"E": [], "I": []}     # spikes of steps 0, 1, ... per population
     for i in range(steps):
         t = np.float32(i) * np.float32(prm.dt)
         for q, pre, post, kind, w in projs:
-            out = orc.jitconn_mv(s[pre].view(bool), "homo", 1, False, orc.LAYOUT_LANE32, size[post], size[pre],
-                                 np.float32(w), 0.0, d.clen[q], d.seed[q])
+            # the projection with delay D reads the spikes of step i - D
+            src = i - max(int(delays[q]), 1)
+            if src < 0:
+                continue
+            out = orc.jitconn_mv(hist[pre][src].view(bool), "homo", 1, False, orc.LAYOUT_LANE32, size[post],
+                                 size[pre], np.float32(w), 0.0, d.clen[q], d.seed[q])
             g[post][kind] += out
         for p in ("E", "I"):
             orc.lif_coba_step(Vs[p], tl[p], s[p], g[p], [prm.tau_e, prm.tau_i], [prm.E_e, prm.E_i], c_g, d.I_ext,
                               PARAMS, t, prm.dt, ltc=bool(prm.ltc))
         raster[i, :ne] = s["E"]
         raster[i, ne:] = s["I"]
+        hist["E"].append(s["E"].copy())
+        hist["I"].append(s["I"].copy())
     return raster
 
 
@@ -168,7 +175,7 @@ def _twin_error(specs, bf16, steps, stored_weights=False):
     worst = 0.0
     g_err = np.zeros((len(specs), 2))
     for _ in range(steps):
-        base = (b.i & 1) * 2 * T
+        base = (b.i % b.n_buf) * 2 * T
         ce = b.counts[base:base + T].double()
         ci = b.counts[base + T:base + 2 * T].double()
         V = b.V.double()
@@ -209,3 +216,41 @@ def test_current_error_vs_f64_twin():
             w = float(np.float32(s[key]))
             delta = abs(bf16_round(s[key]) - w) / w
             assert abs(g_err[k, j] - delta) <= 1e-5, (k, key, g_err[k, j], delta)
+
+
+def test_jit_delays_vs_oracle_and_mix():
+    # per-projection delays (E2E, E2I, I2E, I2I) in steps, from ms as
+    # VarDelay computes them; networks with different delays in one batch
+    specs = _jit_specs([1200, 2500, 1800], seed0=31)
+    specs[0]["delay"] = [0.3, 0.7, 1.0, 0.0]       # 3, 7, 10, 1 steps
+    specs[1]["delay"] = 2.5                        # 25 steps everywhere
+    batch = BatchedCOBA(specs)
+    assert batch.n_buf == 26
+    raster = batch.run(90, raster=True).cpu().numpy()
+    for k, d in ((0, (3, 7, 10, 1)), (1, (25, 25, 25, 25)), (2, (1, 1, 1, 1))):
+        ref = _oracle_jit(batch, k, 90, delays=d)
+        assert ref.sum() > 0
+        np.testing.assert_array_equal(raster[:, batch.net_slice(k)], ref)
+    alone = BatchedCOBA([specs[0]], V_init=batch.V0[batch.net_slice(0)])
+    np.testing.assert_array_equal(alone.run(90, raster=True).cpu().numpy(), raster[:, batch.net_slice(0)])
+
+
+def test_csr_delays_vs_oracle_and_graph():
+    net = EICOBANet(seed=9, mode="fixed_order")
+    csrs = {name: (net.csr[name][1], net.csr[name][2]) for name in ("E2E", "E2I", "I2E", "I2I")}
+    spec = dict(n_exc=net.n_exc, n_inh=net.n_inh, seed=9, w_exc=0.6, w_inh=6.7, I_ext=20.0, csr=csrs,
+                delay=[0.5, 0.2, 1.2, 0.1])
+    b = BatchedCOBA([spec], V_init=net.V0)
+    raster = b.run(120, raster=True).cpu().numpy()
+    from brainpy_amd.dyn.batched import combined_csr
+    idx, ptr = combined_csr(csrs, net.n_exc, net.n_inh)
+    _, ref = orc.coba_net_run(net.n_exc, net.n_inh, ptr, idx, 0.6, 6.7, 20.0, net.V0, 120, PARAMS,
+                              delays=[5, 2, 12, 1])
+    assert ref.sum() > 0
+    np.testing.assert_array_equal(raster, ref)
+    g = BatchedCOBA([spec], V_init=net.V0)
+    g.capture(40)
+    for _ in range(3):
+        g.replay()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(g.V.cpu().numpy(), b.V.cpu().numpy())
