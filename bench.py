@@ -119,6 +119,91 @@ def bench_jitconn(L, _lib, torch, steps, warmup):
     return {"workload": "jitconn_event_mv_prob_1Mx1M_p1e-4_seed42_rate0.02", "results": res}
 
 
+def _time_calls(_lib, torch, call, kid, steps, warmup):
+    """(wall seconds per call, device seconds per launch of kernel `kid`)."""
+    for _ in range(warmup):
+        call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        call()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    _lib.timer_reset()
+    _lib.timer_enable(True)
+    for _ in range(steps):
+        call()
+    torch.cuda.synchronize()
+    _lib.timer_enable(False)
+    ms, cnt = _lib.timer_read(kid)
+    return wall, ms / 1e3 / max(cnt, 1)
+
+
+def bench_coo(_lib, torch, steps=50, warmup=5):
+    """COO products on FixedProb matrices (row-sorted, as a CSR expands):
+    coomv on 100,000 x 100,000, p = 0.01 (nse 1e8), f32 values, int32
+    indices, both orientations; seg_matmul of 100,000 x 100,000, p = 0.001
+    (nse 1e7) with a dense (100,000, 64) f32 operand.  Algorithmic HBM bytes:
+    12 per entry (row, col, value) plus one read of the dense operand and one
+    write (and zero-fill) of the output."""
+    from brainpy_amd.connect import fixed_prob_csr_device
+    import ctypes  # noqa: F401
+    dev = _lib.device()
+    L = _lib.lib()
+    sp = _lib.stream_ptr()
+    kid = _lib.KID["coo"]
+    res = []
+    n = 100_000
+    idx, ptr = fixed_prob_csr_device(n, n, 0.01, seed=2024, device=dev)
+    nse = idx.numel()
+    row = torch.repeat_interleave(torch.arange(n, device=dev, dtype=torch.int32), ptr[1:] - ptr[:-1])
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    w = torch.empty(nse, dtype=torch.float32, device=dev).normal_(0.5, 0.1, generator=g)
+    v = torch.rand(n, generator=g, device=dev)
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    for transpose in (0, 1):
+        def call():
+            rc = L.bpx_coomv(sp, _lib.ptr(w), _lib.BPX_F32, nse, _lib.ptr(row), _lib.ptr(idx), _lib.BPX_I32, nse,
+                             _lib.ptr(v), _lib.BPX_F32, n, n, transpose, _lib.ptr(out))
+            if rc != 0:
+                raise RuntimeError(f"bpx_coomv failed ({rc})")
+        wall, kern = _time_calls(_lib, torch, call, kid, steps, warmup)
+        alg = nse * 12 + n * 4 + n * 4
+        r = {"op": "coomv", "transpose": bool(transpose), "nse": nse, "us_per_call": wall * 1e6,
+             "kernel_us": kern * 1e6, "alg_bytes": alg, "gbs": alg / kern / 1e9,
+             "frac": alg / kern / 1e9 / HBM_PEAK_GBS, "entries_per_s": nse / wall}
+        res.append(r)
+        print(f"# coomv transpose={transpose} nse={nse:.3g} {r['us_per_call']:8.1f} us/call kernel "
+              f"{r['kernel_us']:8.1f} us {r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%)", file=sys.stderr, flush=True)
+    del row, w, idx, ptr
+    idx, ptr = fixed_prob_csr_device(n, n, 0.001, seed=2025, device=dev)
+    nse = idx.numel()
+    row = torch.repeat_interleave(torch.arange(n, device=dev, dtype=torch.int32), ptr[1:] - ptr[:-1])
+    w = torch.empty(nse, dtype=torch.float32, device=dev).normal_(0.0, 1.0, generator=g)
+    k = 64
+    D = torch.rand((n, k), generator=g, device=dev)
+    out2 = torch.empty((n, k), dtype=torch.float32, device=dev)
+
+    def call2():
+        rc = L.bpx_coo_matmat(sp, _lib.ptr(w), _lib.BPX_F32, nse, _lib.ptr(row), _lib.ptr(idx), _lib.BPX_I32, nse,
+                              _lib.ptr(D), _lib.BPX_F32, n, n, k, 1, _lib.ptr(out2))
+        if rc != 0:
+            raise RuntimeError(f"bpx_coo_matmat failed ({rc})")
+    wall, kern = _time_calls(_lib, torch, call2, kid, steps, warmup)
+    alg = nse * 12 + n * k * 4 * 3
+    gathered = nse * k * 4
+    r = {"op": "seg_matmul", "k": k, "nse": nse, "us_per_call": wall * 1e6, "kernel_us": kern * 1e6,
+         "alg_bytes": alg, "gbs": alg / kern / 1e9, "frac": alg / kern / 1e9 / HBM_PEAK_GBS,
+         "gathered_bytes": gathered, "gather_gbs": gathered / kern / 1e9,
+         "gflops": 2 * nse * k / kern / 1e9}
+    res.append(r)
+    print(f"# seg_matmul k={k} nse={nse:.3g} {r['us_per_call']:8.1f} us/call kernel {r['kernel_us']:8.1f} us "
+          f"{r['gbs']:7.0f} GB/s alg ({100 * r['frac']:.1f}%), gathers {r['gather_gbs']:7.0f} GB/s, "
+          f"{r['gflops']:.0f} GFLOP/s", file=sys.stderr, flush=True)
+    return {"workload": "coomv_100kx100k_p0.01_f32 / seg_matmul_100kx100k_p0.001_k64_f32", "results": res}
+
+
 def _percentile(x, q):
     return float(np.percentile(np.asarray(x, np.float64), q)) if len(x) else None
 
@@ -239,6 +324,9 @@ def bench_batched(_lib, torch, steps=2000, n_nets=256, plain_sample=8, plain_ste
     from brainpy_amd.dyn.batched import BatchedCOBA, PlainCOBA, bf16_round
     specs = _c4_specs(n_nets)
     b = BatchedCOBA(specs, bf16=True)
+    tune = b.autotune()            # neurons per block, timed on this batch (state restored)
+    print("# batched C4 autotune (us/step by tile): " + ", ".join(f"{k}: {v:.1f}" for k, v in tune.items())
+          + f" -> {b.tile}", file=sys.stderr, flush=True)
     b.capture(100)
     b.replay()
     torch.cuda.synchronize()
@@ -297,7 +385,8 @@ def bench_batched(_lib, torch, steps=2000, n_nets=256, plain_sample=8, plain_ste
          "plain_events_per_s_est": events / t_plain_est, "speedup_vs_plain": t_plain_est / t_b,
          "plain_sample_networks": int(plain_sample), "plain_sample_steps": plain_steps,
          "bf16_weight_rel_err_max": float(max(rel)), "bf16_weight_rel_err_mean": float(np.mean(rel)),
-         "current_rel_err_vs_f64": cur_err, "f32_weights_s": t_b32, "f32_weights_events_per_s": events32 / t_b32}
+         "current_rel_err_vs_f64": cur_err, "f32_weights_s": t_b32, "f32_weights_events_per_s": events32 / t_b32,
+         "tile": b.tile, "autotune_us_per_step": {str(k): v for k, v in tune.items()}}
     print(f"# batched C4 {t_b:.3f} s ({t_b/steps*1e6:.1f} us/step) {events/t_b:.3e} ev/s, plain est "
           f"{t_plain_est:.1f} s -> x{t_plain_est/t_b:.1f}; current err vs f64: bf16 max "
           f"{cur_err['bf16_weights']['max']:.2e} mean {cur_err['bf16_weights']['mean']:.2e} (vs stored bf16 "
@@ -363,7 +452,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the jitconn and network legs")
-    ap.add_argument("--only", choices=["jitconn", "batched", "coba", "served"], default=None,
+    ap.add_argument("--only", choices=["jitconn", "batched", "coba", "served", "coo"], default=None,
                     help="run only this leg (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run 3 plain device copies of this many MiB (HBM counter calibration)")
@@ -382,7 +471,8 @@ def main():
     if args.only:
         fn = {"jitconn": lambda: bench_jitconn(L, _lib, torch, max(10, args.steps // 10), 3),
               "batched": lambda: bench_batched(_lib, torch, plain_sample=2, plain_steps=20),
-              "coba": lambda: bench_coba(_lib, torch), "served": lambda: bench_served(_lib, torch)}[args.only]
+              "coba": lambda: bench_coba(_lib, torch), "served": lambda: bench_served(_lib, torch),
+              "coo": lambda: bench_coo(_lib, torch)}[args.only]
         print(json.dumps({"only": args.only, "result": fn()}), flush=True)
         return
     stream = torch.cuda.current_stream()
@@ -559,6 +649,7 @@ def main():
         extra["coba"] = bench_coba(_lib, torch)
         extra["batched"] = bench_batched(_lib, torch)
         extra["served"] = bench_served(_lib, torch)
+        extra["coo"] = bench_coo(_lib, torch)
 
     # CPU baseline: the oracle's serial scatter (the reference CPU loop
     # order, event/_csr_matvec.py:321-340), one thread, on a bounded number

@@ -63,6 +63,10 @@ SIGNATURES = {
                                 ctypes.c_double, ctypes.c_double, _vp], _i32),
     "bpx_dense_on_pre": ([_vp, _vp, _i32, _vp, _i32, _i64, _i64, _vp, ctypes.c_double, ctypes.c_double, _vp], _i32),
     "bpx_dense_on_post": ([_vp, _vp, _i32, _vp, _i32, _i64, _i64, _vp, ctypes.c_double, ctypes.c_double, _vp], _i32),
+    "bpx_coomv": ([_vp, _vp, _i32, _i64, _vp, _vp, _i32, _i64, _vp, _i32, _i64, _i64, _i32, _vp], _i32),
+    "bpx_coo_matmat": ([_vp, _vp, _i32, _i64, _vp, _vp, _i32, _i64, _vp, _i32, _i64, _i64, _i64, _i32, _vp], _i32),
+    "bpx_coo_sddmm": ([_vp, _vp, _vp, _i32, _i64, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _i32, _vp],
+                      _i32),
     "bpx_tiled_csr_layout": ([_i64, _i64, _i64, _i32, ctypes.POINTER(TiledCSRPlan), ctypes.POINTER(_sz)], _i32),
     "bpx_tiled_csr_build": ([_vp, _vp, _i32, _i64, _vp, _vp, _i32, ctypes.POINTER(TiledCSRPlan), _vp, _sz], _i32),
     "bpx_event_csrmv_tiled": ([_vp, ctypes.POINTER(TiledCSRPlan), _vp, _i32, _i32, _i32, _vp], _i32),
@@ -163,4 +167,4 @@ def timer_read(kernel_id):
 
 
 KID = dict(event_scatter=0, event_gather=1, csr_scatter=2, csr_gather=3, jit_outdim=4, jit_scatter=5, info=6,
-           lif=7, fixed_order=8, copy=9, net=10, tiled=11)
+           lif=7, fixed_order=8, copy=9, net=10, tiled=11, coo=12)

@@ -92,6 +92,7 @@ enum KernelId {
   KID_COPY = 9,
   KID_NET = 10,
   KID_TILED = 11,
+  KID_COO = 12,
   KID_COUNT = 16
 };
 

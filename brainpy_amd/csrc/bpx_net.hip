@@ -1,8 +1,9 @@
 // bpx_net.hip -- batched E/I COBA network step: many independent networks
 // advanced by one launch per step.
 //
-// Per launch (step i), every 256-thread block owns 256 neurons of one
-// network (a tile from the host-built tile table):
+// Per launch (step i), every block of T threads owns T neurons of one
+// network (a tile from the host-built tile table; T = 64 ... 1024, chosen
+// per batch by the host-side autotuner):
 //   1. update: for each neuron, g_exc += S(c_exc), g_inh += S(c_inh), where
 //      c_* are the numbers of pre-synaptic spikes of step i-1 delivered to it
 //      and S(c) = ((0 + w) + w) + ... (c terms) is exactly the homogeneous
@@ -27,8 +28,6 @@
 #include "bpx_rng.h"
 
 namespace bpx {
-
-constexpr int kNetThreads = 256;
 
 __device__ __forceinline__ float repeat_sum(uint32_t c, float w) {
   float s = 0.f;
@@ -56,9 +55,9 @@ struct NetArgs {
   const int64_t* d_step;       // device base step (graph replays) or null
 };
 
-template <int CONN>
-__global__ __launch_bounds__(kNetThreads) void k_net_step(NetArgs a) {
-  __shared__ int s_spk[kNetThreads];
+template <int CONN, int T>
+__global__ __launch_bounds__(T) void k_net_step(NetArgs a) {
+  __shared__ int s_spk[T];
   __shared__ int s_nspk;
   __shared__ unsigned long long s_events;
   const int net = a.tiles[2 * blockIdx.x];
@@ -107,11 +106,11 @@ __global__ __launch_bounds__(kNetThreads) void k_net_step(NetArgs a) {
   }
   __syncthreads();
   const int nspk = s_nspk;
-  // deliver: 8 groups of 32 lanes, one spiking neuron per group
+  // deliver: T/32 groups of 32 lanes, one spiking neuron per group
   const int grp = threadIdx.x >> 5;
   const uint32_t l = threadIdx.x & 31;
   unsigned long long ev = 0ull;
-  for (int e = grp; e < nspk; e += kNetThreads / 32) {
+  for (int e = grp; e < nspk; e += T / 32) {
     const int p = s_spk[e];
     const bool exc = p < d.n_exc;
     const int64_t kind = exc ? 0 : total;                // excitatory or inhibitory conductance
@@ -187,14 +186,16 @@ void bpx_net_prepare(bpx_net_desc* d) {
   }
 }
 
-int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, int n_tiles, int conn_kind,
-                 int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts, int n_buf,
+int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, int n_tiles, int tile_size,
+                 int conn_kind, int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts, int n_buf,
                  uint8_t* spikes, unsigned long long* events, unsigned int* spike_count,
                  const bpx_net_params* prm, int64_t step, const int64_t* d_step) {
   if (!nets || !tiles || !prm) return BPX_ERR_ARG;
   if (n_buf < 2 || (!d_step && step < 0)) return BPX_ERR_ARG;
   if (n_tiles < 0 || total < 0) return BPX_ERR_SHAPE;
   if (conn_kind != BPX_NET_JIT && conn_kind != BPX_NET_CSR) return BPX_ERR_ARG;
+  if (tile_size != 64 && tile_size != 128 && tile_size != 256 && tile_size != 512 && tile_size != 1024)
+    return BPX_ERR_ARG;
   if (n_tiles == 0) return BPX_OK;
   NetArgs a;
   a.nets = nets;
@@ -227,10 +228,21 @@ int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, i
   a.step = step;
   a.d_step = d_step;
   hipStream_t s = (hipStream_t)stream;
-  if (conn_kind == BPX_NET_JIT)
-    BPX_LAUNCH(KID_NET, k_net_step<BPX_NET_JIT>, dim3((unsigned)n_tiles), dim3(kNetThreads), 0, s, a);
-  else
-    BPX_LAUNCH(KID_NET, k_net_step<BPX_NET_CSR>, dim3((unsigned)n_tiles), dim3(kNetThreads), 0, s, a);
+#define NET_GO(T_)                                                                                     \
+  do {                                                                                                 \
+    if (conn_kind == BPX_NET_JIT)                                                                      \
+      BPX_LAUNCH(KID_NET, (k_net_step<BPX_NET_JIT, T_>), dim3((unsigned)n_tiles), dim3(T_), 0, s, a);  \
+    else                                                                                               \
+      BPX_LAUNCH(KID_NET, (k_net_step<BPX_NET_CSR, T_>), dim3((unsigned)n_tiles), dim3(T_), 0, s, a);  \
+  } while (0)
+  switch (tile_size) {
+    case 64: NET_GO(64); break;
+    case 128: NET_GO(128); break;
+    case 256: NET_GO(256); break;
+    case 512: NET_GO(512); break;
+    default: NET_GO(1024); break;
+  }
+#undef NET_GO
   BPX_CHECK_LAUNCH();
   return BPX_OK;
 }

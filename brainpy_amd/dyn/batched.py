@@ -24,7 +24,9 @@ from ..math.jitconn import conn_len
 from .coba import expon_coef
 
 NET_JIT, NET_CSR = 0, 1
-TILE = 256
+TILE = 256                                  # default neurons per block of the step launch
+TILE_CHOICES = (64, 128, 256, 512, 1024)
+_TUNED = {}                                 # (kind, networks, size class) -> tile chosen by autotune()
 
 
 class NetDesc(ctypes.Structure):
@@ -48,7 +50,7 @@ def _register():
         L.bpx_net_prepare.argtypes = [ctypes.POINTER(NetDesc)]
         L.bpx_net_prepare.restype = None
         L.bpx_net_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
-                                   ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.POINTER(NetParams), ctypes.c_int64, ctypes.c_void_p]
         L.bpx_net_step.restype = ctypes.c_int
@@ -117,10 +119,14 @@ class BatchedCOBA:
     = {E2E, E2I, I2E, I2I: (indices, indptr)}, and optionally ``delay`` (ms;
     one value or four, see :func:`delay_steps`).  All specs must use the same
     kind.  ``bf16=True`` rounds the homogeneous weights to bfloat16.
+    ``tile``: neurons per block of the step launch (64 ... 1024); by default
+    the value :meth:`autotune` found for a batch of the same kind, count and
+    size, else 256.
     """
 
     def __init__(self, specs, V_rest=-60.0, V_th=-50.0, V_reset=-60.0, tau=20.0, tau_ref=5.0, R=1.0, tau_e=5.0,
-                 tau_i=10.0, E_e=0.0, E_i=-80.0, dt=0.1, ltc=True, bf16=False, record=False, V_init=None):
+                 tau_i=10.0, E_e=0.0, E_i=-80.0, dt=0.1, ltc=True, bf16=False, record=False, V_init=None,
+                 tile=None):
         self.dev = _lib.device()
         L = _register()
         self.kind = NET_CSR if "csr" in specs[0] else NET_JIT
@@ -130,7 +136,6 @@ class BatchedCOBA:
         self.sizes = []
         self._keep = []
         off = 0
-        self.tiles = []
         max_delay = 1
         for k, sp in enumerate(specs):
             d = self.descs[k]
@@ -160,15 +165,12 @@ class BatchedCOBA:
                 d.delay[q] = steps
             max_delay = max(max_delay, *d.delay)
             L.bpx_net_prepare(ctypes.byref(d))
-            for s in range(0, N, TILE):
-                self.tiles += [k, s]
             self.sizes.append(N)
             off += N
         self.total = off
         self.n_buf = max_delay + 1             # count buffers: one per step of the longest delay, plus one
         self.d_descs = torch.frombuffer(bytearray(bytes(self.descs)), dtype=torch.uint8).to(self.dev)
-        self.d_tiles = torch.tensor(self.tiles, dtype=torch.int32, device=self.dev)
-        self.n_tiles = len(self.tiles) // 2
+        self.set_tile(tile if tile is not None else _TUNED.get(self._tune_key(), TILE))
         self.params = NetParams(V_rest=V_rest, V_th=V_th, V_reset=V_reset, tau=tau, tau_ref=tau_ref, R=R,
                                 tau_e=tau_e, tau_i=tau_i, E_e=E_e, E_i=E_i, c_e=float(expon_coef(tau_e, dt)),
                                 c_i=float(expon_coef(tau_i, dt)), dt=dt, ltc=int(ltc))
@@ -182,6 +184,62 @@ class BatchedCOBA:
         self.V0 = np.asarray(V_init, np.float32)
         self.record = record
         self.reset()
+
+    def _tune_key(self):
+        # batches of one connectivity kind, network count and total size
+        # (to a factor of two) share a tuned tile size
+        return (self.kind, len(self.specs), int(self.total).bit_length())
+
+    def set_tile(self, tile):
+        """Neurons per block (= threads per block and LDS spike-list length)
+        of the step launch.  Invalidates a captured graph."""
+        if tile not in TILE_CHOICES:
+            raise ValueError(f"tile must be one of {TILE_CHOICES}, got {tile}")
+        tiles = []
+        for k, N in enumerate(self.sizes):
+            for s in range(0, N, tile):
+                tiles += [k, s]
+        self.tile = tile
+        self.tiles = tiles
+        self.d_tiles = torch.tensor(tiles, dtype=torch.int32, device=self.dev)
+        self.n_tiles = len(tiles) // 2
+        self.graph = None
+
+    def autotune(self, candidates=TILE_CHOICES, steps=20, repeats=2):
+        """Pick the tile size (neurons per block) with the shortest step time
+        on this batch: each candidate runs `steps` steps `repeats` times,
+        timed with HIP events on the launch stream, from a snapshot of the
+        state that is restored afterwards (the batch continues exactly where
+        it was).  The choice is kept for later batches of the same kind,
+        count and size.  Returns {tile: microseconds per step}."""
+        state = [self.V, self.t_last, self.g_exc, self.g_inh, self.counts, self.spikes, self.events,
+                 self.spike_count, self.d_step]
+        saved = [t.clone() for t in state]
+        i0 = self.i
+        times = {}
+        for tile in candidates:
+            self.set_tile(tile)
+            best = float("inf")
+            for _ in range(repeats):
+                for t, s in zip(state, saved):
+                    t.copy_(s)
+                self.i = i0
+                start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                start.record()
+                for _ in range(steps):
+                    self.step()
+                stop.record()
+                stop.synchronize()
+                best = min(best, start.elapsed_time(stop) * 1e3 / steps)
+            times[tile] = best
+        for t, s in zip(state, saved):
+            t.copy_(s)
+        self.i = i0
+        choice = min(times, key=times.get)
+        self.set_tile(choice)
+        _TUNED[self._tune_key()] = choice
+        self.tune_times = times
+        return times
 
     def reset(self):
         """Initial state (in place, so a captured graph stays valid)."""
@@ -217,7 +275,7 @@ class BatchedCOBA:
         sp = spikes_out if spikes_out is not None else (self.spikes if self.record else None)
         s = _lib.stream_ptr()
         dev = device_offset is not None
-        rc = L.bpx_net_step(s, _lib.ptr(self.d_descs), _lib.ptr(self.d_tiles), self.n_tiles, self.kind,
+        rc = L.bpx_net_step(s, _lib.ptr(self.d_descs), _lib.ptr(self.d_tiles), self.n_tiles, self.tile, self.kind,
                             self.total, _lib.ptr(self.V), _lib.ptr(self.t_last), _lib.ptr(self.g_exc),
                             _lib.ptr(self.g_inh), _lib.ptr(self.counts), self.n_buf, _lib.ptr(sp),
                             _lib.ptr(self.events) if count else None, _lib.ptr(self.spike_count) if count else None,

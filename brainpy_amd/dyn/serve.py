@@ -218,7 +218,7 @@ class SimServer:
             self._rebuild()
         if self.n_tiles:
             rc = self.L.bpx_net_step(_lib.stream_ptr(), _lib.ptr(self.d_descs), _lib.ptr(self.d_tiles), self.n_tiles,
-                                     0, self.capacity, _lib.ptr(self.V), _lib.ptr(self.t_last), _lib.ptr(self.g_exc),
+                                     TILE, 0, self.capacity, _lib.ptr(self.V), _lib.ptr(self.t_last), _lib.ptr(self.g_exc),
                                      _lib.ptr(self.g_inh), _lib.ptr(self.counts), 2, None, _lib.ptr(self.ev_scratch),
                                      _lib.ptr(self.sc_scratch), ctypes.byref(self.params), self.step_idx, None)
             _lib.check(rc, "bpx_net_step")

@@ -1,9 +1,11 @@
 """Sparse CSR matvec with a dense vector (``bm.sparse.csrmv``,
-``bm.sparse.csrmv_taichi``) and the CSR/COO/dense converters on the gfx950
+``bm.sparse.csrmv_taichi``), COO products (``bm.sparse.coomv``,
+``bm.sparse.seg_matmul``) and the CSR/COO/dense converters on the gfx950
 library.
 
 References: brainpy/_src/math/sparse/_csr_mv.py:29-123,
-sparse/_csr_mv_taichi.py:188-264, sparse/_utils.py:20-120.
+sparse/_csr_mv_taichi.py:188-264, sparse/_coo_mv.py:17-196,
+sparse/_jax_prim.py:15-166, sparse/_utils.py:20-120.
 """
 from typing import Tuple
 
@@ -13,7 +15,7 @@ import torch
 from .. import _lib
 from ._common import acc_dtype, batch_rows, data_tensor, index_tensor, ndim, needs_ad
 
-__all__ = ["csrmv", "csrmv_taichi", "csr_to_dense", "csr_to_coo", "coo_to_csr"]
+__all__ = ["csrmv", "csrmv_taichi", "coomv", "seg_matmul", "csr_to_dense", "csr_to_coo", "coo_to_csr"]
 
 _METHODS = ("cusparse", "scalar", "vector", "adaptive")
 
@@ -184,3 +186,229 @@ def csr_to_dense(data, indices, indptr, *, shape: Tuple[int, int]):
     vals = d.expand(cols.shape[0]) if d.shape[0] == 1 else d
     mat.index_put_((rows.long(), cols.long()), vals, accumulate=True)
     return mat
+
+
+# ---------------------------------------------------------------------------
+# COO products
+# ---------------------------------------------------------------------------
+_COO_DTYPES = (torch.float32, torch.float64, torch.int32)
+
+
+def _coomv_launch(d, row, col, v, shape, transpose):
+    d, row, col, v = d.contiguous(), row.contiguous(), col.contiguous(), v.contiguous()
+    m, n = int(shape[0]), int(shape[1])
+    out = torch.empty(n if transpose else m, dtype=d.dtype, device=v.device)
+    rc = _lib.lib().bpx_coomv(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), d.shape[0], _lib.ptr(row),
+                              _lib.ptr(col), _lib.bpx_dtype(row), row.shape[0], _lib.ptr(v), _lib.bpx_dtype(v), m, n,
+                              int(bool(transpose)), _lib.ptr(out))
+    _lib.check(rc, "bpx_coomv")
+    return out
+
+
+def _coo_sddmm(row, col, a, a_keys, a_stride, a_col_stride, b, b_keys, b_stride, b_col_stride, ncols):
+    """out[e] = sum_c a[row[e]*a_stride + c*a_col_stride] * b[col[e]*b_stride + c*b_col_stride]."""
+    row, col, a, b = row.contiguous(), col.contiguous(), a.contiguous(), b.contiguous()
+    out = torch.empty(row.shape[0], dtype=a.dtype, device=a.device)
+    rc = _lib.lib().bpx_coo_sddmm(_lib.stream_ptr(), _lib.ptr(row), _lib.ptr(col), _lib.bpx_dtype(row),
+                                  row.shape[0], _lib.ptr(a), a_keys, a_stride, a_col_stride, _lib.ptr(b), b_keys,
+                                  b_stride, b_col_stride, ncols, _lib.bpx_dtype(a), _lib.ptr(out))
+    _lib.check(rc, "bpx_coo_sddmm")
+    return out
+
+
+class _COOMV(torch.autograd.Function):
+    """Differentiation rules of sparse/_coo_mv.py:155-186: tangents pass
+    through the same operator; the vector's cotangent is coomv(data, ct) with
+    `transpose` flipped; the values' cotangent is ct[row] * v[col], and for
+    ``transpose=True`` ct[col] * v[row] (the derivative of the transposed
+    product; the reference's rule returns ct[row] * v[col] for both, which
+    only matches the forward map when transpose is False).  A scalar value's
+    cotangent is the sum of those products."""
+
+    @staticmethod
+    def forward(d, v, row, col, shape, transpose):
+        return _coomv_launch(d, row, col, v, shape, transpose)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        d, v, row, col, shape, transpose = inputs
+        ctx.save_for_backward(d, v, row, col)
+        ctx.shape, ctx.transpose = shape, transpose
+
+    @staticmethod
+    def backward(ctx, ct):
+        d, v, row, col = ctx.saved_tensors
+        gd = gv = None
+        m, n = ctx.shape
+        ct = ct.to(d.dtype)
+        if ctx.needs_input_grad[0]:
+            okey, gkey = (col, row) if ctx.transpose else (row, col)
+            out_len, in_len = (n, m) if ctx.transpose else (m, n)
+            per = _coo_sddmm(okey, gkey, ct, out_len, 1, 0, v.to(d.dtype), in_len, 1, 0, 1)
+            gd = per.sum().reshape(1) if d.shape[0] == 1 else per
+        if ctx.needs_input_grad[1]:
+            gv = _COOMV.apply(d, ct, row, col, ctx.shape, not ctx.transpose).to(v.dtype)
+        return gd, gv, None, None, None, None
+
+    @staticmethod
+    def vmap(info, in_dims, d, v, row, col, shape, transpose):
+        return batch_rows(info, in_dims, (d, v, row, col, shape, transpose), _COOMV.apply)
+
+
+def _coo_prepare(data, row, col):
+    d = _lib.to_device(_as_array(data))
+    if d.ndim == 0:
+        d = d.reshape(1)
+    return d, index_tensor(_as_array(row)), index_tensor(_as_array(col))
+
+
+def coomv(data, row, col, vector, *, shape: Tuple[int, int], rows_sorted: bool = False, cols_sorted: bool = False,
+          transpose: bool = False, method: str = 'cusparse') -> torch.Tensor:
+    """Product of a COO sparse matrix and a dense vector.
+
+    ``out = A @ vector`` (``A.T @ vector`` with ``transpose``), where A has
+    the ``nse`` entries ``(row[k], col[k], data[k])`` in any order; ``data``
+    of one element is broadcast.  ``rows_sorted`` / ``cols_sorted`` only
+    choose a cuSPARSE call in the reference and do not change the result:
+    the gfx950 kernel sums runs of equal output rows inside each 256-element
+    tile (a row-sorted matrix costs one atomic per row and tile).  Data and
+    vector share one dtype (float32, float64 or int32); indices int32 or
+    int64 (unsigned become int64).  Reference: sparse/_coo_mv.py:17-89 and
+    :99-105 (semantics), :108-117 (checks).
+    """
+    if method != 'cusparse':
+        raise ValueError
+    d, r, c = _coo_prepare(data, row, col)
+    v = _lib.to_device(_as_array(vector))
+    m, n = int(shape[0]), int(shape[1])
+    if r.shape != c.shape or (d.shape[0] != 1 and d.shape != r.shape):
+        raise AssertionError(f'data, row and col must have one shape (data may have one element), got '
+                             f'{tuple(d.shape)}, {tuple(r.shape)}, {tuple(c.shape)}.')
+    if r.dtype != c.dtype:
+        raise AssertionError(f'row and col must have one dtype, got {r.dtype} and {c.dtype}.')
+    if d.dtype != v.dtype:
+        raise AssertionError(f'data and vector must have one dtype, got {d.dtype} and {v.dtype}.')
+    if d.dtype not in _COO_DTYPES:
+        raise TypeError(f'coomv supports float32, float64 and int32 values, got {d.dtype}.')
+    if v.ndim != 1 or v.shape[0] != (m if transpose else n):
+        raise AssertionError(f'vector of shape {tuple(v.shape)} does not fit matrix {tuple(shape)} '
+                             f'(transpose={transpose}).')
+    if needs_ad(d, v):
+        return _COOMV.apply(d, v, r, c, (m, n), bool(transpose))
+    return _coomv_launch(d, r, c, v, (m, n), transpose)
+
+
+def _matmat_launch(d, row, col, dense, shape, left):
+    d, row, col, dense = d.contiguous(), row.contiguous(), col.contiguous(), dense.contiguous()
+    m, n = int(shape[0]), int(shape[1])
+    k = int(dense.shape[1] if left else dense.shape[0])
+    out = torch.empty((m, k) if left else (k, n), dtype=d.dtype, device=dense.device)
+    rc = _lib.lib().bpx_coo_matmat(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), d.shape[0], _lib.ptr(row),
+                                   _lib.ptr(col), _lib.bpx_dtype(row), row.shape[0], _lib.ptr(dense),
+                                   _lib.bpx_dtype(dense), m, n, k, int(bool(left)), _lib.ptr(out))
+    _lib.check(rc, "bpx_coo_matmat")
+    return out
+
+
+class _SegMatmul(torch.autograd.Function):
+    """Gradients of the JAX expression of sparse/_jax_prim.py:15-94 (a
+    gather, a product, a segment sum) with a two-dimensional dense operand:
+    left, Y = S D: dD = S^T dY, dvalues[e] = <dY[row[e], :], D[col[e], :]>;
+    right, Y = D S: dD = dY S^T, dvalues[e] = <dY[:, col[e]], D[:, row[e]]>.
+    A scalar value's cotangent is the sum over entries."""
+
+    @staticmethod
+    def forward(d, dense, row, col, shape, left):
+        return _matmat_launch(d, row, col, dense, shape, left)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        d, dense, row, col, shape, left = inputs
+        ctx.save_for_backward(d, dense, row, col)
+        ctx.shape, ctx.left = shape, left
+
+    @staticmethod
+    def backward(ctx, ct):
+        d, dense, row, col = ctx.saved_tensors
+        m, n = ctx.shape
+        ct = ct.to(d.dtype).contiguous()
+        D = dense.to(d.dtype)
+        gd = gD = None
+        if ctx.left:
+            k = D.shape[1]
+            if ctx.needs_input_grad[0]:
+                per = _coo_sddmm(row, col, ct, m, k, 1, D, n, k, 1, k)
+                gd = per.sum().reshape(1) if d.shape[0] == 1 else per
+            if ctx.needs_input_grad[1]:
+                gD = _SegMatmul.apply(d, ct, col, row, (n, m), True)
+        else:
+            k = D.shape[0]
+            if ctx.needs_input_grad[0]:
+                per = _coo_sddmm(col, row, ct, n, 1, n, D, m, 1, m, k)
+                gd = per.sum().reshape(1) if d.shape[0] == 1 else per
+            if ctx.needs_input_grad[1]:
+                gD = _SegMatmul.apply(d, ct, col, row, (n, m), False)
+        if gD is not None:
+            gD = gD.to(dense.dtype)
+        return gd, gD, None, None, None, None
+
+    @staticmethod
+    def vmap(info, in_dims, d, dense, row, col, shape, left):
+        return batch_rows(info, in_dims, (d, dense, row, col, shape, left), _SegMatmul.apply)
+
+
+def _is_dense(x):
+    return isinstance(x, (torch.Tensor, np.ndarray))
+
+
+def _seg_matmul(sparse, dense, left):
+    if dense.ndim not in (1, 2):
+        raise AssertionError('Dense matrix must be a one- or two-dimensional matrix.')
+    shape = sparse['shape']
+    if len(shape) != 2:
+        raise ValueError(f'Sparse matrix must be a two-dimensional matrix. But we got {shape}')
+    rows, cols = sparse['index']
+    d, r, c = _coo_prepare(sparse['data'], rows, cols)
+    if r.dtype != c.dtype:
+        r, c = r.to(torch.int64), c.to(torch.int64)
+    D = _lib.to_device(_as_array(dense))
+    dt = torch.promote_types(d.dtype, D.dtype)
+    if dt not in _COO_DTYPES:
+        raise TypeError(f'seg_matmul supports float32, float64 and int32 operands, got {d.dtype} and {D.dtype}.')
+    d, D = d.to(dt), D.to(dt)
+    m, n = int(shape[0]), int(shape[1])
+    if d.shape[0] != 1 and d.shape[0] != r.shape[0]:
+        raise ValueError(f'values of shape {tuple(d.shape)} do not match {r.shape[0]} indices.')
+    inner = D.shape[0] if (left or D.ndim == 1) else D.shape[1]
+    if inner != (n if left else m):
+        raise ValueError(f'dense operand of shape {tuple(D.shape)} does not fit sparse matrix {tuple(shape)}.')
+    if D.ndim == 1:
+        return coomv(d, r, c, D, shape=(m, n), transpose=not left)
+    if needs_ad(d, D):
+        return _SegMatmul.apply(d, D, r, c, (m, n), left)
+    return _matmat_launch(d, r, c, D, (m, n), left)
+
+
+def seg_matmul(A, B):
+    """Sparse-dense matrix product ``A @ B`` where exactly one of A, B is a
+    COO sparse matrix given as ``{'data': values, 'index': (rows, cols),
+    'shape': (N, M)}``.
+
+    Left sparse: ``(N, M) @ (M,)`` or ``(M, K)``; right sparse: ``(N,)`` or
+    ``(K, N) @ (N, M)``.  Values and dense operand are promoted to one dtype
+    (float32, float64 or int32).  One-dimensional dense operands run the
+    coomv kernel; two-dimensional ones the COO matrix-matrix kernels.
+    Reference: sparse/_jax_prim.py:97-166 (dispatch and errors), :15-94
+    (semantics).
+    """
+    if isinstance(A, dict):
+        if not _is_dense(B):
+            raise ValueError('A and B cannot be both sparse. \n'
+                             f'A:\n{A}\n'
+                             f'B:\n{B}')
+        return _seg_matmul(A, B, True)
+    if not isinstance(B, dict):
+        raise ValueError('A and B cannot be both dense. \n'
+                         f'A:\n{A}\n'
+                         f'B:\n{B}')
+    return _seg_matmul(B, A, False)

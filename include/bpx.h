@@ -235,7 +235,9 @@ float bpx_expon_coef(float tau, float dt);
  * step = 0..K-1 followed by bpx_net_advance(d_step, K), and every replay
  * continues in time.
  * tiles: device array of (network, first local neuron) int32 pairs, one per
- * 256-neuron block.  nets: device array of bpx_net_desc prepared with
+ * block of tile_size neurons (64, 128, 256, 512 or 1024: the block size of
+ * the launch and of its LDS spike list; the host picks it per batch by
+ * timing the candidates).  nets: device array of bpx_net_desc prepared with
  * bpx_net_prepare (host).  jitconn projections use seed[k], clen[k] for
  * k = E2E, E2I, I2E, I2I with matrix shape (post, pre); CSR networks give one
  * row per neuron listing local post ids (E targets of E neurons feed the
@@ -267,8 +269,8 @@ typedef struct {
 } bpx_net_params;
 
 void bpx_net_prepare(bpx_net_desc* desc);
-int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, int n_tiles, int conn_kind,
-                 int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts, int n_buf,
+int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, int n_tiles, int tile_size,
+                 int conn_kind, int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts, int n_buf,
                  uint8_t* spikes, unsigned long long* events, unsigned int* spike_count,
                  const bpx_net_params* params, int64_t step, const int64_t* d_step);
 int bpx_net_advance(void* stream, int64_t* d_step, int64_t delta);
@@ -305,6 +307,39 @@ int bpx_dense_on_pre(void* stream, const void* w, int w_dtype, const void* spike
 int bpx_dense_on_post(void* stream, const void* w, int w_dtype, const void* spike, int spike_dtype, int64_t num_pre,
                       int64_t num_post, const void* trace, double w_min, double w_max, void* out);
 
+/* -------------------------------------------------------------------------
+ * COO sparse products.
+ *
+ * bpx_coomv replaces `coomv` (sparse/_coo_mv.py:17-89; cuSPARSE lowering
+ * :122-152, default lowering _coomv_impl :99-105): out (m, or n when
+ * transposed) = A @ vector (A^T @ vector), A given by nse (row, col, data)
+ * triples in any order (the sorted flags of the reference only choose a
+ * cuSPARSE call; the result is the same).  data_len 1 broadcasts one value.
+ * Values and vector share one dtype: f32, f64 or i32 (wrapping).  Index
+ * dtype i32 or i64.  Output keys outside the shape are dropped and gather
+ * indices clamped, as JAX's scatter/gather do.
+ *
+ * bpx_coo_matmat replaces `seg_matmul` (sparse/_jax_prim.py:97-166) with a
+ * two-dimensional dense operand: sparse_left != 0: out (m, k) = S (m, n) @
+ * dense (n, k); sparse_left == 0: out (k, n) = dense (k, m) @ S (m, n).
+ * Row-major dense and out.
+ *
+ * bpx_coo_sddmm: out[e] = sum_c a[row[e]*a_stride + c*a_col_stride] *
+ * b[col[e]*b_stride + c*b_col_stride] for c < ncols -- the value cotangent
+ * of both products (sparse/_coo_mv.py:174-186 for coomv; the JAX gradient
+ * of sparse/_jax_prim.py:55-94 for seg_matmul).  Pairs with a row or col
+ * outside [0, a_keys) / [0, b_keys) give 0.
+ * ------------------------------------------------------------------------- */
+int bpx_coomv(void* stream, const void* data, int data_dtype, int64_t data_len, const void* row, const void* col,
+              int index_dtype, int64_t nse, const void* vector, int vec_dtype, int64_t m, int64_t n, int transpose,
+              void* out);
+int bpx_coo_matmat(void* stream, const void* data, int data_dtype, int64_t data_len, const void* row,
+                   const void* col, int index_dtype, int64_t nse, const void* dense, int dense_dtype, int64_t m,
+                   int64_t n, int64_t k, int sparse_left, void* out);
+int bpx_coo_sddmm(void* stream, const void* row, const void* col, int index_dtype, int64_t nse, const void* a,
+                  int64_t a_keys, int64_t a_stride, int64_t a_col_stride, const void* b, int64_t b_keys,
+                  int64_t b_stride, int64_t b_col_stride, int64_t ncols, int dtype, void* out);
+
 /* Plain device copy (bandwidth calibration of the HBM counters). */
 int bpx_copy(void* stream, const void* src, void* dst, int64_t bytes);
 
@@ -314,7 +349,8 @@ int bpx_copy(void* stream, const void* src, void* dst, int64_t bytes);
  * synchronises those events and returns the summed device milliseconds and
  * launch count for one kernel id (0 event scatter, 1 event gather,
  * 2 csr scatter, 3 csr gather, 4 jitconn outdim, 5 jitconn scatter,
- * 6 event info, 7 lif step, 8 fixed order, 9 copy).
+ * 6 event info, 7 lif step, 8 fixed order, 9 copy, 10 network step,
+ * 11 tiled event scatter, 12 COO products).
  */
 int bpx_timer_enable(int on);
 int bpx_timer_read(int kernel_id, double* total_ms, int64_t* launches);

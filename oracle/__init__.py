@@ -10,6 +10,8 @@ path (``brainpy_amd``) never imports it and has no CPU fallback.
 * ``oracle.np_ref`` -- an independent, vectorised numpy restatement of LFSR88
   and of both just-in-time connectivity layouts, used to cross-check the C
   restatement on small shapes.
+* ``oracle.coo_ref`` -- numpy restatement of the COO products (coomv,
+  seg_matmul).
 
 Parity unpinned: the normal-distribution weights of jitconn (log/sqrt/sin
 are device/libm math, pinned to a tolerance only) and the brainpylib

@@ -99,6 +99,36 @@ def test_alone_equals_in_mix_and_graph():
     assert int(g.events.sum()) == int(mix.events.sum()) > 0
 
 
+@pytest.mark.parametrize("kind", ["jit", "csr"])
+def test_tile_sizes_same_raster_and_autotune_restores(kind):
+    # every tile size (neurons per block) gives the same raster: counts are
+    # integers, so the block decomposition cannot change a result
+    if kind == "jit":
+        specs = _jit_specs([700, 2600, 1500], seed0=31)
+    else:
+        net = EICOBANet(seed=5, mode="fixed_order")
+        csrs = {name: (net.csr[name][1], net.csr[name][2]) for name in ("E2E", "E2I", "I2E", "I2I")}
+        specs = [dict(n_exc=net.n_exc, n_inh=net.n_inh, seed=5, w_exc=0.6, w_inh=6.7, I_ext=20.0, csr=csrs)]
+    ref = BatchedCOBA(specs, tile=256)
+    r_ref = ref.run(50, raster=True).cpu().numpy()
+    assert r_ref.sum() > 0
+    for tile in (64, 128, 512, 1024):
+        b = BatchedCOBA(specs, tile=tile, V_init=ref.V0)
+        np.testing.assert_array_equal(b.run(50, raster=True).cpu().numpy(), r_ref)
+    # autotune runs candidates from a snapshot and restores it: 20 steps,
+    # tune, 30 more steps == 50 steps
+    b = BatchedCOBA(specs, V_init=ref.V0)
+    r1 = b.run(20, raster=True).cpu().numpy()
+    times = b.autotune(steps=5, repeats=1)
+    assert set(times) == {64, 128, 256, 512, 1024} and b.tile == min(times, key=times.get)
+    r2 = b.run(30, raster=True).cpu().numpy()
+    np.testing.assert_array_equal(np.concatenate([r1, r2]), r_ref)
+    np.testing.assert_array_equal(b.V.cpu().numpy(), ref.V.cpu().numpy())
+    assert int(b.events.sum()) == int(ref.events.sum())
+    # a new batch of the same kind, count and size takes the tuned tile
+    assert BatchedCOBA(specs, V_init=ref.V0).tile == b.tile
+
+
 def test_bf16_weights_rounded():
     specs = _jit_specs([2000], seed0=3)
     b = BatchedCOBA(specs, bf16=True)
