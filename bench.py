@@ -324,7 +324,7 @@ def bench_batched(_lib, torch, steps=2000, n_nets=256, plain_sample=8, plain_ste
     from brainpy_amd.dyn.batched import BatchedCOBA, PlainCOBA, bf16_round
     specs = _c4_specs(n_nets)
     b = BatchedCOBA(specs, bf16=True)
-    tune = b.autotune()            # neurons per block, timed on this batch (state restored)
+    tune = b.autotune(steps=300)   # neurons per block, timed on this batch's first 300 steps (state restored)
     print("# batched C4 autotune (us/step by tile): " + ", ".join(f"{k}: {v:.1f}" for k, v in tune.items())
           + f" -> {b.tile}", file=sys.stderr, flush=True)
     b.capture(100)

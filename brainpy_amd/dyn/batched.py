@@ -205,13 +205,18 @@ class BatchedCOBA:
         self.n_tiles = len(tiles) // 2
         self.graph = None
 
-    def autotune(self, candidates=TILE_CHOICES, steps=20, repeats=2):
-        """Pick the tile size (neurons per block) with the shortest step time
-        on this batch: each candidate runs `steps` steps `repeats` times,
-        timed with HIP events on the launch stream, from a snapshot of the
-        state that is restored afterwards (the batch continues exactly where
-        it was).  The choice is kept for later batches of the same kind,
-        count and size.  Returns {tile: microseconds per step}."""
+    def autotune(self, candidates=TILE_CHOICES, steps=200, repeats=2):
+        """Pick the tile size (neurons per block) with the shortest step
+        kernel on this batch: each candidate runs `steps` steps `repeats`
+        times from a snapshot of the state that is restored afterwards (the
+        batch continues exactly where it was), and the step kernel's device
+        time is read from the library's kernel timer (HIP events around each
+        launch on its stream; host launch overhead is not counted, as under
+        graph replay).  The window should span the run's regimes: from a
+        fresh state the first steps carry the synchronised initial spikes
+        (delivery-heavy, favouring larger tiles) and later steps the steady
+        state.  The choice is kept for later batches of the same kind, count
+        and size.  Returns {tile: kernel microseconds per step}."""
         state = [self.V, self.t_last, self.g_exc, self.g_inh, self.counts, self.spikes, self.events,
                  self.spike_count, self.d_step]
         saved = [t.clone() for t in state]
@@ -224,13 +229,17 @@ class BatchedCOBA:
                 for t, s in zip(state, saved):
                     t.copy_(s)
                 self.i = i0
-                start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                start.record()
-                for _ in range(steps):
-                    self.step()
-                stop.record()
-                stop.synchronize()
-                best = min(best, start.elapsed_time(stop) * 1e3 / steps)
+                torch.cuda.synchronize()
+                _lib.timer_reset()
+                _lib.timer_enable(True)
+                try:
+                    for _ in range(steps):
+                        self.step()
+                    torch.cuda.synchronize()
+                finally:
+                    _lib.timer_enable(False)
+                ms, launches = _lib.timer_read(_lib.KID["net"])
+                best = min(best, ms * 1e3 / max(launches, 1))
             times[tile] = best
         for t, s in zip(state, saved):
             t.copy_(s)
