@@ -176,7 +176,30 @@ def bench_coo(_lib, torch, steps=50, warmup=5):
         res.append(r)
         print(f"# coomv transpose={transpose} nse={nse:.3g} {r['us_per_call']:8.1f} us/call kernel "
               f"{r['kernel_us']:8.1f} us {r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%)", file=sys.stderr, flush=True)
-    del row, w, idx, ptr
+    # ceilings of the access pattern on the same arrays (no scan, no atomics)
+    probe_out = torch.empty(32768 * 256, dtype=torch.float32, device=dev)
+    ceil = {}
+    for mode, name in ((0, "read_row_col_value"), (1, "read_col_gather"), (2, "read_all_gather")):
+        def pcall():
+            rc = L.bpx_coo_probe(sp, _lib.ptr(row), _lib.ptr(idx), _lib.ptr(w), _lib.ptr(v), n, nse, mode,
+                                 _lib.ptr(probe_out), probe_out.numel())
+            if rc != 0:
+                raise RuntimeError(f"bpx_coo_probe failed ({rc})")
+        for _ in range(3):
+            pcall()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(10):
+            pcall()
+        ev1.record()
+        ev1.synchronize()
+        ceil[name] = ev0.elapsed_time(ev1) * 1e3 / 10
+    res[0]["gather_ceiling_us"] = ceil
+    res[0]["frac_of_read_gather_ceiling"] = ceil["read_all_gather"] / res[0]["kernel_us"]
+    print("# coomv ceilings (probe): " + ", ".join(f"{k} {u:.1f} us" for k, u in ceil.items())
+          + f"; coomv at {100 * res[0]['frac_of_read_gather_ceiling']:.0f}% of read+gather",
+          file=sys.stderr, flush=True)
+    del row, w, idx, ptr, probe_out
     idx, ptr = fixed_prob_csr_device(n, n, 0.001, seed=2025, device=dev)
     nse = idx.numel()
     row = torch.repeat_interleave(torch.arange(n, device=dev, dtype=torch.int32), ptr[1:] - ptr[:-1])

@@ -340,6 +340,18 @@ int bpx_coo_sddmm(void* stream, const void* row, const void* col, int index_dtyp
                   int64_t a_keys, int64_t a_stride, int64_t a_col_stride, const void* b, int64_t b_keys,
                   int64_t b_stride, int64_t b_col_stride, int64_t ncols, int dtype, void* out);
 
+/*
+ * Measurement helper, no counterpart in the reference: the ceilings of the
+ * COO access pattern on a given (row, col, data) set -- mode 0 streams row,
+ * col and data (12 B per entry), mode 1 streams col and gathers src[col],
+ * mode 2 does both -- with one f32 sum per thread written to out (out_len a
+ * multiple of 256; one 256-thread block per 256 entries of out, at most
+ * 65536 blocks).  nse must be a multiple of 4, the arrays 16-byte aligned,
+ * and col values inside [0, src_len).
+ */
+int bpx_coo_probe(void* stream, const int32_t* row, const int32_t* col, const float* data, const float* src,
+                  int64_t src_len, int64_t nse, int mode, float* out, int64_t out_len);
+
 /* Plain device copy (bandwidth calibration of the HBM counters). */
 int bpx_copy(void* stream, const void* src, void* dst, int64_t bytes);
 
