@@ -227,6 +227,70 @@ def bench_coo(_lib, torch, steps=50, warmup=5):
     return {"workload": "coomv_100kx100k_p0.01_f32 / seg_matmul_100kx100k_p0.001_k64_f32", "results": res}
 
 
+def bench_forms(_lib, torch, steps=30, warmup=3):
+    """The other forms of the operators on the headline matrix (100,000 x
+    100,000 CSR, p = 0.01, nnz 1e8, int32, heterogeneous f32 weights):
+    bm.sparse.csrmv with a dense f32 vector (gather, and transpose = scatter
+    with atomics) and bm.event.csrmv without transpose (gather of 10 %
+    Bernoulli bool events).  The gathers read a random 4-byte (or 1-byte)
+    element per synapse, so bpx_coo_probe's read-col-and-gather time on the
+    same indices is reported as their ceiling."""
+    from brainpy_amd.connect import fixed_prob_csr_device
+    dev = _lib.device()
+    L = _lib.lib()
+    sp = _lib.stream_ptr()
+    n = 100_000
+    idx, ptr = fixed_prob_csr_device(n, n, 0.01, seed=2024, device=dev)
+    nnz = idx.numel()
+    g = torch.Generator(device=dev)
+    g.manual_seed(9)
+    w = torch.empty(nnz, dtype=torch.float32, device=dev).normal_(0.5, 0.1, generator=g)
+    x = torch.rand(n, generator=g, device=dev)
+    ev = torch.rand(n, generator=g, device=dev) < 0.1
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    res = []
+    legs = (("sparse.csrmv", 0, x, _lib.BPX_F32, _lib.KID["csr_gather"], 8),
+            ("sparse.csrmv", 1, x, _lib.BPX_F32, _lib.KID["csr_scatter"], 8),
+            ("event.csrmv", 0, ev, _lib.BPX_BOOL, _lib.KID["event_gather"], 8))
+    for name, transpose, vec, vdt, kid, per in legs:
+        def call():
+            if name == "sparse.csrmv":
+                rc = L.bpx_csrmv(sp, _lib.ptr(w), _lib.BPX_F32, nnz, _lib.ptr(idx), _lib.ptr(ptr), _lib.BPX_I32, nnz,
+                                 _lib.ptr(vec), vdt, n, n, transpose, _lib.ptr(out))
+            else:
+                rc = L.bpx_event_csrmv(sp, _lib.ptr(w), _lib.BPX_F32, nnz, _lib.ptr(idx), _lib.ptr(ptr), _lib.BPX_I32,
+                                       nnz, _lib.ptr(vec), vdt, _lib.RULE_POSITIVE, n, n, transpose, _lib.ptr(out))
+            if rc != 0:
+                raise RuntimeError(f"{name} failed ({rc})")
+        wall, kern = _time_calls(_lib, torch, call, kid, steps, warmup)
+        alg = nnz * per + (n + 1) * 4 + n * 4 + n * (1 if vdt == _lib.BPX_BOOL else 4)
+        r = {"op": name, "transpose": bool(transpose), "nnz": nnz, "us_per_call": wall * 1e6,
+             "kernel_us": kern * 1e6, "alg_bytes": alg, "gbs": alg / kern / 1e9,
+             "frac": alg / kern / 1e9 / HBM_PEAK_GBS, "synapses_per_s": nnz / wall}
+        res.append(r)
+        print(f"# {name:12s} transpose={transpose} {r['us_per_call']:8.1f} us/call kernel {r['kernel_us']:8.1f} us "
+              f"{r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%) {r['synapses_per_s']:.3e} synapses/s",
+              file=sys.stderr, flush=True)
+    probe_out = torch.empty(32768 * 256, dtype=torch.float32, device=dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for k in range(13):
+        if k == 3:
+            ev0.record()
+        rc = L.bpx_coo_probe(sp, None, _lib.ptr(idx), None, _lib.ptr(x), n, nnz, 1, _lib.ptr(probe_out),
+                             probe_out.numel())
+        if rc != 0:
+            raise RuntimeError(f"bpx_coo_probe failed ({rc})")
+    ev1.record()
+    ev1.synchronize()
+    ceiling = ev0.elapsed_time(ev1) * 1e3 / 10
+    for r in res:
+        if not r["transpose"]:
+            r["read_col_gather_ceiling_us"] = ceiling
+    print(f"# gather ceiling (read col + gather, bpx_coo_probe mode 1): {ceiling:.1f} us", file=sys.stderr,
+          flush=True)
+    return {"workload": "csrmv_forms_100kx100k_p0.01_heter_f32", "results": res}
+
+
 def _percentile(x, q):
     return float(np.percentile(np.asarray(x, np.float64), q)) if len(x) else None
 
@@ -475,7 +539,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the jitconn and network legs")
-    ap.add_argument("--only", choices=["jitconn", "batched", "coba", "served", "coo"], default=None,
+    ap.add_argument("--only", choices=["jitconn", "batched", "coba", "served", "coo", "forms"], default=None,
                     help="run only this leg (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run 3 plain device copies of this many MiB (HBM counter calibration)")
@@ -495,7 +559,7 @@ def main():
         fn = {"jitconn": lambda: bench_jitconn(L, _lib, torch, max(10, args.steps // 10), 3),
               "batched": lambda: bench_batched(_lib, torch, plain_sample=2, plain_steps=20),
               "coba": lambda: bench_coba(_lib, torch), "served": lambda: bench_served(_lib, torch),
-              "coo": lambda: bench_coo(_lib, torch)}[args.only]
+              "coo": lambda: bench_coo(_lib, torch), "forms": lambda: bench_forms(_lib, torch)}[args.only]
         print(json.dumps({"only": args.only, "result": fn()}), flush=True)
         return
     stream = torch.cuda.current_stream()
@@ -673,6 +737,7 @@ def main():
         extra["batched"] = bench_batched(_lib, torch)
         extra["served"] = bench_served(_lib, torch)
         extra["coo"] = bench_coo(_lib, torch)
+        extra["forms"] = bench_forms(_lib, torch)
 
     # CPU baseline: the oracle's serial scatter (the reference CPU loop
     # order, event/_csr_matvec.py:321-340), one thread, on a bounded number
