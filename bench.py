@@ -288,7 +288,33 @@ def bench_forms(_lib, torch, steps=30, warmup=3):
             r["read_col_gather_ceiling_us"] = ceiling
     print(f"# gather ceiling (read col + gather, bpx_coo_probe mode 1): {ceiling:.1f} us", file=sys.stderr,
           flush=True)
-    return {"workload": "csrmv_forms_100kx100k_p0.01_heter_f32", "results": res}
+    del idx, ptr, w
+    # bm.event.info: count, scan and write kernels; bytes = one read of the
+    # bool events + the int32 ids row (active ids, then -1) + the counts
+    info = []
+    for batch, m in ((1, 100_000), (256, 100_000), (1, 100_000_000)):
+        evb = torch.rand((batch, m), generator=g, device=dev) < 0.1
+        ids = torch.empty((batch, m), dtype=torch.int32, device=dev)
+        num = torch.empty(batch, dtype=torch.int32, device=dev)
+        wsb = int(L.bpx_event_info_workspace(batch, m))
+        ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
+
+        def icall():
+            rc = L.bpx_event_info(sp, _lib.ptr(evb), _lib.BPX_BOOL, batch, m, _lib.ptr(ids), _lib.ptr(num),
+                                  _lib.ptr(ws), wsb)
+            if rc != 0:
+                raise RuntimeError(f"bpx_event_info failed ({rc})")
+        wall, kern = _time_calls(_lib, torch, icall, _lib.KID["info"], steps, warmup)
+        alg = batch * m * 5 + batch * 4
+        r = {"op": "event.info", "batch": batch, "n": m, "us_per_call": wall * 1e6, "write_kernel_us": kern * 1e6,
+             "alg_bytes": alg, "gbs": alg / wall / 1e9, "frac": alg / wall / 1e9 / HBM_PEAK_GBS,
+             "events_per_s": batch * m / wall}
+        info.append(r)
+        print(f"# event.info batch={batch} n={m:.0e} {r['us_per_call']:8.1f} us/call (write kernel "
+              f"{r['write_kernel_us']:.1f} us) {r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%)", file=sys.stderr,
+              flush=True)
+        del evb, ids, ws
+    return {"workload": "csrmv_forms_100kx100k_p0.01_heter_f32 / event_info", "results": res, "info": info}
 
 
 def _percentile(x, q):

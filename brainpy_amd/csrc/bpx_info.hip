@@ -7,6 +7,11 @@
 //           inside the wave, an LDS exchange of the 4 wave counts gives the
 //           slot inside the block, the tile offset the slot in the row.
 //           Positions at or past num are filled with -1 by their owners.
+// Bool events whose rows start 16-byte aligned take k_info_count_b8 /
+// k_info_write_b8: each lane loads its 16 consecutive events with one
+// 16-byte load and tests the bytes four at a time; the write pass compacts a
+// tile's ids into LDS (block scan of the lane counts) and stores them as one
+// coalesced run.
 // The result equals the serial loop of event/_info_collection.py:143-154.
 #include "bpx_common.h"
 
@@ -40,35 +45,92 @@ __global__ __launch_bounds__(256) void k_info_count(const E* __restrict__ ev, in
   if (threadIdx.x == 0) counts[b * ntiles + tile] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
+// exclusive scan of one batch row's tile counts: thread t owns the
+// contiguous chunk [t*per, (t+1)*per) -- a serial sum, one block scan of the
+// 1024 chunk sums, then the chunk's offsets
 __global__ __launch_bounds__(1024) void k_info_scan(const int32_t* __restrict__ counts, int64_t ntiles,
                                                     int32_t* __restrict__ offsets, int32_t* __restrict__ num) {
   const int64_t b = blockIdx.x;
-  __shared__ int s_w[16];
-  __shared__ int s_carry;
-  if (threadIdx.x == 0) s_carry = 0;
-  __syncthreads();
-  const uint32_t w = threadIdx.x >> 6;
-  for (int64_t base = 0; base < ntiles; base += 1024) {
-    const int64_t i = base + threadIdx.x;
-    const int x = i < ntiles ? counts[b * ntiles + i] : 0;
-    // inclusive wave scan
-    int v = x;
+  const int32_t* cnt = counts + b * ntiles;
+  int32_t* off = offsets + b * ntiles;
+  const int64_t per = (ntiles + 1023) / 1024;
+  const int64_t lo = (int64_t)threadIdx.x * per;
+  const int64_t hi = lo + per < ntiles ? lo + per : ntiles;
+  int sum = 0;
+  for (int64_t i = lo; i < hi; ++i) sum += cnt[i];
+  int v = sum;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      int y = __shfl_up(v, o, 64);
-      if (lane_id() >= o) v += y;
-    }
-    if (lane_id() == 63) s_w[w] = v;
-    __syncthreads();
-    int wo = 0;
-    for (uint32_t k = 0; k < w; ++k) wo += s_w[k];
-    const int carry = s_carry;
-    if (i < ntiles) offsets[b * ntiles + i] = carry + wo + v - x;
-    __syncthreads();
-    if (threadIdx.x == 1023) s_carry = carry + wo + v;
-    __syncthreads();
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(v, o, 64);
+    if (lane_id() >= o) v += y;
   }
-  if (threadIdx.x == 0) num[b] = s_carry;
+  __shared__ int s_w[16];
+  const uint32_t w = threadIdx.x >> 6;
+  if (lane_id() == 63) s_w[w] = v;
+  __syncthreads();
+  int run = v - sum;
+  for (uint32_t k = 0; k < w; ++k) run += s_w[k];
+  for (int64_t i = lo; i < hi; ++i) {
+    off[i] = run;
+    run += cnt[i];
+  }
+  if (threadIdx.x == 0) {
+    int total = 0;
+    for (int k = 0; k < 16; ++k) total += s_w[k];
+    num[b] = total;
+  }
+}
+
+// the same scan with the row's counts staged in LDS (coalesced load and
+// store; ntiles <= kScanLdsTiles)
+constexpr int64_t kScanLdsTiles = 32768;
+
+__global__ __launch_bounds__(1024) void k_info_scan_lds(const int32_t* __restrict__ counts, int64_t ntiles,
+                                                        int32_t* __restrict__ offsets, int32_t* __restrict__ num) {
+  extern __shared__ int32_t s_c[];
+  const int64_t b = blockIdx.x;
+  const int32_t* cnt = counts + b * ntiles;
+  int32_t* off = offsets + b * ntiles;
+  for (int64_t i = threadIdx.x; i < ntiles; i += 1024) s_c[i] = cnt[i];
+  __syncthreads();
+  const int64_t per = (ntiles + 1023) / 1024;
+  const int64_t lo = (int64_t)threadIdx.x * per;
+  const int64_t hi = lo + per < ntiles ? lo + per : ntiles;
+  int sum = 0;
+  for (int64_t i = lo; i < hi; ++i) sum += s_c[i];
+  int v = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(v, o, 64);
+    if (lane_id() >= o) v += y;
+  }
+  __shared__ int s_w[16];
+  const uint32_t w = threadIdx.x >> 6;
+  if (lane_id() == 63) s_w[w] = v;
+  __syncthreads();
+  int run = v - sum;
+  for (uint32_t k = 0; k < w; ++k) run += s_w[k];
+  for (int64_t i = lo; i < hi; ++i) {
+    const int c = s_c[i];
+    s_c[i] = run;
+    run += c;
+  }
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < ntiles; i += 1024) off[i] = s_c[i];
+  if (threadIdx.x == 0) {
+    int total = 0;
+    for (int k = 0; k < 16; ++k) total += s_w[k];
+    num[b] = total;
+  }
+}
+
+static void launch_info_scan(hipStream_t s, const int32_t* counts, int64_t ntiles, int64_t batch, int32_t* offsets,
+                             int32_t* num) {
+  if (ntiles <= kScanLdsTiles)
+    hipLaunchKernelGGL(k_info_scan_lds, dim3((unsigned)batch), dim3(1024), (size_t)ntiles * 4, s, counts, ntiles,
+                       offsets, num);
+  else
+    hipLaunchKernelGGL(k_info_scan, dim3((unsigned)batch), dim3(1024), 0, s, counts, ntiles, offsets, num);
 }
 
 template <typename E>
@@ -105,6 +167,94 @@ __global__ __launch_bounds__(256) void k_info_write(const E* __restrict__ ev, in
   }
 }
 
+// bool (0/1 byte) events: bit 7 of each byte of the result is set iff that
+// byte of w is non-zero
+__device__ __forceinline__ uint32_t nz_bytes(uint32_t w) {
+  return (((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w) & 0x80808080u;
+}
+
+// the 16 events of lane t of a tile: [tile*4096 + 16t, +16), zero past n
+__device__ __forceinline__ void load16(const uint8_t* __restrict__ row, int64_t i0, int64_t n, uint32_t (&m)[4]) {
+  if (i0 + 16 <= n) {
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    const u4 v = *(const u4*)(row + i0);
+    m[0] = v[0]; m[1] = v[1]; m[2] = v[2]; m[3] = v[3];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint32_t w = 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t i = i0 + 4 * q + j;
+        if (i < n) w |= (uint32_t)row[i] << (8 * j);
+      }
+      m[q] = w;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) m[q] = nz_bytes(m[q]);
+}
+
+__global__ __launch_bounds__(256) void k_info_count_b8(const uint8_t* __restrict__ ev, int64_t n, int64_t ntiles,
+                                                       int32_t* __restrict__ counts) {
+  const int64_t b = blockIdx.y;
+  const int64_t tile = blockIdx.x;
+  uint32_t m[4];
+  load16(ev + b * n, tile * kInfoTile + (int64_t)threadIdx.x * 16, n, m);
+  int v = __popc(m[0]) + __popc(m[1]) + __popc(m[2]) + __popc(m[3]);
+  __shared__ int s_w[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (lane_id() == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) counts[b * ntiles + tile] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+__global__ __launch_bounds__(256) void k_info_write_b8(const uint8_t* __restrict__ ev, int64_t n, int64_t ntiles,
+                                                       const int32_t* __restrict__ offsets,
+                                                       const int32_t* __restrict__ num, int32_t* __restrict__ ids) {
+  const int64_t b = blockIdx.y;
+  const int64_t tile = blockIdx.x;
+  int32_t* out = ids + b * n;
+  const int64_t t0 = tile * kInfoTile;
+  const int64_t i0 = t0 + (int64_t)threadIdx.x * 16;
+  uint32_t m[4];
+  load16(ev + b * n, i0, n, m);
+  const int c = __popc(m[0]) + __popc(m[1]) + __popc(m[2]) + __popc(m[3]);
+  // block exclusive scan of the lane counts
+  int v = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(v, o, 64);
+    if (lane_id() >= o) v += y;
+  }
+  __shared__ int s_w[4];
+  __shared__ int32_t s_ids[kInfoTile];
+  const uint32_t w = threadIdx.x >> 6;
+  if (lane_id() == 63) s_w[w] = v;
+  __syncthreads();
+  int pos = v - c;
+  for (uint32_t q = 0; q < w; ++q) pos += s_w[q];
+  const int cnt = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (m[q] & (0x80u << (8 * j))) s_ids[pos++] = (int32_t)(i0 + 4 * q + j);
+    }
+  }
+  __syncthreads();
+  const int64_t off = offsets[b * ntiles + tile];
+  for (int k = threadIdx.x; k < cnt; k += 256) out[off + k] = s_ids[k];
+  // positions at or past num belong to the -1 fill; each tile fills its own range
+  const int64_t total = num[b];
+#pragma unroll
+  for (int k = 0; k < kInfoRounds; ++k) {
+    const int64_t i = t0 + (int64_t)k * 256 + threadIdx.x;
+    if (i < n && i >= total) out[i] = -1;
+  }
+}
+
 template <typename E>
 static int run_info(hipStream_t s, const void* events, int64_t batch, int64_t n, int32_t* ids, int32_t* num,
                     int32_t* ws) {
@@ -112,8 +262,18 @@ static int run_info(hipStream_t s, const void* events, int64_t batch, int64_t n,
   int32_t* counts = ws;
   int32_t* offsets = ws + batch * ntiles;
   dim3 grid((unsigned)ntiles, (unsigned)batch);
+  // bool rows that start 16-byte aligned: 16 events per lane per load
+  const bool b8 = sizeof(E) == 1 && ((uintptr_t)events & 15u) == 0 && (n & 15) == 0;
+  if (b8) {
+    hipLaunchKernelGGL(k_info_count_b8, grid, dim3(256), 0, s, (const uint8_t*)events, n, ntiles, counts);
+    launch_info_scan(s, counts, ntiles, batch, offsets, num);
+    BPX_LAUNCH(KID_INFO, k_info_write_b8, grid, dim3(256), 0, s, (const uint8_t*)events, n, ntiles, offsets, num,
+               ids);
+    BPX_CHECK_LAUNCH();
+    return BPX_OK;
+  }
   hipLaunchKernelGGL(k_info_count<E>, grid, dim3(256), 0, s, (const E*)events, n, ntiles, counts);
-  hipLaunchKernelGGL(k_info_scan, dim3((unsigned)batch), dim3(1024), 0, s, counts, ntiles, offsets, num);
+  launch_info_scan(s, counts, ntiles, batch, offsets, num);
   BPX_LAUNCH(KID_INFO, k_info_write<E>, grid, dim3(256), 0, s, (const E*)events, n, ntiles, offsets, num, ids);
   BPX_CHECK_LAUNCH();
   return BPX_OK;

@@ -144,6 +144,22 @@ def test_event_info_batched_exact():
     np.testing.assert_array_equal(num.cpu().numpy(), rnum)
 
 
+@pytest.mark.parametrize("n", [16, 4112, 65584, 1000000])
+def test_event_info_bool_16_aligned_rows(n):
+    # rows of a multiple of 16 bool events take the 16-events-per-load path;
+    # a view starting one byte in takes the generic path: both exact
+    ev = np.stack([bernoulli(n, r, 31 + k) for k, r in enumerate((0.5, 0.02, 1.0, 0.0))])
+    ids, num = bm.event.info(ev)
+    rid, rnum = orc.event_info(ev)
+    np.testing.assert_array_equal(ids.cpu().numpy(), rid)
+    np.testing.assert_array_equal(num.cpu().numpy(), rnum)
+    flat = torch.tensor(np.concatenate([[True], ev[1]]), device="cuda")
+    ids1, num1 = bm.event.info(flat[1:])
+    rid1, rnum1 = orc.event_info(ev[1])
+    np.testing.assert_array_equal(ids1.cpu().numpy(), rid1)
+    np.testing.assert_array_equal(num1.cpu().numpy(), rnum1)
+
+
 def test_fixed_order_bit_exact_vs_serial_scatter():
     m, n = 3000, 2500
     indices, indptr = fixed_prob_csr(m, n, 0.02, 11, sort=False)
