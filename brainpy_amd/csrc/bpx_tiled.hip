@@ -270,20 +270,25 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
     // list the batch's active rows with their tile-p segments
     // (inactive lanes read row 0's pair: one shared line, no branch, so the
     // four chunks' loads are in flight together)
-    int2 sgl[kChunksPerBatch];
-#pragma unroll
-    for (int u = 0; u < kChunksPerBatch; ++u) {
-      const int64_t r = act[u] ? ((q0 + cstep * u) << 6) + lane : 0;
-      const int32_t* rt = rowtile + r * (P + 1) + p;
-      sgl[u] = make_int2(rt[0], rt[1]);
-    }
     int cnt = 0;
+    bool any = false;
 #pragma unroll
-    for (int u = 0; u < kChunksPerBatch; ++u) {
-      const bool nz = act[u] && sgl[u].y > sgl[u].x;
-      const uint64_t mask = __ballot(nz);
-      if (nz) seg[cnt + lanes_below(mask)] = sgl[u];
-      cnt += __popcll(mask);
+    for (int u = 0; u < kChunksPerBatch; ++u) any = any || act[u];
+    if (__ballot(any) != 0ull) {   // wave-uniform: a batch without active rows skips the offset loads
+      int2 sgl[kChunksPerBatch];
+#pragma unroll
+      for (int u = 0; u < kChunksPerBatch; ++u) {
+        const int64_t r = act[u] ? ((q0 + cstep * u) << 6) + lane : 0;
+        const int32_t* rt = rowtile + r * (P + 1) + p;
+        sgl[u] = make_int2(rt[0], rt[1]);
+      }
+#pragma unroll
+      for (int u = 0; u < kChunksPerBatch; ++u) {
+        const bool nz = act[u] && sgl[u].y > sgl[u].x;
+        const uint64_t mask = __ballot(nz);
+        if (nz) seg[cnt + lanes_below(mask)] = sgl[u];
+        cnt += __popcll(mask);
+      }
     }
     // prefetch the next batch's events
     const int64_t qn = q0 + cstep * kChunksPerBatch;
