@@ -52,13 +52,15 @@ INT_PEAK_TOPS = 39.3     # 256 CUs x 64 lanes x 2.4 GHz, 32-bit integer VALU
 OPS_PER_DRAW = 24        # LFSR88 next (18) + 2 xor + modulo + 3 add/compare
 
 
-def bench_jitconn(L, _lib, torch, steps, warmup):
+def bench_jitconn(L, _lib, torch, steps, warmup, cpu=True):
     """1M x 1M just-in-time connectivity, p = 1e-4 (clen 19999), seed 42,
     2 % Bernoulli bool spikes; homo / uniform [0,1] / normal (0.5, 0.1);
     outdim_parallel True (row-parallel, the default) and False (event-driven
     scatter).  Work = LFSR88 draws, counted exactly from the generated
     connections: per stream 1 initial draw + 1 per connection (+1 uniform /
-    +2 normal weight draws per connection)."""
+    +2 normal weight draws per connection).  With `cpu`, each configuration
+    is also run once by the oracle's serial C restatement on the host
+    (1 thread) on the same events, and the outputs are compared."""
     import ctypes
     from brainpy_amd.math.jitconn import conn_len
     n = 1_000_000
@@ -112,9 +114,25 @@ def bench_jitconn(L, _lib, torch, steps, warmup):
             r = {"dist": ["homo", "uniform", "normal"][dist], "outdim_parallel": bool(od), "us_per_call": wall * 1e6,
                  "kernel_us": kern * 1e6, "events_per_call": events, "events_per_s": events / wall,
                  "draws": draws, "int_tops": ops / kern / 1e12, "int_frac": ops / kern / 1e12 / INT_PEAK_TOPS}
+            if cpu:
+                # the oracle's serial restatement of the same lane32 kernel
+                # (the reference's Taichi loop order), one call, 1 thread
+                from oracle import orc
+                ev_h = ev.cpu().numpy()
+                t0 = time.perf_counter()
+                y_cpu = orc.jitconn_mv(ev_h, dist, 1, od, orc.LAYOUT_LANE32, n, n, np.float32(w0), np.float32(w1),
+                                       clen, 42)
+                t_cpu = time.perf_counter() - t0
+                y_gpu = out.cpu().numpy()
+                r["cpu_port_s"] = t_cpu
+                r["cpu_port_events_per_s"] = events / t_cpu
+                r["gpu_equals_cpu"] = bool(np.array_equal(y_gpu, y_cpu))
+                r["gpu_vs_cpu_max_abs_diff"] = float(np.max(np.abs(y_gpu.astype(np.float64) - y_cpu)))
             res.append(r)
             print(f"# jitconn od={od} {r['dist']:7s} {r['us_per_call']:9.1f} us/call kernel {r['kernel_us']:9.1f} us "
-                  f"{r['int_tops']:6.2f} Tops ({100*r['int_frac']:.1f}%) {r['events_per_s']:.3e} ev/s",
+                  f"{r['int_tops']:6.2f} Tops ({100*r['int_frac']:.1f}%) {r['events_per_s']:.3e} ev/s"
+                  + (f" | cpu port {r['cpu_port_s']:.3f} s/call (1 thread), equal={r['gpu_equals_cpu']} "
+                     f"max|diff|={r['gpu_vs_cpu_max_abs_diff']:.2e}" if cpu else ""),
                   file=sys.stderr, flush=True)
     return {"workload": "jitconn_event_mv_prob_1Mx1M_p1e-4_seed42_rate0.02", "results": res}
 
@@ -758,7 +776,7 @@ def main():
 
     extra = {}
     if not args.no_extra:
-        extra["jitconn"] = bench_jitconn(L, _lib, torch, max(10, args.steps // 10), 3)
+        extra["jitconn"] = bench_jitconn(L, _lib, torch, max(10, args.steps // 10), 3, cpu=not args.no_cpu)
         extra["coba"] = bench_coba(_lib, torch)
         extra["batched"] = bench_batched(_lib, torch)
         extra["served"] = bench_served(_lib, torch)
