@@ -243,6 +243,39 @@ def test_tiled_plan_accumulate_and_float_events():
     np.testing.assert_array_equal(base.cpu().numpy(), ref)
 
 
+def test_full_size_headline_matrix():
+    # the 100k x 100k, p = 0.01 matrix (1e8 synapses, int32) with 10 % bool
+    # spikes: with weight 1 every output is an integer count, so the tiled
+    # plan, the stateless scatter and gather, and the oracle agree exactly;
+    # heterogeneous f32 weights are checked against the oracle's FP64
+    # accumulation within 1e-5 of each output's sum of |w|
+    from brainpy_amd.connect import fixed_prob_csr_device
+    m = n = 100_000
+    dev = torch.device("cuda")
+    idx, ptr = fixed_prob_csr_device(m, n, 0.01, seed=2024, device=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(17)
+    ev = torch.rand(m, generator=g, device=dev) < 0.1
+    idx_h, ptr_h, ev_h = idx.cpu().numpy(), ptr.cpu().numpy(), ev.cpu().numpy()
+    one = np.float32(1.0)
+    ref = orc.event_csrmv(one, idx_h, ptr_h, ev_h, (m, n), True)
+    assert float(ref.sum()) == float(ev_h.sum()) * 1000
+    np.testing.assert_array_equal(bm.event.csrmv(one, idx, ptr, ev, shape=(m, n), transpose=True).cpu().numpy(),
+                                  ref)
+    np.testing.assert_array_equal(bm.event.TiledCSR(one, idx, ptr, shape=(m, n)).csrmv(ev).cpu().numpy(), ref)
+    ev_n = torch.rand(n, generator=g, device=dev) < 0.1
+    np.testing.assert_array_equal(bm.event.csrmv(one, idx, ptr, ev_n, shape=(m, n)).cpu().numpy(),
+                                  orc.event_csrmv(one, idx_h, ptr_h, ev_n.cpu().numpy(), (m, n), False))
+    w = torch.empty(idx.numel(), dtype=torch.float32, device=dev).normal_(0.5, 0.1, generator=g)
+    w_h = w.cpu().numpy().astype(np.float64)
+    ref64 = orc.event_csrmv(w_h, idx_h, ptr_h, ev_h, (m, n), True)
+    mag = orc.event_csrmv(np.abs(w_h), idx_h, ptr_h, ev_h, (m, n), True)
+    for y in (bm.event.csrmv(w, idx, ptr, ev, shape=(m, n), transpose=True),
+              bm.event.TiledCSR(w, idx, ptr, shape=(m, n)).csrmv(ev)):
+        err = np.abs(y.cpu().numpy().astype(np.float64) - ref64)
+        assert np.all(err <= 1e-5 * mag + 1e-30), float(np.max(err / np.maximum(mag, 1e-30)))
+
+
 def bm_connect_csr(m, n, p, seed, sort):
     from brainpy_amd.connect import fixed_prob_csr as fp
     return fp(m, n, p, seed, sort=sort)
