@@ -38,3 +38,19 @@ def f32_to_bf16_bits(x):
 
 def bf16_bits_to_f32(b):
     return (np.asarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def coba4000_inputs(seed=0):
+    """The 4000-neuron E/I COBA network of EICOBANet(seed=seed) on the host:
+    the four FixedProb p=0.02 projections in EICOBANet's order (E2I, I2I,
+    I2E, E2E; seed * 7919 + k + 1), their combined per-neuron CSR, and the
+    initial potentials U[-60, -50] from PCG64(seed)."""
+    from brainpy_amd.dyn.batched import combined_csr
+    size = {"E": 3200, "I": 800}
+    csrs = {}
+    for k, (name, pre, post) in enumerate((("E2I", "E", "I"), ("I2I", "I", "I"), ("I2E", "I", "E"),
+                                           ("E2E", "E", "E"))):
+        csrs[name] = fixed_prob_csr(size[pre], size[post], 0.02, seed * 7919 + k + 1)
+    V0 = np.random.Generator(np.random.PCG64(seed)).uniform(-60.0, -50.0, 4000).astype(np.float32)
+    idx, ptr = combined_csr(csrs, 3200, 800)
+    return csrs, idx, ptr, V0

@@ -148,6 +148,21 @@ def test_csr_network_vs_eicoba_oracle():
     np.testing.assert_array_equal(raster, ref)
 
 
+def test_coba4000_fixture_first_100_steps():
+    # the committed 100-step raster of the 4000-neuron network: the op-by-op
+    # fixed-order path and the batched step reproduce it bit for bit
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "coba4000_raster100.npz"))
+    ref = np.unpackbits(d["raster_bits"], axis=1)[:, :4000]
+    net = EICOBANet(seed=0, mode="fixed_order")
+    np.testing.assert_array_equal(net.V0, d["V0"])
+    np.testing.assert_array_equal(net.run(100).cpu().numpy(), ref)
+    csrs = {name: (net.csr[name][1], net.csr[name][2]) for name in ("E2E", "E2I", "I2E", "I2I")}
+    spec = dict(n_exc=3200, n_inh=800, seed=0, w_exc=0.6, w_inh=6.7, I_ext=20.0, csr=csrs)
+    b = BatchedCOBA([spec], V_init=net.V0)
+    np.testing.assert_array_equal(b.run(100, raster=True).cpu().numpy(), ref)
+
+
 def test_plain_path_equals_batched():
     from brainpy_amd.dyn.batched import PlainCOBA
     specs = _jit_specs([1500, 3000], seed0=11)

@@ -243,6 +243,20 @@ def test_tiled_plan_accumulate_and_float_events():
     np.testing.assert_array_equal(base.cpu().numpy(), ref)
 
 
+def test_event_csrmv_fixture():
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "event_csrmv_200x100.npz"))
+    idx, ptr, w = d["indices"], d["indptr"], d["data"]
+    y_t = bm.event.csrmv(w, idx, ptr, d["events_transpose"], shape=(200, 100), transpose=True).cpu().numpy()
+    y_g = bm.event.csrmv(w, idx, ptr, d["events_gather"], shape=(200, 100)).cpu().numpy()
+    np.testing.assert_allclose(y_t, d["y_transpose"], rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(y_g, d["y_gather"], rtol=1e-5, atol=1e-5)
+    # the fixed-order mode sums in the serial loop's order: bit-exact
+    rows, colptr, w_csc = bm.event.csr_to_csc(idx, ptr, (200, 100), data=w)
+    y_fo = bm.event.csrmv_fixed_order(w_csc, rows, colptr, d["events_transpose"], shape=(200, 100)).cpu().numpy()
+    np.testing.assert_array_equal(y_fo, d["y_transpose"])
+
+
 def test_full_size_headline_matrix():
     # the 100k x 100k, p = 0.01 matrix (1e8 synapses, int32) with 10 % bool
     # spikes: with weight 1 every output is an integer count, so the tiled
