@@ -335,6 +335,39 @@ def bench_forms(_lib, torch, steps=30, warmup=3):
     return {"workload": "csrmv_forms_100kx100k_p0.01_heter_f32 / event_info", "results": res, "info": info}
 
 
+def bench_conn(_lib, torch, reps=5):
+    """FixedProb.build_csr of the headline matrix (100,000 x 100,000,
+    p = 0.01, 1e8 targets): the device builder (bpx_fixed_prob_csr: uniform
+    distinct sorted targets), the torch-based input generator the other
+    benchmarks use, and the host numpy builder on a 2,000-row sample scaled
+    to all rows (1 thread; the reference draws rows with Numba's rng.choice,
+    connect/random_conn.py:92-105)."""
+    from brainpy_amd.connect import fixed_prob_csr, fixed_prob_csr_device, fixed_prob_csr_gpu
+    dev = _lib.device()
+    n = 100_000
+    res = {}
+    for name, fn in (("device_builder", lambda: fixed_prob_csr_gpu(n, n, 0.01, seed=2024)),
+                     ("torch_generator", lambda: fixed_prob_csr_device(n, n, 0.01, seed=2024, device=dev))):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        res[name + "_s"] = (time.perf_counter() - t0) / reps
+        del out
+    sample = 2000
+    t0 = time.perf_counter()
+    fixed_prob_csr(sample, n, 0.01, seed=2024)
+    res["host_numpy_s_scaled"] = (time.perf_counter() - t0) * n / sample
+    res["synapses"] = n * 1000
+    res["device_synapses_per_s"] = n * 1000 / res["device_builder_s"]
+    print(f"# conn FixedProb 100k x 100k p=0.01: device builder {res['device_builder_s'] * 1e3:.1f} ms "
+          f"({res['device_synapses_per_s']:.3e} synapses/s), torch generator {res['torch_generator_s'] * 1e3:.1f} ms, "
+          f"host numpy (scaled from {sample} rows) {res['host_numpy_s_scaled']:.1f} s", file=sys.stderr, flush=True)
+    return {"workload": "fixed_prob_build_csr_100kx100k_p0.01", **res}
+
+
 def _percentile(x, q):
     return float(np.percentile(np.asarray(x, np.float64), q)) if len(x) else None
 
@@ -583,7 +616,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the jitconn and network legs")
-    ap.add_argument("--only", choices=["jitconn", "batched", "coba", "served", "coo", "forms"], default=None,
+    ap.add_argument("--only", choices=["jitconn", "batched", "coba", "served", "coo", "forms", "conn"], default=None,
                     help="run only this leg (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run 3 plain device copies of this many MiB (HBM counter calibration)")
@@ -603,7 +636,8 @@ def main():
         fn = {"jitconn": lambda: bench_jitconn(L, _lib, torch, max(10, args.steps // 10), 3),
               "batched": lambda: bench_batched(_lib, torch, plain_sample=2, plain_steps=20),
               "coba": lambda: bench_coba(_lib, torch), "served": lambda: bench_served(_lib, torch),
-              "coo": lambda: bench_coo(_lib, torch), "forms": lambda: bench_forms(_lib, torch)}[args.only]
+              "coo": lambda: bench_coo(_lib, torch), "forms": lambda: bench_forms(_lib, torch),
+              "conn": lambda: bench_conn(_lib, torch)}[args.only]
         print(json.dumps({"only": args.only, "result": fn()}), flush=True)
         return
     stream = torch.cuda.current_stream()
@@ -782,6 +816,7 @@ def main():
         extra["served"] = bench_served(_lib, torch)
         extra["coo"] = bench_coo(_lib, torch)
         extra["forms"] = bench_forms(_lib, torch)
+        extra["conn"] = bench_conn(_lib, torch)
 
     # CPU baseline: the oracle's serial scatter (the reference CPU loop
     # order, event/_csr_matvec.py:321-340), one thread, on a bounded number

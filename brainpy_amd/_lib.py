@@ -68,6 +68,8 @@ SIGNATURES = {
     "bpx_coo_sddmm": ([_vp, _vp, _vp, _i32, _i64, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _i32, _vp],
                       _i32),
     "bpx_coo_probe": ([_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i32, _vp, _i64], _i32),
+    "bpx_fixed_prob_workspace": ([_i64], _sz),
+    "bpx_fixed_prob_csr": ([_vp, _i64, _i64, _i64, ctypes.c_uint64, _i32, _i32, _vp, _vp, _vp, _sz], _i32),
     "bpx_tiled_csr_layout": ([_i64, _i64, _i64, _i32, ctypes.POINTER(TiledCSRPlan), ctypes.POINTER(_sz)], _i32),
     "bpx_tiled_csr_build": ([_vp, _vp, _i32, _i64, _vp, _vp, _i32, ctypes.POINTER(TiledCSRPlan), _vp, _sz], _i32),
     "bpx_event_csrmv_tiled": ([_vp, ctypes.POINTER(TiledCSRPlan), _vp, _i32, _i32, _i32, _vp], _i32),

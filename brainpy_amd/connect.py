@@ -68,15 +68,63 @@ class CSRConn:
         return self.indices, self.indptr
 
 
-class FixedProb(CSRConn):
-    """``FixedProb(prob, seed=...)`` applied to ``(pre_num, post_num)``
-    (connect/random_conn.py:85-128 with ``pre_ratio=1``, no self-connection
-    filter): every pre-synaptic neuron gets ``int(post_num * prob)`` distinct
-    targets, drawn here from a seeded PCG64 stream."""
+class ConnectorError(Exception):
+    """Raised for connector arguments the reference rejects (brainpy.errors.ConnectorError)."""
 
-    def __init__(self, prob, pre_num, post_num, seed=0):
+
+def fixed_prob_csr_gpu(m, n, prob, seed, include_self=True, allow_multi_conn=False):
+    """``FixedProb.build_csr`` on the GPU (``bpx_fixed_prob_csr``): every
+    one of the m rows gets ``k = int(n * prob)`` targets drawn uniformly from
+    [0, n) -- distinct and ascending, or with replacement and unsorted when
+    ``allow_multi_conn`` -- and with ``include_self=False`` (m == n) a row
+    drops its own index when drawn (connect/random_conn.py:73-76, 118-128).
+    Returns int32 device tensors ``(indices, indptr)``; the seeded stream is
+    this library's own (the same seed gives the same matrix)."""
+    import torch
+    from . import _lib
+    m, n = int(m), int(n)
+    if not include_self and m != n:
+        raise ConnectorError(f'We found pre_num != post_num ({m} != {n}). But `include_self` is set to True.')
+    k = int(n * prob)
+    dev = _lib.device()
+    indices = torch.empty(max(m * k, 1), dtype=torch.int32, device=dev)
+    indptr = torch.empty(m + 1, dtype=torch.int32, device=dev)
+    ws_bytes = int(_lib.lib().bpx_fixed_prob_workspace(m)) if not include_self else 0
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    rc = _lib.lib().bpx_fixed_prob_csr(_lib.stream_ptr(), m, n, k, int(seed) & 0xFFFFFFFFFFFFFFFF, int(include_self),
+                                       int(allow_multi_conn), _lib.ptr(indices), _lib.ptr(indptr), _lib.ptr(ws),
+                                       ws_bytes)
+    _lib.check(rc, "bpx_fixed_prob_csr")
+    total = m * k if (include_self or allow_multi_conn) else int(indptr[-1])
+    return indices[:total], indptr
+
+
+class FixedProb(CSRConn):
+    """``FixedProb(prob, include_self=..., allow_multi_conn=..., seed=...)``
+    applied to ``(pre_num, post_num)`` (connect/random_conn.py:85-128 with
+    ``pre_ratio=1``): every pre-synaptic neuron gets ``int(post_num * prob)``
+    targets.  ``backend="gpu"`` builds the CSR on the device with
+    :func:`fixed_prob_csr_gpu` (uniform distinct targets, the reference's
+    semantics); ``backend="host"`` uses the seeded numpy generator of
+    :func:`fixed_prob_csr` (the synthetic inputs of the tests), where
+    include_self/allow_multi_conn are not supported."""
+
+    def __init__(self, prob, pre_num, post_num, seed=0, include_self=True, allow_multi_conn=False, backend="host"):
         if not 0.0 <= prob <= 1.0:
             raise ValueError(f"prob must be in [0, 1], got {prob}")
         self.prob, self.seed = float(prob), int(seed)
-        indices, indptr = fixed_prob_csr(int(pre_num), int(post_num), prob, seed)
+        self.include_self, self.allow_multi_conn = bool(include_self), bool(allow_multi_conn)
+        if backend == "gpu":
+            indices, indptr = fixed_prob_csr_gpu(pre_num, post_num, prob, seed, include_self, allow_multi_conn)
+        elif backend == "host":
+            if not include_self or allow_multi_conn:
+                raise ValueError('backend="host" builds distinct targets with self-connections; '
+                                 'use backend="gpu" for include_self=False or allow_multi_conn=True')
+            indices, indptr = fixed_prob_csr(int(pre_num), int(post_num), prob, seed)
+        else:
+            raise ValueError(f'backend must be "host" or "gpu", got {backend!r}')
         super().__init__(indices, indptr, pre_num, post_num)
+
+    def build_csr(self):
+        """``(indices, indptr)`` (connect/random_conn.py:118-128)."""
+        return self.indices, self.indptr

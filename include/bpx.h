@@ -340,6 +340,24 @@ int bpx_coo_sddmm(void* stream, const void* row, const void* col, int index_dtyp
                   int64_t a_keys, int64_t a_stride, int64_t a_col_stride, const void* b, int64_t b_keys,
                   int64_t b_stride, int64_t b_col_stride, int64_t ncols, int dtype, void* out);
 
+/* -------------------------------------------------------------------------
+ * FixedProb connectivity on the device.  Replaces FixedProb.build_csr
+ * (connect/random_conn.py:85-128, pre_ratio = 1): each of the m rows gets
+ * k = int(n * prob) distinct targets drawn uniformly from [0, n) (the
+ * reference's rng.choice(n, k, replace=False)), written in ascending order;
+ * include_self == 0 (m == n required, as the reference raises otherwise)
+ * drops a row's own index when drawn (row length k - 1), which needs a
+ * workspace of bpx_fixed_prob_workspace(m) bytes; allow_multi_conn != 0 draws
+ * the k targets with replacement, unsorted.  The random stream (SplitMix64
+ * of seed, row and draw index) is this library's own: the same seed gives the
+ * same matrix on every call, not the reference's matrix.  n <= 1,310,720
+ * without replacement (the row bitmap lives in LDS); m * k < 2^31.
+ * ------------------------------------------------------------------------- */
+size_t bpx_fixed_prob_workspace(int64_t m);
+int bpx_fixed_prob_csr(void* stream, int64_t m, int64_t n, int64_t k, uint64_t seed, int include_self,
+                       int allow_multi_conn, int32_t* indices, int32_t* indptr, void* workspace,
+                       size_t workspace_bytes);
+
 /*
  * Measurement helper, no counterpart in the reference: the ceilings of the
  * COO access pattern on a given (row, col, data) set -- mode 0 streams row,
