@@ -289,6 +289,24 @@ def bench_forms(_lib, torch, steps=30, warmup=3):
         print(f"# {name:12s} transpose={transpose} {r['us_per_call']:8.1f} us/call kernel {r['kernel_us']:8.1f} us "
               f"{r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%) {r['synapses_per_s']:.3e} synapses/s",
               file=sys.stderr, flush=True)
+    # the transpose with the plan for a static matrix (TiledCSR.matvec): all
+    # 1e8 synapses streamed once at 2 + 4 bytes, no global atomic per synapse
+    from brainpy_amd.math.event import TiledCSR
+    plan = TiledCSR(w, idx, ptr, shape=(n, n))
+    xs = torch.rand(n, generator=g, device=dev) + 0.5
+
+    def pcall():
+        plan.matvec(xs)
+    wall, kern = _time_calls(_lib, torch, pcall, _lib.KID["tiled"], steps, warmup)
+    alg = n * 4 + n * 8 * plan.plan.tiles + nnz * 6 + n * 4
+    r = {"op": "sparse.csrmv (TiledCSR.matvec)", "transpose": True, "nnz": nnz, "us_per_call": wall * 1e6,
+         "kernel_us": kern * 1e6, "alg_bytes": alg, "gbs": alg / kern / 1e9, "frac": alg / kern / 1e9 / HBM_PEAK_GBS,
+         "synapses_per_s": nnz / wall}
+    res.append(r)
+    print(f"# sparse.csrmv transpose=1 on the plan {r['us_per_call']:8.1f} us/call kernel {r['kernel_us']:8.1f} us "
+          f"{r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%) {r['synapses_per_s']:.3e} synapses/s", file=sys.stderr,
+          flush=True)
+    del plan, xs
     probe_out = torch.empty(32768 * 256, dtype=torch.float32, device=dev)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for k in range(13):

@@ -73,6 +73,7 @@ SIGNATURES = {
     "bpx_tiled_csr_layout": ([_i64, _i64, _i64, _i32, ctypes.POINTER(TiledCSRPlan), ctypes.POINTER(_sz)], _i32),
     "bpx_tiled_csr_build": ([_vp, _vp, _i32, _i64, _vp, _vp, _i32, ctypes.POINTER(TiledCSRPlan), _vp, _sz], _i32),
     "bpx_event_csrmv_tiled": ([_vp, ctypes.POINTER(TiledCSRPlan), _vp, _i32, _i32, _i32, _vp], _i32),
+    "bpx_csrmv_tiled": ([_vp, ctypes.POINTER(TiledCSRPlan), _vp, _i32, _i32, _vp, _vp, _sz], _i32),
     "bpx_expon_coef": ([_f32, _f32], _f32),
     "bpx_timer_enable": ([_i32], _i32),
     "bpx_timer_read": ([_i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)], _i32),

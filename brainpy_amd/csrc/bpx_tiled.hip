@@ -216,16 +216,34 @@ constexpr int kWavesPerBlock = kTiledThreads / 64;
 __device__ __forceinline__ long long fixed_of(float w, double scale) { return __double2ll_rn((double)w * scale); }
 
 
-template <int WK, typename E, int R>
+// DENSE: `ev` is a dense f32 vector x (bm.sparse.csrmv with transpose): the
+// rows with x != 0 are listed with x[row] beside their segment, each weight
+// is multiplied by it (one f32 rounding, as the plain product), and the
+// fixed-point shift comes from the plan's less ceil(log2 max|x|) (device
+// scalar xmax: the f32 bits of max|x|), so the sums still stay below 2^61.
+template <int WK, typename E, int R, bool DENSE>
 __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* __restrict__ rowtile,
                                                                  const uint16_t* __restrict__ lidx,
                                                                  const void* __restrict__ wptr,
                                                                  const float* __restrict__ homo_w,
                                                                  const E* __restrict__ ev, int rule, int64_t m,
                                                                  int64_t n, int P, int T, int cpx, double fx_scale,
-                                                                 double fx_inv, float* __restrict__ out) {
+                                                                 double fx_inv, const uint32_t* __restrict__ xmax,
+                                                                 float* __restrict__ out) {
   extern __shared__ unsigned long long s_acc[];
   __shared__ int2 s_seg[kWavesPerBlock][kListCap];
+  __shared__ float s_xv[DENSE ? kWavesPerBlock : 1][DENSE ? kListCap : 1];
+  bool finite_x = true;
+  if constexpr (DENSE) {
+    const float mx = __uint_as_float(*xmax);
+    finite_x = mx <= 3.402823466e38f;       // false for inf and NaN
+    int ex = 0;
+    if (finite_x && mx > 0.f) frexpf(mx, &ex);   // mx < 2^ex
+    if (ex > 0) {
+      fx_scale = ldexp(fx_scale, -ex);
+      fx_inv = ldexp(fx_inv, ex);
+    }
+  }
   const int b = blockIdx.x;
   const int x = b & 7;                 // XCD of this block (round-robin dispatch)
   const int jb = b >> 3;
@@ -251,7 +269,7 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
 
   // loads are issued for every lane (rows past the end read row m-1 and are
   // masked), so the four chunks' loads are in flight together
-  auto load_batch = [&](int64_t qb, bool* act) {
+  auto load_batch = [&](int64_t qb, bool* act, float* xv) {
     E v[kChunksPerBatch];
 #pragma unroll
     for (int u = 0; u < kChunksPerBatch; ++u) {
@@ -261,11 +279,13 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
 #pragma unroll
     for (int u = 0; u < kChunksPerBatch; ++u) {
       const int64_t r = ((qb + cstep * u) << 6) + lane;
-      act[u] = r < m && ev_true(v[u], rule);
+      act[u] = finite_x && r < m && ev_true(v[u], rule);
+      if constexpr (DENSE) xv[u] = (float)v[u];
     }
   };
   bool act[kChunksPerBatch];
-  load_batch(q0, act);
+  float xv[kChunksPerBatch];
+  load_batch(q0, act, xv);
   while (q0 < nchunks) {
     // list the batch's active rows with their tile-p segments
     // (inactive lanes read row 0's pair: one shared line, no branch, so the
@@ -286,14 +306,18 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
       for (int u = 0; u < kChunksPerBatch; ++u) {
         const bool nz = act[u] && sgl[u].y > sgl[u].x;
         const uint64_t mask = __ballot(nz);
-        if (nz) seg[cnt + lanes_below(mask)] = sgl[u];
+        if (nz) {
+          seg[cnt + lanes_below(mask)] = sgl[u];
+          if constexpr (DENSE) s_xv[wave][cnt + lanes_below(mask)] = xv[u];
+        }
         cnt += __popcll(mask);
       }
     }
     // prefetch the next batch's events
     const int64_t qn = q0 + cstep * kChunksPerBatch;
     bool nact[kChunksPerBatch];
-    load_batch(qn, nact);
+    float nxv[kChunksPerBatch];
+    load_batch(qn, nact, nxv);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -301,8 +325,16 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
     // 8 indices (one 16-byte load) and 8 weights per row and round
     for (int e = g; e < cnt; e += R * NGRP) {
       int2 sg[R];
+      float xr[R];
+      long long hq[R];
 #pragma unroll
-      for (int q = 0; q < R; ++q) sg[q] = (e + q * NGRP < cnt) ? seg[e + q * NGRP] : make_int2(0, 0);
+      for (int q = 0; q < R; ++q) {
+        sg[q] = (e + q * NGRP < cnt) ? seg[e + q * NGRP] : make_int2(0, 0);
+        xr[q] = 1.f;
+        if constexpr (DENSE) xr[q] = (e + q * NGRP < cnt) ? s_xv[wave][e + q * NGRP] : 0.f;
+        hq[q] = hfx;
+        if constexpr (DENSE && WK == 0) hq[q] = fixed_of(hw * xr[q], fx_scale);
+      }
       int32_t len = 0;
 #pragma unroll
       for (int q = 0; q < R; ++q) len = max(len, sg[q].y - sg[q].x);
@@ -338,7 +370,10 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
           for (int u = 0; u < kVec; ++u) {
             const uint32_t c = (l4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
             if (c < (uint32_t)tlen) {
-              const long long fx = WK == 0 ? hfx : fixed_of(wv[q][u], fx_scale);
+              long long fx;
+              if constexpr (WK == 0) fx = hq[q];
+              else if constexpr (DENSE) fx = fixed_of(wv[q][u] * xr[q], fx_scale);
+              else fx = fixed_of(wv[q][u], fx_scale);
               __hip_atomic_fetch_add(s_acc + c, (unsigned long long)fx, __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_WORKGROUP);
             }
@@ -348,10 +383,20 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
     }
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int u = 0; u < kChunksPerBatch; ++u) act[u] = nact[u];
+    for (int u = 0; u < kChunksPerBatch; ++u) {
+      act[u] = nact[u];
+      xv[u] = nxv[u];
+    }
     q0 = qn;
   }
   __syncthreads();
+  if (DENSE && !finite_x) {
+    // inf or NaN in x: the products are not representable in fixed point;
+    // every output of the tile becomes NaN (written by copy 0)
+    if (copy == 0)
+      for (int k = threadIdx.x; k < tlen; k += kTiledThreads) out[t0 + k] = __uint_as_float(0x7FC00000u);
+    return;
+  }
   for (int k = threadIdx.x; k < tlen; k += kTiledThreads) {
     const long long a = (long long)s_acc[k];
     if (a != 0) atomic_add(out + t0 + k, (float)((double)a * fx_inv));
@@ -581,8 +626,9 @@ int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* e
   const size_t shm = (size_t)T * sizeof(unsigned long long);
   const double fx_scale = ldexp(1.0, plan->fx_shift), fx_inv = ldexp(1.0, -plan->fx_shift);
 #define TLR(WK, E, R)                                                                                     \
-  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E, R>), dim3(blocks), dim3(kTiledThreads), shm, s, rowtile, lidx, \
-             w, hw, (const E*)events, ev_rule, m, n, P, T, cpx, fx_scale, fx_inv, out)
+  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E, R, false>), dim3(blocks), dim3(kTiledThreads), shm, s, rowtile, \
+             lidx, w, hw, (const E*)events, ev_rule, m, n, P, T, cpx, fx_scale, fx_inv,                         \
+             (const uint32_t*)nullptr, out)
 #define TL(WK, E)                          \
   do {                                     \
     if (rows == 1) TLR(WK, E, 1);          \
@@ -600,6 +646,55 @@ int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* e
 #undef TL_E
 #undef TL
 #undef TLR
+  BPX_CHECK_LAUNCH();
+  return BPX_OK;
+}
+
+int bpx_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* x, int x_dtype, int accumulate, float* out,
+                    void* workspace, size_t workspace_bytes) {
+  if (!plan || !plan->mem) return BPX_ERR_ARG;
+  if (x_dtype != BPX_F32) return BPX_ERR_DTYPE;
+  if (!workspace || workspace_bytes < sizeof(uint32_t)) return BPX_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t m = plan->m, n = plan->n;
+  if (!accumulate) {
+    int64_t zb = (n + 255) / 256;
+    if (zb > 4096) zb = 4096;
+    hipLaunchKernelGGL(k_fill_zero_t<float>, dim3((unsigned)zb), dim3(256), 0, s, out, n);
+  }
+  if (m == 0) {
+    BPX_CHECK_LAUNCH();
+    return BPX_OK;
+  }
+  uint32_t* xmax = (uint32_t*)workspace;
+  hipLaunchKernelGGL(k_fill_zero_t<uint32_t>, dim3(1), dim3(64), 0, s, xmax, (int64_t)1);
+  int64_t ab = (m + 255) / 256;
+  if (ab > 1024) ab = 1024;
+  hipLaunchKernelGGL(k_absmax<float>, dim3((unsigned)ab), dim3(256), 0, s, (const float*)x, m, xmax);
+  const char* base = (const char*)plan->mem;
+  const int32_t* rowtile = (const int32_t*)(base + plan->rowtile_offset);
+  const uint16_t* lidx = (const uint16_t*)(base + plan->lidx_offset);
+  const void* w = base + plan->weights_offset;
+  const float* hw = (const float*)(base + plan->homo_offset);
+  const int P = plan->tiles, T = plan->tile_len;
+  const int cpx = tiled_copies_per_xcd();
+  const int rows = tiled_rows_per_group();
+  const unsigned blocks = (unsigned)(8 * cpx * P);
+  const size_t shm = (size_t)T * sizeof(unsigned long long);
+  const double fx_scale = ldexp(1.0, plan->fx_shift), fx_inv = ldexp(1.0, -plan->fx_shift);
+#define TLD(WK, R)                                                                                            \
+  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, float, R, true>), dim3(blocks), dim3(kTiledThreads), shm, s,      \
+             rowtile, lidx, w, hw, (const float*)x, BPX_RULE_NONZERO, m, n, P, T, cpx, fx_scale, fx_inv, xmax, out)
+#define TLDR(WK)                         \
+  do {                                   \
+    if (rows == 1) TLD(WK, 1);           \
+    else TLD(WK, 2);                     \
+  } while (0)
+  if (plan->weight_dtype == BPX_TILED_HOMO) TLDR(0);
+  else if (plan->weight_dtype == BPX_F32) TLDR(1);
+  else TLDR(2);
+#undef TLDR
+#undef TLD
   BPX_CHECK_LAUNCH();
   return BPX_OK;
 }

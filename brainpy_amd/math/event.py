@@ -328,3 +328,30 @@ class TiledCSR:
                                               _lib.bpx_dtype(ev), rule, int(accumulate), _lib.ptr(out))
         _lib.check(rc, "bpx_event_csrmv_tiled")
         return out
+
+    def matvec(self, x, *, out: torch.Tensor = None) -> torch.Tensor:
+        """``x @ A`` for a dense float32 vector x -- ``bm.sparse.csrmv(data,
+        indices, indptr, x, shape=shape, transpose=True)`` on the plan: the
+        rows with x != 0 stream their segments and every product ``w *
+        x[row]`` (rounded to f32) is summed exactly in fixed point, with no
+        global atomic per synapse.  ``out += x @ A`` when ``out`` is given.
+        Non-finite x gives NaN outputs.  Calls on one plan share a 4-byte
+        device workspace: do not overlap them on different streams."""
+        xt = _lib.to_device(_as_array(x))
+        if xt.dtype != torch.float32:
+            xt = xt.to(torch.float32)
+        xt = xt.contiguous()
+        m, n = self.shape
+        if xt.ndim != 1 or xt.shape[0] != m:
+            raise ValueError(f'Shape mismatch, vec ({xt.shape[0]},) @ mat {self.shape}.')
+        accumulate = out is not None
+        if out is None:
+            out = torch.empty(n, dtype=torch.float32, device=xt.device)
+        elif out.dtype != torch.float32 or out.shape != (n,) or not out.is_contiguous():
+            raise ValueError(f'out must be a contiguous float32 vector of shape ({n},).')
+        if getattr(self, "_ws", None) is None:
+            self._ws = torch.empty(4, dtype=torch.uint8, device=xt.device)
+        rc = _lib.lib().bpx_csrmv_tiled(_lib.stream_ptr(), ctypes.byref(self.plan), _lib.ptr(xt),
+                                        _lib.BPX_F32, int(accumulate), _lib.ptr(out), _lib.ptr(self._ws), 4)
+        _lib.check(rc, "bpx_csrmv_tiled")
+        return out

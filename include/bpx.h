@@ -165,6 +165,17 @@ int bpx_tiled_csr_build(void* stream, const void* data, int data_dtype, int64_t 
 /* out[n] = events @ A (accumulate=0) or out += events @ A (accumulate=1) */
 int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* events, int ev_dtype,
                           int ev_rule, int accumulate, float* out);
+/*
+ * The same plan with a dense f32 vector: out (+)= x @ A, the transpose
+ * `csrmv` of sparse/_csr_mv.py:29-123 (cuSPARSE transpose SpMV) for a
+ * static matrix.  Rows with x != 0 are streamed; each product w * x[row] is
+ * rounded to f32 and summed exactly in fixed point (shift lowered by
+ * ceil(log2 max|x|), computed on the device: workspace >= 4 bytes; calls
+ * that share a workspace must not overlap).  Non-finite x makes every
+ * output NaN.
+ */
+int bpx_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* x, int x_dtype, int accumulate, float* out,
+                    void* workspace, size_t workspace_bytes);
 
 /*
  * Just-in-time connectivity matvec y = M @ v, M never stored.  Replaces the

@@ -230,6 +230,50 @@ def test_tiled_plan_vs_oracle(shape, wkind, sort):
             _tol_check(y, ref, scale)
 
 
+@pytest.mark.parametrize("shape", [(100, 200), (3000, 50), (5000, 100000)])
+@pytest.mark.parametrize("wkind", ["homo", "f32", "bf16"])
+@pytest.mark.parametrize("xscale", [1.0, 3e5, 1e-4])
+def test_tiled_plan_dense_matvec(shape, wkind, xscale):
+    # x @ A for a dense vector with zeros, on the plan, against the FP64
+    # dense product within 1e-5 of the sum of term magnitudes (the products
+    # w * x are rounded to f32 once, as the plain sparse.csrmv does)
+    m, n = shape
+    p = min(0.5, 50.0 / n)
+    indices, indptr = bm_connect_csr(m, n, p, 23, False)
+    rng = np.random.default_rng(8)
+    data = np.float32(0.7) if wkind == "homo" else rng.normal(0.5, 0.3, indices.size).astype(np.float32)
+    plan = bm.event.TiledCSR(data, indices, indptr, shape=shape,
+                             weight_dtype=torch.bfloat16 if wkind == "bf16" else None)
+    x = (rng.standard_normal(m) * xscale).astype(np.float32)
+    x[rng.random(m) < 0.3] = 0.0
+    y = plan.matvec(x).cpu().numpy().astype(np.float64)
+    w = bf16_bits_to_f32(f32_to_bf16_bits(data)) if wkind == "bf16" else data
+    D = dense_of(w, indices, indptr, shape)
+    ref = x.astype(np.float64) @ D
+    mag = np.abs(x.astype(np.float64)) @ np.abs(D)
+    assert np.all(np.abs(y - ref) <= 1e-5 * mag + 1e-30)
+    if wkind == "f32":
+        y2 = bm.sparse.csrmv(data, indices, indptr, x, shape=shape, transpose=True).cpu().numpy()
+        assert np.all(np.abs(y2 - y) <= 2e-5 * mag + 1e-30)
+
+
+def test_tiled_plan_dense_matvec_accumulate_and_nonfinite():
+    m, n = 2000, 3000
+    indices, indptr = bm_connect_csr(m, n, 0.01, 29, True)
+    w = np.random.default_rng(9).normal(0.5, 0.1, indices.size).astype(np.float32)
+    plan = bm.event.TiledCSR(w, indices, indptr, shape=(m, n))
+    x = np.random.default_rng(10).standard_normal(m).astype(np.float32)
+    base = torch.full((n,), 2.0, dtype=torch.float32, device="cuda")
+    plan.matvec(x, out=base)
+    D = dense_of(w, indices, indptr, (m, n))
+    np.testing.assert_allclose(base.cpu().numpy(), 2.0 + x.astype(np.float64) @ D, rtol=1e-5, atol=1e-5)
+    x[7] = np.inf
+    assert torch.isnan(plan.matvec(x)).all()
+    x[7] = np.nan
+    assert torch.isnan(plan.matvec(x)).all()
+    assert not torch.isnan(plan.matvec(np.zeros(m, np.float32))).any()
+
+
 def test_tiled_plan_accumulate_and_float_events():
     m, n = 2000, 7000
     indices, indptr = bm_connect_csr(m, n, 0.01, 21, False)
