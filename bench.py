@@ -640,7 +640,7 @@ def main():
                     help="also run 3 plain device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs on a single MI355X and "
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the library runs on a single MI355X and "
               "has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
 
