@@ -603,12 +603,47 @@ def bench_served(_lib, torch, n_requests=80, rate=50.0, seed=7):
 
     served = run(None)
     fifo = run(1)
+    # the plain per-network FP32 op-by-op path (PlainCOBA, ~14 launches per
+    # step): per-step time measured on 4 requests spread over the size range;
+    # each request's service time = its steps x the per-step time of the
+    # nearest sampled size, served one at a time in arrival order (a FIFO
+    # queue on the same trace) for its throughput and latencies
+    from brainpy_amd.dyn.batched import PlainCOBA
+    order = np.argsort(sizes)
+    pick = order[np.linspace(0, n_requests - 1, 4).astype(int)]
+    sampled = []
+    for k in pick:
+        n_k, seed_k, we, wi, I, _, _ = params[k]
+        ne = int(round(n_k * 0.8))
+        pc = PlainCOBA(dict(n_exc=ne, n_inh=n_k - ne, seed=seed_k, w_exc=we, w_inh=wi, I_ext=I, in_degree=80))
+        pc.run(3)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pc.run(20)
+        torch.cuda.synchronize()
+        sampled.append((n_k, (time.perf_counter() - t0) / 20))
+        del pc
+    ss = np.array([x[0] for x in sampled], np.float64)
+    ts = np.array([x[1] for x in sampled], np.float64)
+    t_free, lat, busy = 0.0, [], 0.0
+    for k in np.argsort(arrivals):
+        svc = float(durs[k]) * float(ts[np.argmin(np.abs(np.log(ss) - np.log(float(sizes[k]))))])
+        start = max(float(arrivals[k]), t_free)
+        t_free = start + svc
+        busy += svc
+        lat.append(t_free - float(arrivals[k]))
+    plain = {"busy_s": busy, "events_per_s_est": served["events"] / busy, "p50_latency_s_est": _percentile(lat, 50),
+             "p99_latency_s_est": _percentile(lat, 99),
+             "sampled_per_step_s": {str(int(a)): float(b) for a, b in sampled}}
     r = {"workload": f"served_coba_poisson{rate:g}_{n_requests}req_4k-1M_100-5000steps_jit80_bf16",
-         "requests": n_requests, "continuous_batching": served, "fifo_queue": fifo,
-         "throughput_ratio": served["events_per_s"] / fifo["events_per_s"] if fifo["events_per_s"] else None}
+         "requests": n_requests, "continuous_batching": served, "fifo_queue": fifo, "plain_fp32_fifo_est": plain,
+         "throughput_ratio": served["events_per_s"] / fifo["events_per_s"] if fifo["events_per_s"] else None,
+         "throughput_vs_plain": served["events_per_s"] / plain["events_per_s_est"]}
     print(f"# served C5 batching {served['events_per_s']:.3e} ev/s p50 {served['p50_latency_s']:.3f} s "
           f"p99 {served['p99_latency_s']:.3f} s | fifo {fifo['events_per_s']:.3e} ev/s p50 "
-          f"{fifo['p50_latency_s']:.3f} s p99 {fifo['p99_latency_s']:.3f} s", file=sys.stderr, flush=True)
+          f"{fifo['p50_latency_s']:.3f} s p99 {fifo['p99_latency_s']:.3f} s | plain FP32 (est.) "
+          f"{plain['events_per_s_est']:.3e} ev/s p50 {plain['p50_latency_s_est']:.1f} s p99 "
+          f"{plain['p99_latency_s_est']:.1f} s -> x{r['throughput_vs_plain']:.0f}", file=sys.stderr, flush=True)
     return r
 
 
