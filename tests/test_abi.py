@@ -139,3 +139,32 @@ int main(){
         subprocess.check_call(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "brainpy_amd", "csrc"), c, "-o", exe])
         out = subprocess.run([exe], capture_output=True, text=True)
         assert out.returncode == 0 and out.stdout.split()[0] == "0", out.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libbpx.so not built")
+def test_argument_validation_of_coo_conn_and_plan_entry_points():
+    # shape, dtype and argument errors come back as status codes before
+    # anything is launched (no GPU needed)
+    from brainpy_amd import _lib
+    L = _lib.lib()
+    SHAPE, DTYPE, ARG, WS = -2, -1, -5, -3
+    F32, I32, F64, BF16 = _lib.BPX_F32, _lib.BPX_I32, _lib.BPX_F64, _lib.BPX_BF16
+    assert L.bpx_coomv(None, None, F32, 1, None, None, I32, 10, None, F32, -1, 5, 0, None) == SHAPE
+    assert L.bpx_coomv(None, None, F32, 3, None, None, I32, 10, None, F32, 4, 5, 0, None) == SHAPE   # data_len
+    assert L.bpx_coomv(None, None, F32, 1, None, None, I32, 10, None, F64, 4, 5, 0, None) == DTYPE   # vector dtype
+    assert L.bpx_coomv(None, None, BF16, 1, None, None, I32, 10, None, BF16, 4, 5, 0, None) == DTYPE
+    assert L.bpx_coo_matmat(None, None, F32, 1, None, None, I32, 10, None, F32, 4, 5, -1, 1, None) == SHAPE
+    assert L.bpx_coo_matmat(None, None, F32, 1, None, None, I32, 10, None, I32, 4, 5, 2, 1, None) == DTYPE
+    assert L.bpx_coo_sddmm(None, None, None, I32, -1, None, 1, 1, 0, None, 1, 1, 0, 1, F32, None) == SHAPE
+    assert L.bpx_coo_probe(None, None, None, None, None, 1, 10, 1, None, 256) == SHAPE         # nse % 4
+    assert L.bpx_coo_probe(None, None, None, None, None, 1, 8, 1, None, 100) == SHAPE          # out_len % 256
+    assert L.bpx_coo_probe(None, None, None, None, None, 1, 8, 3, None, 256) == SHAPE          # mode
+    assert L.bpx_fixed_prob_csr(None, 10, 5, 6, 1, 1, 0, None, None, None, 0) == SHAPE         # k > n
+    assert L.bpx_fixed_prob_csr(None, 10, 20, 5, 1, 0, 0, None, None, None, 0) == SHAPE        # no self, m != n
+    assert L.bpx_fixed_prob_csr(None, 10, 2_000_000, 5, 1, 1, 0, None, None, None, 0) == ARG   # null outputs
+    assert L.bpx_fixed_prob_workspace(7) == 28
+    assert L.bpx_csrmv_tiled(None, None, None, F32, 0, None, None, 0) == ARG
+    plan = _lib.TiledCSRPlan()
+    plan.mem = 1
+    assert L.bpx_csrmv_tiled(None, ctypes.byref(plan), None, F64, 0, None, None, 0) == DTYPE
+    assert L.bpx_csrmv_tiled(None, ctypes.byref(plan), None, F32, 0, None, None, 0) == WS
