@@ -14,6 +14,8 @@
 //   shuffle reduction inside the group, one store per row.
 #include "bpx_common.h"
 
+#include <stdlib.h>
+
 namespace bpx {
 
 // ---------------------------------------------------------------------------
@@ -147,6 +149,76 @@ __global__ __launch_bounds__(256) void k_csr_gather(const V* __restrict__ data,
 }
 
 // ---------------------------------------------------------------------------
+// event gather with the events as a bitmask in LDS (bool events, n bits that
+// fit the block's LDS): each block packs the whole event vector once (two
+// 16-byte loads per 32 events, bytes tested four at a time) and walks rows
+// grid-stride, so a synapse costs its index and weight loads plus an LDS bit
+// test instead of a gathered global read.  Rows sum in the same lane order
+// as k_csr_gather.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t nz_flags4(uint32_t w) {
+  // bit 7 of each byte set iff the byte is non-zero, then gathered into 4 bits
+  const uint32_t t = ((((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w) & 0x80808080u) >> 7;
+  return ((t * 0x00204081u) >> 21) & 0xFu;
+}
+
+template <int G, int U, bool HOMO, typename I, typename V>
+__global__ __launch_bounds__(256) void k_csr_gather_bits(const V* __restrict__ data,
+                                                         const I* __restrict__ indices,
+                                                         const I* __restrict__ indptr, int64_t nnz,
+                                                         const uint8_t* __restrict__ ev, int64_t m, int64_t n,
+                                                         typename acc_of<V>::type* __restrict__ out) {
+  using A = typename acc_of<V>::type;
+  extern __shared__ uint32_t s_ev[];
+  const int64_t nw = (n + 31) >> 5;
+  for (int64_t w = threadIdx.x; w < nw; w += 256) {
+    const int64_t c0 = w << 5;
+    uint32_t bits = 0;
+    if (c0 + 32 <= n) {
+      const uint4 a = *reinterpret_cast<const uint4*>(ev + c0);
+      const uint4 b = *reinterpret_cast<const uint4*>(ev + c0 + 16);
+      const uint32_t q[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int k = 0; k < 8; ++k) bits |= nz_flags4(q[k]) << (4 * k);
+    } else {
+      for (int64_t c = c0; c < n; ++c) bits |= (ev[c] != 0 ? 1u : 0u) << (c - c0);
+    }
+    s_ev[w] = bits;
+  }
+  __syncthreads();
+  const A homo_w = HOMO ? (A)to_f32(data[0]) : A(0);
+  constexpr int NG = 256 / G;
+  const int lane = (int)threadIdx.x % G;
+  const int64_t groups = (m + NG - 1) / NG;
+  for (int64_t gi = blockIdx.x; gi < groups; gi += gridDim.x) {
+    const int64_t row = gi * NG + (int)threadIdx.x / G;
+    A acc = A(0);
+    if (row < m) {
+      int64_t s = (int64_t)indptr[row];
+      int64_t t = (int64_t)indptr[row + 1];
+      s = s < 0 ? 0 : s;
+      t = t > nnz ? nnz : t;
+      for (int64_t base = s; base < t; base += (int64_t)G * U) {
+        int64_t idx[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int64_t j = base + lane + (int64_t)u * G;
+          idx[u] = j < t ? (int64_t)indices[j] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int64_t j = base + lane + (int64_t)u * G;
+          if ((uint64_t)idx[u] < (uint64_t)n && ((s_ev[idx[u] >> 5] >> (idx[u] & 31)) & 1u))
+            acc += HOMO ? homo_w : (A)to_f32(data[j]);
+        }
+      }
+    }
+    acc = group_sum<G>(acc);
+    if (row < m && lane == 0) out[row] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // fixed order: one thread per post-synaptic column, ascending CSC rows
 // ---------------------------------------------------------------------------
 template <bool HOMO, typename V, typename E>
@@ -173,6 +245,20 @@ __global__ __launch_bounds__(256) void k_fixed_order(const V* __restrict__ data,
 // ---------------------------------------------------------------------------
 // host dispatch
 // ---------------------------------------------------------------------------
+// blocks of the bitmask gather: 8 per CU (each packs the event vector once)
+static int64_t gather_bits_grid() {
+  static int64_t g = [] {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    const char* e = getenv("BPX_GATHER_BLOCKS_PER_CU");
+    const int per = e ? atoi(e) : 8;
+    return (int64_t)cus * (per > 0 ? per : 8);
+  }();
+  return g;
+}
+
 static int pick_group(int64_t nnz, int64_t m) {
   const double avg = m > 0 ? (double)nnz / (double)m : 0.0;
   if (avg <= 48.0) return 16;
@@ -195,7 +281,16 @@ static int launch_csr(hipStream_t s, int transpose, const void* data, const void
   } else {
     constexpr int NG = 256 / G;
     const int64_t blocks = (m + NG - 1) / NG;
-    if (blocks > 0) {
+    const int64_t nw = (n + 31) / 32;
+    // bool events whose bitmask fits 64 KB of LDS (n <= 524,288), 16-byte aligned
+    const bool bits = EVENT && sizeof(E) == 1 && nw * 4 <= 65536 && ((uintptr_t)vec & 15u) == 0 && n > 0;
+    if (blocks > 0 && bits) {
+      int64_t grid = gather_bits_grid();
+      if (grid > blocks) grid = blocks;
+      BPX_LAUNCH(KID_EVENT_GATHER, (k_csr_gather_bits<G, U, HOMO, I, V>), dim3((unsigned)grid), dim3(256),
+                 (size_t)nw * 4, s, (const V*)data, (const I*)indices, (const I*)indptr, nnz,
+                 (const uint8_t*)vec, m, n, (A*)out);
+    } else if (blocks > 0) {
       BPX_LAUNCH(EVENT ? KID_EVENT_GATHER : KID_CSR_GATHER, (k_csr_gather<G, U, EVENT, HOMO, I, V, E>),
                  dim3((unsigned)blocks), dim3(256), 0, s, (const V*)data, (const I*)indices,
                  (const I*)indptr, nnz, (const E*)vec, rule, m, n, (A*)out);
