@@ -386,6 +386,60 @@ def bench_conn(_lib, torch, reps=5):
     return {"workload": "fixed_prob_build_csr_100kx100k_p0.01", **res}
 
 
+def bench_stdp(_lib, torch, steps=30, warmup=3):
+    """STDP weight updates on the headline matrix (100,000 x 100,000 CSR,
+    p = 0.01, f32 weights, int32 indices), 10 % Bernoulli spikes, in place:
+    csr_on_pre_update (rows of spiking pre neurons; per synapse 4 B index,
+    4 B weight read and write, a trace gather) and csc_on_post_update
+    (columns of spiking post neurons through the CSC view: 4 B pre id and
+    4 B weight id per synapse, the weight read and written at scattered
+    positions)."""
+    from brainpy_amd.connect import fixed_prob_csr_device
+    from brainpy_amd import dnn
+    dev = _lib.device()
+    n = 100_000
+    idx, ptr = fixed_prob_csr_device(n, n, 0.01, seed=2024, device=dev)
+    nnz = idx.numel()
+    g = torch.Generator(device=dev)
+    g.manual_seed(21)
+    w = torch.empty(nnz, dtype=torch.float32, device=dev).uniform_(0.2, 0.8, generator=g)
+    trace = torch.empty(n, dtype=torch.float32, device=dev).normal_(0.0, 1e-3, generator=g)
+    spk = torch.rand(n, generator=g, device=dev) < 0.1
+    rows = torch.repeat_interleave(torch.arange(n, device=dev, dtype=torch.int32), ptr[1:] - ptr[:-1])
+    order = torch.argsort(idx, stable=True)
+    pre_ids = rows[order].contiguous()
+    w_ids = order.to(torch.int32)
+    colptr = torch.zeros(n + 1, dtype=torch.int32, device=dev)
+    colptr[1:] = torch.cumsum(torch.bincount(idx.long(), minlength=n), 0).to(torch.int32)
+    del rows, order
+    act = int(spk.sum())
+    syn_rows = int((ptr[1:] - ptr[:-1])[spk].sum())
+    syn_cols = int((colptr[1:] - colptr[:-1])[spk].sum())
+    res = []
+    for name, call, per, syn in (
+            ("csr_on_pre_update", lambda: dnn.csr_on_pre_update(w, idx, ptr, spk, trace, 0.0, 1.0, out=w), 12,
+             syn_rows),
+            ("csc_on_post_update", lambda: dnn.csc_on_post_update(w, pre_ids, colptr, w_ids, spk, trace, 0.0, 1.0,
+                                                                  out=w), 16, syn_cols)):
+        for _ in range(warmup):
+            call()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            call()
+        e1.record()
+        e1.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / steps
+        alg = syn * per + n + (n + 1) * 4
+        r = {"op": name, "active": act, "synapses": syn, "us_per_call": us, "alg_bytes": alg,
+             "gbs": alg / us / 1e3, "frac": alg / us / 1e3 / HBM_PEAK_GBS, "synapses_per_s": syn / us * 1e6}
+        res.append(r)
+        print(f"# stdp {name:20s} {us:8.1f} us/call {r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%) "
+              f"{r['synapses_per_s']:.3e} synapses/s", file=sys.stderr, flush=True)
+    return {"workload": "stdp_100kx100k_p0.01_rate0.10_f32_in_place", "results": res}
+
+
 def _percentile(x, q):
     return float(np.percentile(np.asarray(x, np.float64), q)) if len(x) else None
 
@@ -669,7 +723,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the jitconn and network legs")
-    ap.add_argument("--only", choices=["jitconn", "batched", "coba", "served", "coo", "forms", "conn"], default=None,
+    ap.add_argument("--only", choices=["jitconn", "batched", "coba", "served", "coo", "forms", "conn", "stdp"],
+                    default=None,
                     help="run only this leg (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run 3 plain device copies of this many MiB (HBM counter calibration)")
@@ -690,7 +745,7 @@ def main():
               "batched": lambda: bench_batched(_lib, torch, plain_sample=2, plain_steps=20),
               "coba": lambda: bench_coba(_lib, torch), "served": lambda: bench_served(_lib, torch),
               "coo": lambda: bench_coo(_lib, torch), "forms": lambda: bench_forms(_lib, torch),
-              "conn": lambda: bench_conn(_lib, torch)}[args.only]
+              "conn": lambda: bench_conn(_lib, torch), "stdp": lambda: bench_stdp(_lib, torch)}[args.only]
         print(json.dumps({"only": args.only, "result": fn()}), flush=True)
         return
     stream = torch.cuda.current_stream()
@@ -885,6 +940,7 @@ def main():
         extra["coo"] = bench_coo(_lib, torch)
         extra["forms"] = bench_forms(_lib, torch)
         extra["conn"] = bench_conn(_lib, torch)
+        extra["stdp"] = bench_stdp(_lib, torch)
 
     # CPU baseline: the oracle's serial scatter (the reference CPU loop
     # order, event/_csr_matvec.py:321-340), one thread, on a bounded number

@@ -18,10 +18,12 @@
 // gathers).  The update runs in place on `out` after `w` is copied into it
 // when the two differ.
 #include "bpx_common.h"
+#include <stdlib.h>
 
 namespace bpx {
 
 constexpr int kPlastThreads = 256;
+constexpr int kPlastU = 8;   // entries per lane per step of a row walk
 
 template <typename W>
 __device__ __forceinline__ W clamp_mm(W x, W lo, W hi) {
@@ -40,13 +42,15 @@ __global__ __launch_bounds__(kPlastThreads) void k_sparse_on_spike(const I* __re
                                                                    const E* __restrict__ spike, int rule,
                                                                    int64_t nrows, const W* __restrict__ trace,
                                                                    int64_t ntrace, W w_min, W w_max,
-                                                                   W* __restrict__ out, int64_t nnz) {
+                                                                   W* __restrict__ out, int64_t nnz, int rw) {
+  // a wave owns chunks of rw (1..64) consecutive rows; lanes below rw read
+  // their row's spike flag and the wave walks the active rows one by one
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * kPlastThreads + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * kPlastThreads) >> 6;
-  for (int64_t base = wave * 64; base < nrows; base += nwaves * 64) {
+  for (int64_t base = wave * rw; base < nrows; base += nwaves * rw) {
     const int64_t r = base + lane;
-    const bool act = r < nrows && ev_true(spike[r], rule);
+    const bool act = lane < rw && r < nrows && ev_true(spike[r], rule);
     uint64_t mask = __ballot(act);
     while (mask) {
       const int b = __builtin_ctzll(mask);
@@ -55,10 +59,29 @@ __global__ __launch_bounds__(kPlastThreads) void k_sparse_on_spike(const I* __re
       int64_t s = (int64_t)ptr[row], t = (int64_t)ptr[row + 1];
       s = s < 0 ? 0 : s;
       t = t > nnz ? nnz : t;
-      for (int64_t k = s + lane; k < t; k += 64) {
-        const int64_t j = (int64_t)other[k];
-        const int64_t l = wid ? (int64_t)wid[k] : k;
-        if ((uint64_t)j < (uint64_t)ntrace && (uint64_t)l < (uint64_t)nnz) out[l] = clamp_mm(out[l] + trace[j], w_min, w_max);
+      // kPlastU entries per lane per step, all loads of a step issued before
+      // any of their uses (a row of 1000 synapses takes two steps)
+      for (int64_t k0 = s + lane; k0 < t; k0 += 64 * kPlastU) {
+        int64_t j[kPlastU], l[kPlastU];
+        bool ok[kPlastU];
+#pragma unroll
+        for (int u = 0; u < kPlastU; ++u) {
+          const int64_t k = k0 + 64 * u;
+          ok[u] = k < t;
+          const int64_t kc = ok[u] ? k : s;
+          j[u] = (int64_t)other[kc];
+          l[u] = wid ? (int64_t)wid[kc] : kc;
+        }
+        W wv[kPlastU], tv[kPlastU];
+#pragma unroll
+        for (int u = 0; u < kPlastU; ++u) {
+          ok[u] = ok[u] && (uint64_t)j[u] < (uint64_t)ntrace && (uint64_t)l[u] < (uint64_t)nnz;
+          wv[u] = ok[u] ? out[l[u]] : W(0);
+          tv[u] = ok[u] ? trace[j[u]] : W(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kPlastU; ++u)
+          if (ok[u]) out[l[u]] = clamp_mm(wv[u] + tv[u], w_min, w_max);
       }
     }
   }
@@ -99,8 +122,23 @@ static void copy_weights(hipStream_t s, const void* w, void* out, int64_t n) {
   hipLaunchKernelGGL(k_copy_t<W>, dim3((unsigned)b), dim3(256), 0, s, (const W*)w, (W*)out, n);
 }
 
-static unsigned plast_blocks(int64_t nrows) {
-  int64_t waves = (nrows + 63) / 64;
+// rows per wave chunk: the largest power of two (at most 64) that still gives
+// about 16384 waves, so that enough rows' dependent index -> weight chains
+// are in flight (BPX_PLAST_ROWS_PER_WAVE overrides, 1..64)
+static int plast_rows_per_wave(int64_t nrows) {
+  static const int forced = [] {
+    const char* e = getenv("BPX_PLAST_ROWS_PER_WAVE");
+    const int v = e ? atoi(e) : 0;
+    return v >= 1 && v <= 64 ? v : 0;
+  }();
+  if (forced) return forced;
+  int rw = 64;
+  while (rw > 1 && nrows / rw < 16384) rw >>= 1;
+  return rw;
+}
+
+static unsigned plast_blocks(int64_t nrows, int rw) {
+  int64_t waves = (nrows + rw - 1) / rw;
   int64_t b = (waves + kPlastThreads / 64 - 1) / (kPlastThreads / 64);
   if (b > 4096) b = 4096;
   return (unsigned)(b < 1 ? 1 : b);
@@ -112,11 +150,12 @@ static int sparse_on_spike(hipStream_t s, const void* w, const void* other, cons
                            int64_t ntrace, double w_min, double w_max, void* out) {
   copy_weights<W>(s, w, out, nnz);
   if (nrows == 0 || nnz == 0) return BPX_OK;
-  const unsigned blocks = plast_blocks(nrows);
+  const int rw = plast_rows_per_wave(nrows);
+  const unsigned blocks = plast_blocks(nrows, rw);
 #define LAUNCH_E(E)                                                                                            \
   hipLaunchKernelGGL((k_sparse_on_spike<I, E, W>), dim3(blocks), dim3(kPlastThreads), 0, s, (const I*)other,    \
                      (const I*)ptr, (const I*)wid, (const E*)spike, BPX_RULE_NONZERO, nrows, (const W*)trace,   \
-                     ntrace, (W)w_min, (W)w_max, (W*)out, nnz)
+                     ntrace, (W)w_min, (W)w_max, (W*)out, nnz, rw)
   if (sp_dtype == BPX_BOOL) LAUNCH_E(uint8_t);
   else if (sp_dtype == BPX_F32) LAUNCH_E(float);
   else if (sp_dtype == BPX_F64) LAUNCH_E(double);
