@@ -28,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_MFMA_TFLOPS = 157.3   # dense f32 matrix-core peak
 M = N = 100_000
 PROB = 0.01
 K_ROW = int(N * PROB)
@@ -36,15 +37,22 @@ HEADLINE = ("heter_f32", 0.10)
 TILES = 32   # output tiles per copy on MI355X (CUs per XCD); set from the plan at run time
 
 
-def algorithmic_bytes(active_rows, kind, path="csr"):
-    """Bytes the scatter must move once: events (1 B each), the row offsets
-    of each active row, column indices and weights of active rows, one write
-    of the output.  The plain CSR path reads 4-byte column indices and one
-    offset pair per active row; the tiled layout reads 2-byte tile-local
-    indices and one offset pair per active row and tile."""
+def algorithmic_bytes(active_rows, kind):
+    """Algorithmic bytes of one event csrmv transpose call, as the operator
+    receives its inputs (whatever the kernel's own layout): events (1 B
+    each), one 8-byte offset pair per active row, the 4-byte column indices
+    and the weights of active rows, one 4-byte write of each output."""
     w_bytes = {"homo": 0, "heter_f32": 4, "heter_bf16": 2}[kind]
+    return M * 1 + active_rows * 8 + active_rows * K_ROW * (4 + w_bytes) + N * 4
+
+
+def layout_bytes(active_rows, kind, path="csr"):
+    """Bytes the kernel itself must read and write once in its own layout:
+    the plain CSR path as above; the tiled plan reads 2-byte tile-local
+    indices and one offset pair per active row and tile."""
     if path == "csr":
-        return M * 1 + active_rows * 8 + active_rows * K_ROW * (4 + w_bytes) + N * 4
+        return algorithmic_bytes(active_rows, kind)
+    w_bytes = {"homo": 0, "heter_f32": 4, "heter_bf16": 2}[kind]
     return M * 1 + active_rows * 8 * TILES + active_rows * K_ROW * (2 + w_bytes) + N * 4
 
 
@@ -386,6 +394,66 @@ def bench_conn(_lib, torch, reps=5):
     return {"workload": "fixed_prob_build_csr_100kx100k_p0.01", **res}
 
 
+def bench_bsr(_lib, torch, steps=30, warmup=3):
+    """Block-sparse products.  cusparse_bcsr_matvec: 100,000 x 100,000 as
+    6250 x 6250 blocks of 16 x 16, 62 blocks per block row (FixedProb over
+    blocks, p = 0.01), f32, int32 indices, both orientations; algorithmic
+    bytes = the block values (1 KiB per block), block indices and pointers,
+    one read of x and one write of y.  bcsrmm: A (4096 x 4096) @ B (4096 x
+    4096) with 12 of the 128 32 x 32 blocks of each column block present,
+    f32 on MFMA; work = 2 * m * 1024 flops per block."""
+    from brainpy_amd.connect import fixed_prob_csr_device
+    dev = _lib.device()
+    L = _lib.lib()
+    sp = _lib.stream_ptr()
+    kid = _lib.KID["bsr"]
+    res = []
+    bs, mb = 16, 6250
+    idx, ptr = fixed_prob_csr_device(mb, mb, 0.01, seed=7, device=dev)
+    nnzb = idx.numel()
+    g = torch.Generator(device=dev)
+    g.manual_seed(9)
+    data = torch.empty(nnzb * bs * bs, dtype=torch.float32, device=dev).normal_(0.0, 1.0, generator=g)
+    x = torch.rand(mb * bs, generator=g, device=dev)
+    y = torch.empty(mb * bs, dtype=torch.float32, device=dev)
+    for transpose in (0, 1):
+        def call():
+            rc = L.bpx_bsrmv(sp, _lib.ptr(data), _lib.BPX_F32, _lib.ptr(idx), _lib.ptr(ptr), _lib.BPX_I32, nnzb, bs,
+                             _lib.ptr(x), mb, mb, transpose, _lib.ptr(y))
+            if rc != 0:
+                raise RuntimeError(f"bpx_bsrmv failed ({rc})")
+        wall, kern = _time_calls(_lib, torch, call, kid, steps, warmup)
+        alg = nnzb * (bs * bs * 4 + 4) + (mb + 1) * 4 + 2 * mb * bs * 4
+        r = {"op": "bsrmv", "transpose": bool(transpose), "blocksize": bs, "nnzb": nnzb, "us_per_call": wall * 1e6,
+             "kernel_us": kern * 1e6, "alg_bytes": alg, "gbs": alg / kern / 1e9,
+             "frac": alg / kern / 1e9 / HBM_PEAK_GBS}
+        res.append(r)
+        print(f"# bsrmv transpose={transpose} bs={bs} nnzb={nnzb} {r['us_per_call']:8.1f} us/call kernel "
+              f"{r['kernel_us']:8.1f} us {r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%)", file=sys.stderr, flush=True)
+    del data, x, y, idx, ptr
+    m = k = n = 4096
+    cidx, cptr = fixed_prob_csr_device(n // 32, k // 32, 0.1, seed=11, device=dev)
+    nblk = cidx.numel()
+    bidx = torch.stack([cidx, torch.arange(nblk, device=dev, dtype=torch.int32)], 1).contiguous()
+    bdata = torch.empty(nblk * 1024, dtype=torch.float32, device=dev).normal_(0.0, 1.0, generator=g)
+    A = torch.empty(m * k, dtype=torch.float32, device=dev).normal_(0.0, 1.0, generator=g)
+    C = torch.empty(m * n, dtype=torch.float32, device=dev)
+
+    def mm():
+        rc = L.bpx_bcsrmm(sp, _lib.ptr(A), _lib.BPX_F32, _lib.ptr(bdata), _lib.ptr(bidx), _lib.ptr(cptr),
+                          _lib.BPX_I32, nblk, m, k, n, _lib.ptr(C))
+        if rc != 0:
+            raise RuntimeError(f"bpx_bcsrmm failed ({rc})")
+    wall, kern = _time_calls(_lib, torch, mm, kid, steps, warmup)
+    flops = 2.0 * m * nblk * 1024
+    r = {"op": "bcsrmm", "m": m, "k": k, "n": n, "blocks": nblk, "us_per_call": wall * 1e6, "kernel_us": kern * 1e6,
+         "flops": flops, "tflops": flops / kern / 1e12, "frac": flops / kern / 1e12 / FP32_MFMA_TFLOPS}
+    res.append(r)
+    print(f"# bcsrmm {m}x{k} @ {k}x{n} ({nblk} blocks) {r['us_per_call']:8.1f} us/call kernel {r['kernel_us']:8.1f} us "
+          f"{r['tflops']:6.1f} TFLOP/s ({100 * r['frac']:.1f}% of f32 MFMA)", file=sys.stderr, flush=True)
+    return {"workload": "bsrmv_100kx100k_bs16_p0.01_f32 + bcsrmm_4096_f32_10pct_blocks", "results": res}
+
+
 def bench_stdp(_lib, torch, steps=30, warmup=3):
     """STDP weight updates on the headline matrix (100,000 x 100,000 CSR,
     p = 0.01, f32 weights, int32 indices), 10 % Bernoulli spikes, in place:
@@ -723,7 +791,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the jitconn and network legs")
-    ap.add_argument("--only", choices=["jitconn", "batched", "coba", "served", "coo", "forms", "conn", "stdp"],
+    ap.add_argument("--only", choices=["jitconn", "batched", "coba", "served", "coo", "forms", "conn", "stdp", "bsr"],
                     default=None,
                     help="run only this leg (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -745,7 +813,8 @@ def main():
               "batched": lambda: bench_batched(_lib, torch, plain_sample=2, plain_steps=20),
               "coba": lambda: bench_coba(_lib, torch), "served": lambda: bench_served(_lib, torch),
               "coo": lambda: bench_coo(_lib, torch), "forms": lambda: bench_forms(_lib, torch),
-              "conn": lambda: bench_conn(_lib, torch), "stdp": lambda: bench_stdp(_lib, torch)}[args.only]
+              "conn": lambda: bench_conn(_lib, torch), "stdp": lambda: bench_stdp(_lib, torch),
+              "bsr": lambda: bench_bsr(_lib, torch)}[args.only]
         print(json.dumps({"only": args.only, "result": fn()}), flush=True)
         return
     stream = torch.cuda.current_stream()
@@ -814,12 +883,13 @@ def main():
         _lib.timer_enable(False)
         ms, launches = _lib.timer_read(kid)
         kern_s = ms / 1e3 / max(launches, 1)
-        abytes = algorithmic_bytes(active, kind, path)
+        abytes = algorithmic_bytes(active, kind)
+        lbytes = layout_bytes(active, kind, path)
         return {
             "path": path, "kind": kind, "rate": r, "active_rows": active, "events_per_call": events,
             "ms_per_step": wall / steps * 1e3, "events_per_s": events * steps / wall,
             "kernel_us": kern_s * 1e6, "kernel_launches": launches, "algorithmic_bytes": abytes,
-            "kernel_GBs": abytes / kern_s / 1e9,
+            "kernel_GBs": abytes / kern_s / 1e9, "layout_bytes": lbytes, "layout_GBs": lbytes / kern_s / 1e9,
         }
 
     global TILES
@@ -941,6 +1011,7 @@ def main():
         extra["forms"] = bench_forms(_lib, torch)
         extra["conn"] = bench_conn(_lib, torch)
         extra["stdp"] = bench_stdp(_lib, torch)
+        extra["bsr"] = bench_bsr(_lib, torch)
 
     # CPU baseline: the oracle's serial scatter (the reference CPU loop
     # order, event/_csr_matvec.py:321-340), one thread, on a bounded number
@@ -993,7 +1064,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": head["kernel_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": head["kernel_GBs"] / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_tiled_scatter (heter f32, bool events)", "kernel_us": head["kernel_us"],
-                     "algorithmic_bytes_per_launch": head["algorithmic_bytes"]},
+                     "algorithmic_bytes_per_launch": head["algorithmic_bytes"],
+                     "layout_bytes_per_launch": head["layout_bytes"],
+                     "layout_frac": head["layout_GBs"] / HBM_PEAK_GBS},
         "current_rel_err_vs_f64": dict(f64_err, against="FP64 sum of the FP32 weights, outputs with |y| > 1"),
         "graph_amortised": graph,
         "high_rate": high,

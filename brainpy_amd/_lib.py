@@ -68,6 +68,9 @@ SIGNATURES = {
     "bpx_coo_sddmm": ([_vp, _vp, _vp, _i32, _i64, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _i32, _vp],
                       _i32),
     "bpx_coo_probe": ([_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i32, _vp, _i64], _i32),
+    "bpx_bsrmv": ([_vp, _vp, _i32, _vp, _vp, _i32, _i64, _i64, _vp, _i64, _i64, _i32, _vp], _i32),
+    "bpx_bsr_sddmm": ([_vp, _vp, _vp, _i32, _i64, _i64, _vp, _vp, _i32, _i64, _i64, _vp], _i32),
+    "bpx_bcsrmm": ([_vp, _vp, _i32, _vp, _vp, _vp, _i32, _i64, _i64, _i64, _i64, _vp], _i32),
     "bpx_fixed_prob_workspace": ([_i64], _sz),
     "bpx_fixed_prob_csr": ([_vp, _i64, _i64, _i64, ctypes.c_uint64, _i32, _i32, _vp, _vp, _vp, _sz], _i32),
     "bpx_tiled_csr_layout": ([_i64, _i64, _i64, _i32, ctypes.POINTER(TiledCSRPlan), ctypes.POINTER(_sz)], _i32),
@@ -171,4 +174,4 @@ def timer_read(kernel_id):
 
 
 KID = dict(event_scatter=0, event_gather=1, csr_scatter=2, csr_gather=3, jit_outdim=4, jit_scatter=5, info=6,
-           lif=7, fixed_order=8, copy=9, net=10, tiled=11, coo=12)
+           lif=7, fixed_order=8, copy=9, net=10, tiled=11, coo=12, bsr=13)

@@ -93,6 +93,7 @@ enum KernelId {
   KID_NET = 10,
   KID_TILED = 11,
   KID_COO = 12,
+  KID_BSR = 13,
   KID_COUNT = 16
 };
 

@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libbpx.so")
-SOURCES = ["bpx_runtime.hip", "bpx_csr.hip", "bpx_info.hip", "bpx_jitconn.hip", "bpx_neuron.hip", "bpx_tiled.hip", "bpx_net.hip", "bpx_plast.hip", "bpx_coo.hip", "bpx_conn.hip"]
+SOURCES = ["bpx_runtime.hip", "bpx_csr.hip", "bpx_info.hip", "bpx_jitconn.hip", "bpx_neuron.hip", "bpx_tiled.hip", "bpx_net.hip", "bpx_plast.hip", "bpx_coo.hip", "bpx_conn.hip", "bpx_bsr.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -1,11 +1,13 @@
 """Sparse CSR matvec with a dense vector (``bm.sparse.csrmv``,
 ``bm.sparse.csrmv_taichi``), COO products (``bm.sparse.coomv``,
-``bm.sparse.seg_matmul``) and the CSR/COO/dense converters on the gfx950
-library.
+``bm.sparse.seg_matmul``), block-sparse products (``cusparse_bcsr_matvec``
+and ``bcsrmm`` of ``brainpy._src.math.sparse``) and the CSR/COO/dense
+converters on the gfx950 library.
 
 References: brainpy/_src/math/sparse/_csr_mv.py:29-123,
 sparse/_csr_mv_taichi.py:188-264, sparse/_coo_mv.py:17-196,
-sparse/_jax_prim.py:15-166, sparse/_utils.py:20-120.
+sparse/_jax_prim.py:15-166, sparse/_utils.py:20-120, sparse/_bsr_mv.py:24-210,
+sparse/_bsr_mm.py:210-307.
 """
 from typing import Tuple
 
@@ -15,7 +17,8 @@ import torch
 from .. import _lib
 from ._common import acc_dtype, batch_rows, data_tensor, index_tensor, ndim, needs_ad
 
-__all__ = ["csrmv", "csrmv_taichi", "coomv", "seg_matmul", "csr_to_dense", "csr_to_coo", "coo_to_csr"]
+__all__ = ["csrmv", "csrmv_taichi", "coomv", "seg_matmul", "csr_to_dense", "csr_to_coo", "coo_to_csr",
+           "cusparse_bcsr_matvec", "bcsrmm"]
 
 _METHODS = ("cusparse", "scalar", "vector", "adaptive")
 
@@ -412,3 +415,177 @@ def seg_matmul(A, B):
                          f'A:\n{A}\n'
                          f'B:\n{B}')
     return _seg_matmul(B, A, False)
+
+
+# ---------------------------------------------------------------------------
+# block-sparse (BSR) products
+# ---------------------------------------------------------------------------
+_BSR_METHODS = ("scalar", "vector", "adaptive")
+
+
+def _bsr_launch(d, idx, ptr, v, bs, nnzb, shape, transpose):
+    d, idx, ptr, v = d.contiguous(), idx.contiguous(), ptr.contiguous(), v.contiguous()
+    mb, nb = int(shape[0]), int(shape[1])
+    out = torch.empty((nb if transpose else mb) * bs, dtype=d.dtype, device=v.device)
+    rc = _lib.lib().bpx_bsrmv(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), _lib.ptr(idx), _lib.ptr(ptr),
+                              _lib.bpx_dtype(idx), nnzb, bs, _lib.ptr(v), mb, nb, int(bool(transpose)), _lib.ptr(out))
+    _lib.check(rc, "bpx_bsrmv")
+    return out
+
+
+def _bsr_sddmm(idx, ptr, bs, nnzb, rowv, colv, shape):
+    """out[j][p][q] = rowv[i*bs + p] * colv[indices[j]*bs + q], i = block row of block j."""
+    idx, ptr, rowv, colv = idx.contiguous(), ptr.contiguous(), rowv.contiguous(), colv.contiguous()
+    out = torch.empty(nnzb * bs * bs, dtype=rowv.dtype, device=rowv.device)
+    rc = _lib.lib().bpx_bsr_sddmm(_lib.stream_ptr(), _lib.ptr(idx), _lib.ptr(ptr), _lib.bpx_dtype(idx), nnzb, bs,
+                                  _lib.ptr(rowv), _lib.ptr(colv), _lib.bpx_dtype(rowv), int(shape[0]), int(shape[1]),
+                                  _lib.ptr(out))
+    _lib.check(rc, "bpx_bsr_sddmm")
+    return out
+
+
+class _BSRMV(torch.autograd.Function):
+    """Differentiation rules of sparse/_bsr_mv.py:170-199: tangents of the
+    values pass through the same operator; the vector's cotangent is the
+    product with `transpose` flipped; the values' cotangent of block j in
+    block row i is ct[i*bs + p] * vector[indices[j]*bs + q] (transposed:
+    vector[i*bs + p] * ct[indices[j]*bs + q]), as the reference's rule
+    writes it."""
+
+    @staticmethod
+    def forward(d, v, idx, ptr, bs, nnzb, shape, transpose):
+        return _bsr_launch(d.reshape(-1), idx, ptr, v, bs, nnzb, shape, transpose)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        d, v, idx, ptr, bs, nnzb, shape, transpose = inputs
+        ctx.save_for_backward(d, v, idx, ptr)
+        ctx.bs, ctx.nnzb, ctx.shape, ctx.transpose = bs, nnzb, shape, transpose
+
+    @staticmethod
+    def backward(ctx, ct):
+        d, v, idx, ptr = ctx.saved_tensors
+        gd = gv = None
+        ct = ct.to(d.dtype)
+        if ctx.needs_input_grad[0]:
+            rowv, colv = (v.to(d.dtype), ct) if ctx.transpose else (ct, v.to(d.dtype))
+            gd = _bsr_sddmm(idx, ptr, ctx.bs, ctx.nnzb, rowv, colv, ctx.shape).reshape(d.shape)
+        if ctx.needs_input_grad[1]:
+            gv = _BSRMV.apply(d, ct, idx, ptr, ctx.bs, ctx.nnzb, ctx.shape, not ctx.transpose).to(v.dtype)
+        return gd, gv, None, None, None, None, None, None
+
+    @staticmethod
+    def vmap(info, in_dims, d, v, idx, ptr, bs, nnzb, shape, transpose):
+        return batch_rows(info, in_dims, (d, v, idx, ptr, bs, nnzb, shape, transpose), _BSRMV.apply)
+
+
+def cusparse_bcsr_matvec(data, indices, indptr, vector, *, blocksize: int, nnzb: int, shape: Tuple[int, int],
+                         method: str = 'vector', transpose: bool = False) -> torch.Tensor:
+    """Product of a block-sparse (BSR) matrix and a dense vector.
+
+    The matrix has ``shape = (mb, nb)`` blocks of ``blocksize x blocksize``;
+    ``data`` holds the ``nnzb`` blocks row-major (any shape with
+    ``nnzb * blocksize**2`` elements: ``(nnzb * bs, bs)`` as the reference's
+    benchmark builds it, or ``(nnzb, bs, bs)``), ``indices`` the block
+    columns and ``indptr`` the ``mb + 1`` block-row pointers.  Returns
+    ``A @ vector`` (``mb * bs`` entries), or ``A.T @ vector`` (``nb * bs``
+    entries) with ``transpose`` -- the form the reference's transpose rule
+    calls; its own lowerings take no transpose.  The GPU lowering of the
+    reference is cuSPARSE bsrmv, whose block-j semantics this follows; its
+    Numba CPU loop slices block ``indices[j]`` instead of block ``j``
+    (sparse/_bsr_mv.py:28-33), which agrees only where the two are equal.
+    ``method`` is checked and otherwise ignored.  Data and vector share one
+    dtype, float32 or float64.  Reference: sparse/_bsr_mv.py:60-91.
+    """
+    data = _as_array(data)
+    if ndim(data) == 0:
+        data = data.reshape(1)
+    vector = _as_array(vector)
+    if method not in _BSR_METHODS:
+        raise ValueError('Only support methods: scalar, vector, and adaptive. '
+                         f'But we got {method}.')
+    d = _lib.to_device(data)
+    if d.ndim == 0:
+        d = d.reshape(1)
+    v = _lib.to_device(vector)
+    if d.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f'Only support float32 and float64. But we got {d.dtype}.')
+    if d.dtype != v.dtype:
+        raise TypeError('The types of data and vector should be the same. '
+                        f'But we got {d.dtype} != {v.dtype}.')
+    bs, nnzb = int(blocksize), int(nnzb)
+    mb, nb = int(shape[0]), int(shape[1])
+    if bs < 1:
+        raise ValueError(f'blocksize must be positive, got {blocksize}.')
+    idx = index_tensor(_as_array(indices))
+    ptr = index_tensor(_as_array(indptr))
+    if ptr.dtype != idx.dtype:
+        ptr = ptr.to(idx.dtype)
+    if d.numel() != nnzb * bs * bs:
+        raise ValueError(f'data has {d.numel()} elements, {nnzb} blocks of {bs} x {bs} need {nnzb * bs * bs}.')
+    if idx.ndim != 1 or idx.shape[0] != nnzb:
+        raise ValueError(f'indices should have shape ({nnzb},), but we got {tuple(idx.shape)}.')
+    if ptr.ndim != 1 or ptr.shape[0] != mb + 1:
+        raise ValueError(f'indptr should have shape ({mb + 1},), but we got {tuple(ptr.shape)}.')
+    if v.ndim != 1 or v.shape[0] != (mb if transpose else nb) * bs:
+        raise ValueError(f'Shape mismatch, vector {tuple(v.shape)} with a matrix of {shape} blocks of {bs} x {bs}, '
+                         f'transpose={transpose}.')
+    if needs_ad(d, v):
+        return _BSRMV.apply(d, v, idx, ptr, bs, nnzb, (mb, nb), bool(transpose))
+    return _bsr_launch(d.reshape(-1), idx, ptr, v, bs, nnzb, (mb, nb), transpose)
+
+
+def bcsrmm(A_data, B_data, B_indices, B_ptr, *, shape: Tuple[int, int], block_size: Tuple[int, int],
+           transpose: bool = False, method: str = 'cutlass') -> torch.Tensor:
+    """Dense-times-block-sparse product ``C = A @ B``.
+
+    ``A`` is dense ``(m, k)``; ``B`` of ``shape = (k, n)`` is stored by
+    32-column blocks: ``B_ptr`` (``n // 32 + 1``) points into ``B_indices``,
+    whose rows are ``(k block, data block)`` pairs, and ``B_data`` holds the
+    data blocks as ``(nblk, 32, 32)`` (or ``(nblk * 32, 32)``) with
+    ``B[ki*32 + b, ni*32 + a] = B_data[bi, a, b]`` -- the layout of the
+    reference's loop (sparse/_bsr_mm.py:292-307, reading the k slice as
+    ``ki*32 : ki*32 + 32``).  Returns ``C`` of shape ``(m, n)``.  float32 runs
+    on MFMA, float64 on FMA.  ``block_size`` must be ``(32, 32)`` and
+    ``method`` ``'cutlass'``, as in the reference; ``transpose=True`` and
+    differentiation are not provided (the reference's primitive rejects the
+    transpose keyword and its differentiation rules are stubs,
+    sparse/_bsr_mm.py:246-259, 377-391).  Reference: sparse/_bsr_mm.py:210-259.
+    """
+    A = _lib.to_device(_as_array(A_data))
+    Bd = _lib.to_device(_as_array(B_data))
+    Bi = index_tensor(_as_array(B_indices))
+    Bp = index_tensor(_as_array(B_ptr))
+    if Bp.dtype != Bi.dtype:
+        Bp = Bp.to(Bi.dtype)
+    k, n = int(shape[0]), int(shape[1])
+    if A.ndim != 2 or A.shape[1] != k:
+        raise AssertionError(f'A_data of shape {tuple(A.shape)} does not fit B of shape {tuple(shape)}.')
+    if method != 'cutlass':
+        raise ValueError
+    if tuple(int(b) for b in block_size) != (32, 32):
+        raise AssertionError('cutlass based block-sparse mm only support block size (32, 32)')
+    if transpose:
+        raise NotImplementedError('bcsrmm computes A @ B only (transpose=True is not supported).')
+    if A.dtype != Bd.dtype:
+        raise AssertionError(f'A_data and B_data must have one dtype, got {A.dtype} and {Bd.dtype}.')
+    if A.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f'Only support float32 and float64. But we got {A.dtype}.')
+    if k % 32 or n % 32:
+        raise ValueError(f'shape {tuple(shape)} must be a multiple of the 32 x 32 block size.')
+    if Bi.ndim != 2 or Bi.shape[1] != 2:
+        raise AssertionError(f'B_indices must have shape (nblk, 2), got {tuple(Bi.shape)}.')
+    nblk = int(Bi.shape[0])
+    if Bd.numel() != nblk * 32 * 32:
+        raise AssertionError(f'B_data has {Bd.numel()} elements, {nblk} blocks of 32 x 32 need {nblk * 1024}.')
+    if Bp.ndim != 1 or Bp.shape[0] != n // 32 + 1:
+        raise AssertionError(f'B_ptr should have shape ({n // 32 + 1},), but we got {tuple(Bp.shape)}.')
+    if needs_ad(A, Bd):
+        raise NotImplementedError('bcsrmm has no differentiation rule.')
+    A, Bd, Bi, Bp = A.contiguous(), Bd.contiguous(), Bi.contiguous(), Bp.contiguous()
+    m = int(A.shape[0])
+    out = torch.empty((m, n), dtype=A.dtype, device=A.device)
+    rc = _lib.lib().bpx_bcsrmm(_lib.stream_ptr(), _lib.ptr(A), _lib.bpx_dtype(A), _lib.ptr(Bd), _lib.ptr(Bi),
+                               _lib.ptr(Bp), _lib.bpx_dtype(Bi), nblk, m, k, n, _lib.ptr(out))
+    _lib.check(rc, "bpx_bcsrmm")
+    return out

@@ -352,6 +352,41 @@ int bpx_coo_sddmm(void* stream, const void* row, const void* col, int index_dtyp
                   int64_t b_stride, int64_t b_col_stride, int64_t ncols, int dtype, void* out);
 
 /* -------------------------------------------------------------------------
+ * Block-sparse (BSR) products.
+ *
+ * bpx_bsrmv replaces `cusparse_bcsr_matvec` (sparse/_bsr_mv.py:60-91; GPU
+ * lowering :118-146 = cuSPARSE bsrmv): out = A @ vector, A of mb x nb blocks
+ * of blocksize x blocksize given by data[nnzb][bs][bs] (row-major blocks, the
+ * layout the reference's transpose rule reads, :175-199), block columns
+ * indices[nnzb] and block-row pointers indptr[mb + 1]; out has mb * bs
+ * entries.  transpose != 0: out = A^T @ vector, nb * bs entries (the
+ * reference's transpose rule calls it; its own lowerings do not take it).
+ * dtype f32 or f64 for data, vector and out; indices i32 or i64.  Blocks
+ * whose column lies outside [0, nb) are skipped.
+ *
+ * bpx_bsr_sddmm: the value cotangent of bpx_bsrmv -- out[j][p][q] =
+ * rowv[i*bs + p] * colv[indices[j]*bs + q] for block j of block row i
+ * (pass rowv = cotangent, colv = vector; transposed: rowv = vector,
+ * colv = cotangent).
+ *
+ * bpx_bcsrmm replaces `bcsrmm` (sparse/_bsr_mm.py:210-259; Numba loop
+ * :292-307, GPU lowering :344-375): out (m x n, row-major) = A (m x k,
+ * row-major) @ B, B (k x n) stored by 32-column blocks: b_ptr[n/32 + 1] over
+ * column blocks, b_indices[nblk][2] = (k block, data block),
+ * b_data[nblk][32][32] with B[ki*32 + b][ni*32 + a] = b_data[bi][a][b].
+ * k and n multiples of 32 (the reference's block size is fixed at 32 x 32).
+ * f32 runs on MFMA (A and b_data 16-byte aligned), f64 on FMA.  Blocks with
+ * a k block or data block outside range are skipped.
+ * ------------------------------------------------------------------------- */
+int bpx_bsrmv(void* stream, const void* data, int dtype, const void* indices, const void* indptr, int index_dtype,
+              int64_t nnzb, int64_t blocksize, const void* vector, int64_t mb, int64_t nb, int transpose, void* out);
+int bpx_bsr_sddmm(void* stream, const void* indices, const void* indptr, int index_dtype, int64_t nnzb,
+                  int64_t blocksize, const void* rowv, const void* colv, int dtype, int64_t mb, int64_t nb,
+                  void* out);
+int bpx_bcsrmm(void* stream, const void* a, int dtype, const void* b_data, const void* b_indices, const void* b_ptr,
+               int index_dtype, int64_t nblk, int64_t m, int64_t k, int64_t n, void* out);
+
+/* -------------------------------------------------------------------------
  * FixedProb connectivity on the device.  Replaces FixedProb.build_csr
  * (connect/random_conn.py:85-128, pre_ratio = 1): each of the m rows gets
  * k = int(n * prob) distinct targets drawn uniformly from [0, n) (the
@@ -391,7 +426,7 @@ int bpx_copy(void* stream, const void* src, void* dst, int64_t bytes);
  * launch count for one kernel id (0 event scatter, 1 event gather,
  * 2 csr scatter, 3 csr gather, 4 jitconn outdim, 5 jitconn scatter,
  * 6 event info, 7 lif step, 8 fixed order, 9 copy, 10 network step,
- * 11 tiled event scatter, 12 COO products).
+ * 11 tiled event scatter, 12 COO products, 13 BSR products).
  */
 int bpx_timer_enable(int on);
 int bpx_timer_read(int kernel_id, double* total_ms, int64_t* launches);

@@ -12,6 +12,8 @@ path (``brainpy_amd``) never imports it and has no CPU fallback.
   restatement on small shapes.
 * ``oracle.coo_ref`` -- numpy restatement of the COO products (coomv,
   seg_matmul).
+* ``oracle.bsr_ref`` -- numpy restatement of the block-sparse products
+  (cusparse_bcsr_matvec, bcsrmm).
 
 Parity unpinned: the normal-distribution weights of jitconn (log/sqrt/sin
 are device/libm math, pinned to a tolerance only) and the brainpylib

@@ -168,3 +168,20 @@ def test_argument_validation_of_coo_conn_and_plan_entry_points():
     plan.mem = 1
     assert L.bpx_csrmv_tiled(None, ctypes.byref(plan), None, F64, 0, None, None, 0) == DTYPE
     assert L.bpx_csrmv_tiled(None, ctypes.byref(plan), None, F32, 0, None, None, 0) == WS
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libbpx.so not built")
+def test_argument_validation_of_bsr_entry_points():
+    from brainpy_amd import _lib
+    L = _lib.lib()
+    SHAPE, DTYPE, ARG = -2, -1, -5
+    F32, I32, BF16 = _lib.BPX_F32, _lib.BPX_I32, _lib.BPX_BF16
+    assert L.bpx_bsrmv(None, None, F32, None, None, I32, 4, 0, None, 2, 2, 0, None) == SHAPE    # blocksize 0
+    assert L.bpx_bsrmv(None, None, F32, None, None, I32, 4, 4, None, -1, 2, 0, None) == SHAPE   # mb < 0
+    assert L.bpx_bsrmv(None, None, F32, None, None, I32, 4, 4, None, 2, 2, 0, None) == ARG      # null arrays
+    assert L.bpx_bsr_sddmm(None, None, None, I32, 4, 0, None, None, F32, 2, 2, None) == SHAPE
+    assert L.bpx_bsr_sddmm(None, None, None, I32, 4, 4, None, None, F32, 2, 2, None) == ARG
+    assert L.bpx_bcsrmm(None, None, F32, None, None, None, I32, 1, 8, 48, 64, None) == SHAPE    # k % 32
+    assert L.bpx_bcsrmm(None, None, F32, None, None, None, I32, 1, 8, 64, 40, None) == SHAPE    # n % 32
+    assert L.bpx_bcsrmm(None, None, F32, None, None, None, I32, 1, 8, 64, 64, None) == ARG      # null arrays
+    assert L.bpx_bcsrmm(None, None, BF16, None, None, None, I32, 0, 0, 64, 64, None) == 0       # empty: no work
