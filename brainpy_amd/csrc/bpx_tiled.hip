@@ -11,8 +11,9 @@
 //   * the rows are dealt to the 8C copies in 64-row chunks (chunk q -> copy
 //     q % 8C); inside a block each wave walks chunks of its copy, lists the
 //     active rows with their tile-p segment (row-tile offsets precomputed),
-//     and groups of 4 lanes stream the segments with 16-byte vector loads
-//     (segments are padded to 8 entries), accumulating at 16-bit tile-local
+//     and groups of 8 lanes stream the segments with vector loads (8 bytes
+//     of indices, 16 of f32 weights; segments are padded to 4 entries),
+//     accumulating at 16-bit tile-local
 //     column indices with LDS integer atomics in 64-bit fixed point (gfx950's
 //     LDS float atomics run ~20x slower than integer ones);
 //   * at the end each block adds its non-zero LDS entries into the output
@@ -48,9 +49,11 @@ static int device_cus() {
 // tile-local indices (u16) and weights in tile-grouped order (stable)
 // ---------------------------------------------------------------------------
 // Segments are padded to a multiple of kVec elements so the exec kernel can
-// fetch 8 tile-local indices (16 B) and 8 weights with one vector load each;
+// fetch kVec tile-local indices and kVec weights with one vector load each
+// (kVec = 4: 8 B of indices, 16 B of f32 weights; 8 measured slower at most
+// spike rates, its padding costs ~11 % more segment bytes);
 // padding entries carry the index 0xFFFF, which no tile reaches.
-constexpr int kVec = 8;
+constexpr int kVec = 4;
 
 template <typename I>
 __global__ __launch_bounds__(kBuildThreads) void k_tiled_count(const I* __restrict__ indices,
@@ -202,9 +205,9 @@ __global__ void k_store_scalar(const void* src, int src_dtype, float* dst) {
 // Each wave walks its chunks of 64 rows four at a time: the next four
 // chunks' events are loaded before the current ones are processed.  Active
 // rows of a batch with a non-empty tile-p segment (ballot + mbcnt) are
-// listed in LDS with the segment [start, end); then 16 groups of 4 lanes
-// take one listed row each per round, a lane loading 8 indices (16 bytes)
-// and 8 weights per pass before its LDS atomics.  Every load of a phase is
+// listed in LDS with the segment [start, end); then 8 groups of 8 lanes
+// take one listed row each per round, a lane loading 4 indices (8 bytes)
+// and 4 weights per pass before its LDS atomics.  Every load of a phase is
 // issued without a branch around it, so the four chunks' loads are in
 // flight together.
 // ---------------------------------------------------------------------------
@@ -259,7 +262,7 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
   const long long hfx = fixed_of(hw, fx_scale);
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  constexpr int GV = 4;                 // lanes per row segment, kVec elements each
+  constexpr int GV = 32 / kVec;         // lanes per row segment, kVec elements each
   constexpr int NGRP = 64 / GV;         // lane groups per wave
   const int g = lane / GV, lv = lane % GV;
   int2* seg = s_seg[wave];
@@ -321,8 +324,8 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // each group of 4 lanes takes R listed rows per round; a lane fetches
-    // 8 indices (one 16-byte load) and 8 weights per row and round
+    // each group of GV lanes takes R listed rows per round; a lane fetches
+    // kVec indices and kVec weights (one vector load each) per pass
     for (int e = g; e < cnt; e += R * NGRP) {
       int2 sg[R];
       float xr[R];
@@ -339,24 +342,39 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
 #pragma unroll
       for (int q = 0; q < R; ++q) len = max(len, sg[q].y - sg[q].x);
       for (int32_t k = kVec * lv; k < len; k += kVec * GV) {
-        uint4 li[R];
+        constexpr int NW = kVec / 2;    // 32-bit words of kVec u16 indices
+        uint32_t li[R][NW];
         float wv[R][kVec];
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           const int32_t j = sg[q].x + k;
-          li[q] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-          if (j < sg[q].y) {
-            li[q] = *reinterpret_cast<const uint4*>(lidx + j);
-            if constexpr (WK == 1) {
-              const float4 a = *reinterpret_cast<const float4*>((const float*)wptr + j);
-              const float4 b = *reinterpret_cast<const float4*>((const float*)wptr + j + 4);
-              wv[q][0] = a.x; wv[q][1] = a.y; wv[q][2] = a.z; wv[q][3] = a.w;
-              wv[q][4] = b.x; wv[q][5] = b.y; wv[q][6] = b.z; wv[q][7] = b.w;
-            } else if constexpr (WK == 2) {
-              const uint4 h = *reinterpret_cast<const uint4*>((const bf16_t*)wptr + j);
-              const uint32_t hw4[4] = {h.x, h.y, h.z, h.w};
 #pragma unroll
-              for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < NW; ++u) li[q][u] = 0xFFFFFFFFu;
+          if (j < sg[q].y) {
+            if constexpr (kVec == 8) {
+              const uint4 a = *reinterpret_cast<const uint4*>(lidx + j);
+              li[q][0] = a.x; li[q][1] = a.y; li[q][2] = a.z; li[q][3] = a.w;
+            } else {
+              const uint2 a = *reinterpret_cast<const uint2*>(lidx + j);
+              li[q][0] = a.x; li[q][1] = a.y;
+            }
+            if constexpr (WK == 1) {
+#pragma unroll
+              for (int h = 0; h < kVec / 4; ++h) {
+                const float4 a = *reinterpret_cast<const float4*>((const float*)wptr + j + 4 * h);
+                wv[q][4 * h] = a.x; wv[q][4 * h + 1] = a.y; wv[q][4 * h + 2] = a.z; wv[q][4 * h + 3] = a.w;
+              }
+            } else if constexpr (WK == 2) {
+              uint32_t hw4[NW];
+              if constexpr (kVec == 8) {
+                const uint4 h = *reinterpret_cast<const uint4*>((const bf16_t*)wptr + j);
+                hw4[0] = h.x; hw4[1] = h.y; hw4[2] = h.z; hw4[3] = h.w;
+              } else {
+                const uint2 h = *reinterpret_cast<const uint2*>((const bf16_t*)wptr + j);
+                hw4[0] = h.x; hw4[1] = h.y;
+              }
+#pragma unroll
+              for (int u = 0; u < NW; ++u) {
                 wv[q][2 * u] = __uint_as_float(hw4[u] << 16);
                 wv[q][2 * u + 1] = __uint_as_float(hw4[u] & 0xFFFF0000u);
               }
@@ -365,7 +383,7 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
         }
 #pragma unroll
         for (int q = 0; q < R; ++q) {
-          const uint32_t l4[4] = {li[q].x, li[q].y, li[q].z, li[q].w};
+          const uint32_t* l4 = li[q];
 #pragma unroll
           for (int u = 0; u < kVec; ++u) {
             const uint32_t c = (l4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
