@@ -124,21 +124,24 @@ __global__ __launch_bounds__(256) void k_csr_gather(const V* __restrict__ data,
     s = s < 0 ? 0 : s;
     t = t > nnz ? nnz : t;
     for (int64_t base = s; base < t; base += (int64_t)G * U) {
+      // indices and weights are loaded together (the weights do not wait for
+      // the gathered value), then the gathers
       int64_t idx[U];
+      A wv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int64_t j = base + lane + (int64_t)u * G;
         idx[u] = j < t ? (int64_t)indices[j] : -1;
+        wv[u] = HOMO ? homo_w : (j < t ? (A)to_f32(data[j]) : A(0));
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int64_t j = base + lane + (int64_t)u * G;
         if ((uint64_t)idx[u] < (uint64_t)n) {
           const E x = vec[idx[u]];
           if constexpr (EVENT) {
-            if (ev_true(x, rule)) acc += HOMO ? homo_w : (A)to_f32(data[j]);
+            if (ev_true(x, rule)) acc += wv[u];
           } else {
-            acc += (HOMO ? homo_w : (A)to_f32(data[j])) * (A)x;
+            acc += wv[u] * (A)x;
           }
         }
       }
@@ -199,17 +202,20 @@ __global__ __launch_bounds__(256) void k_csr_gather_bits(const V* __restrict__ d
       s = s < 0 ? 0 : s;
       t = t > nnz ? nnz : t;
       for (int64_t base = s; base < t; base += (int64_t)G * U) {
+        // the weights are loaded with the indices, not after the event test
+        // (with 10 % events most of their lines are touched anyway), so a
+        // pass costs one memory round trip
         int64_t idx[U];
+        A wv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int64_t j = base + lane + (int64_t)u * G;
           idx[u] = j < t ? (int64_t)indices[j] : -1;
+          wv[u] = HOMO ? homo_w : (j < t ? (A)to_f32(data[j]) : A(0));
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const int64_t j = base + lane + (int64_t)u * G;
-          if ((uint64_t)idx[u] < (uint64_t)n && ((s_ev[idx[u] >> 5] >> (idx[u] & 31)) & 1u))
-            acc += HOMO ? homo_w : (A)to_f32(data[j]);
+          if ((uint64_t)idx[u] < (uint64_t)n && ((s_ev[idx[u] >> 5] >> (idx[u] & 31)) & 1u)) acc += wv[u];
         }
       }
     }
