@@ -35,7 +35,7 @@ __device__ __forceinline__ W clamp_mm(W x, W lo, W hi) {
 // CSR (pre) or CSC (post) form: rows of `ptr` are the spiking side; `other`
 // maps an entry to the trace index; `wid` maps an entry to its synapse
 // (null: the entry index itself).
-template <typename I, typename E, typename W>
+template <typename I, typename E, typename W, bool WID>
 __global__ __launch_bounds__(kPlastThreads) void k_sparse_on_spike(const I* __restrict__ other,
                                                                    const I* __restrict__ ptr,
                                                                    const I* __restrict__ wid,
@@ -64,19 +64,22 @@ __global__ __launch_bounds__(kPlastThreads) void k_sparse_on_spike(const I* __re
       for (int64_t k0 = s + lane; k0 < t; k0 += 64 * kPlastU) {
         int64_t j[kPlastU], l[kPlastU];
         bool ok[kPlastU];
+        W wv[kPlastU], tv[kPlastU];
 #pragma unroll
         for (int u = 0; u < kPlastU; ++u) {
           const int64_t k = k0 + 64 * u;
           ok[u] = k < t;
           const int64_t kc = ok[u] ? k : s;
           j[u] = (int64_t)other[kc];
-          l[u] = wid ? (int64_t)wid[kc] : kc;
+          l[u] = WID ? (int64_t)wid[kc] : kc;
+          // CSR form: the weight sits at the entry itself, so its load goes
+          // out with the index load instead of after it
+          if constexpr (!WID) wv[u] = out[kc];
         }
-        W wv[kPlastU], tv[kPlastU];
 #pragma unroll
         for (int u = 0; u < kPlastU; ++u) {
           ok[u] = ok[u] && (uint64_t)j[u] < (uint64_t)ntrace && (uint64_t)l[u] < (uint64_t)nnz;
-          wv[u] = ok[u] ? out[l[u]] : W(0);
+          if constexpr (WID) wv[u] = ok[u] ? out[l[u]] : W(0);
           tv[u] = ok[u] ? trace[j[u]] : W(0);
         }
 #pragma unroll
@@ -152,15 +155,21 @@ static int sparse_on_spike(hipStream_t s, const void* w, const void* other, cons
   if (nrows == 0 || nnz == 0) return BPX_OK;
   const int rw = plast_rows_per_wave(nrows);
   const unsigned blocks = plast_blocks(nrows, rw);
-#define LAUNCH_E(E)                                                                                            \
-  hipLaunchKernelGGL((k_sparse_on_spike<I, E, W>), dim3(blocks), dim3(kPlastThreads), 0, s, (const I*)other,    \
-                     (const I*)ptr, (const I*)wid, (const E*)spike, BPX_RULE_NONZERO, nrows, (const W*)trace,   \
+#define LAUNCH_EW(E, HW)                                                                                        \
+  hipLaunchKernelGGL((k_sparse_on_spike<I, E, W, HW>), dim3(blocks), dim3(kPlastThreads), 0, s, (const I*)other, \
+                     (const I*)ptr, (const I*)wid, (const E*)spike, BPX_RULE_NONZERO, nrows, (const W*)trace,    \
                      ntrace, (W)w_min, (W)w_max, (W*)out, nnz, rw)
+#define LAUNCH_E(E)            \
+  do {                         \
+    if (wid) LAUNCH_EW(E, true); \
+    else LAUNCH_EW(E, false);  \
+  } while (0)
   if (sp_dtype == BPX_BOOL) LAUNCH_E(uint8_t);
   else if (sp_dtype == BPX_F32) LAUNCH_E(float);
   else if (sp_dtype == BPX_F64) LAUNCH_E(double);
   else return BPX_ERR_DTYPE;
 #undef LAUNCH_E
+#undef LAUNCH_EW
   return BPX_OK;
 }
 
