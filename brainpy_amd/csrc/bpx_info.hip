@@ -91,7 +91,21 @@ __global__ __launch_bounds__(1024) void k_info_scan_lds(const int32_t* __restric
   const int64_t b = blockIdx.x;
   const int32_t* cnt = counts + b * ntiles;
   int32_t* off = offsets + b * ntiles;
-  for (int64_t i = threadIdx.x; i < ntiles; i += 1024) s_c[i] = cnt[i];
+  // all of a thread's loads (ntiles <= 32 * 1024) are issued before any LDS
+  // store, so the copy costs one memory round trip
+  {
+    int32_t v[kScanLdsTiles / 1024];
+#pragma unroll
+    for (int u = 0; u < kScanLdsTiles / 1024; ++u) {
+      const int64_t i = threadIdx.x + (int64_t)u * 1024;
+      v[u] = i < ntiles ? cnt[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kScanLdsTiles / 1024; ++u) {
+      const int64_t i = threadIdx.x + (int64_t)u * 1024;
+      if (i < ntiles) s_c[i] = v[u];
+    }
+  }
   __syncthreads();
   const int64_t per = (ntiles + 1023) / 1024;
   const int64_t lo = (int64_t)threadIdx.x * per;
