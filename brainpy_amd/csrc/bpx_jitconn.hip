@@ -45,8 +45,13 @@ __device__ __forceinline__ float draw_weight(Lfsr88& k, float w0, float w1) {
     float u1 = (float)k.next() * 2.3283064365386963e-10f;
     const float u2 = (float)k.next() * 2.3283064365386963e-10f;
     u1 = u1 > 1e-10f ? u1 : 1e-10f;
-    const float mag = sqrtf(-2.0f * logf(u1));
-    const float z = mag * sinf(6.283185307179586f * u2);
+    // hardware transcendentals: v_log_f32 is log2 (ln u1 = log2 u1 * ln 2),
+    // v_sin_f32 takes revolutions (sin(2 pi u2) from u2 itself, no rounding
+    // of the 2 pi product), v_sqrt_f32; each within a few ulp, the weights
+    // are pinned to a tolerance
+    const float ln_u1 = __builtin_amdgcn_logf(u1) * 0.6931471805599453f;
+    const float mag = __builtin_amdgcn_sqrtf(-2.0f * ln_u1);
+    const float z = mag * __builtin_amdgcn_sinf(u2);
     return w0 + w1 * z;
   } else {
     return w0;
@@ -422,8 +427,7 @@ static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t c
   const uint32_t step = (uint32_t)(((num_row + 1) >> 5) > 1 ? ((num_row + 1) >> 5) : 1);
   if (outdim) {
     const int64_t nwords = (num_col + 31) / 32;
-    if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && DIST != BPX_DIST_NORMAL && jit_rows_kernel() &&
-        bits_fit(nwords) && ws &&
+    if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && jit_rows_kernel() && bits_fit(nwords) && ws &&
         ws_bytes >= (size_t)nwords * 4 && num_col < 0xFFFFFFFFll) {
       uint32_t* bits = (uint32_t*)ws;
       hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((num_col + 255) / 256)), dim3(256), 0, s, (const uint8_t*)vec,
