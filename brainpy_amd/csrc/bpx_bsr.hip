@@ -421,6 +421,7 @@ int bpx_bsrmv(void* stream, const void* data, int dtype, const void* indices, co
   if (mb < 0 || nb < 0 || nnzb < 0 || blocksize < 1) return BPX_ERR_SHAPE;
   if ((mb > 0 && !indptr) || (nnzb > 0 && (!data || !indices)) || (!out && (transpose ? nb : mb) > 0))
     return BPX_ERR_ARG;
+  if (!vector && mb > 0 && nb > 0 && nnzb > 0) return BPX_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   BPX_BSR_VAL_SWITCH(dtype, BPX_BSR_IDX_SWITCH(index_dtype, {
     if (transpose)

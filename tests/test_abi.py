@@ -179,6 +179,8 @@ def test_argument_validation_of_bsr_entry_points():
     assert L.bpx_bsrmv(None, None, F32, None, None, I32, 4, 0, None, 2, 2, 0, None) == SHAPE    # blocksize 0
     assert L.bpx_bsrmv(None, None, F32, None, None, I32, 4, 4, None, -1, 2, 0, None) == SHAPE   # mb < 0
     assert L.bpx_bsrmv(None, None, F32, None, None, I32, 4, 4, None, 2, 2, 0, None) == ARG      # null arrays
+    dummy = ctypes.c_void_p(16)   # never dereferenced: the null vector is rejected first
+    assert L.bpx_bsrmv(None, dummy, F32, dummy, dummy, I32, 4, 4, None, 2, 2, 0, dummy) == ARG  # null vector
     assert L.bpx_bsr_sddmm(None, None, None, I32, 4, 0, None, None, F32, 2, 2, None) == SHAPE
     assert L.bpx_bsr_sddmm(None, None, None, I32, 4, 4, None, None, F32, 2, 2, None) == ARG
     assert L.bpx_bcsrmm(None, None, F32, None, None, None, I32, 1, 8, 48, 64, None) == SHAPE    # k % 32
