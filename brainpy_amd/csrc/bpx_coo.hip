@@ -401,6 +401,11 @@ int bpx_coomv(void* stream, const void* data, int data_dtype, int64_t data_len, 
   if (m < 0 || n < 0 || nse < 0) return BPX_ERR_SHAPE;
   if (data_len != 1 && data_len != nse) return BPX_ERR_SHAPE;
   if (vec_dtype != data_dtype) return BPX_ERR_DTYPE;
+  if (data_dtype != BPX_F32 && data_dtype != BPX_F64 && data_dtype != BPX_I32) return BPX_ERR_DTYPE;
+  // (a vector of length 0 may come as a null pointer)
+  if ((!out && (transpose ? n : m) > 0) ||
+      (nse > 0 && (!data || !row || !col || (!vector && (transpose ? m : n) > 0))))
+    return BPX_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   BPX_COO_VAL_SWITCH(data_dtype, BPX_COO_IDX_SWITCH(index_dtype,
       return (coomv_typed<V, I>(s, data, data_len, row, col, nse, vector, m, n, transpose, out))));
@@ -412,6 +417,10 @@ int bpx_coo_matmat(void* stream, const void* data, int data_dtype, int64_t data_
   if (m < 0 || n < 0 || k < 0 || nse < 0) return BPX_ERR_SHAPE;
   if (data_len != 1 && data_len != nse) return BPX_ERR_SHAPE;
   if (dense_dtype != data_dtype) return BPX_ERR_DTYPE;
+  if (data_dtype != BPX_F32 && data_dtype != BPX_F64 && data_dtype != BPX_I32) return BPX_ERR_DTYPE;
+  if ((!out && (sparse_left ? m : n) * k > 0) ||
+      (nse > 0 && k > 0 && (!data || !row || !col || (!dense && (sparse_left ? n : m) > 0))))
+    return BPX_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   BPX_COO_VAL_SWITCH(data_dtype, BPX_COO_IDX_SWITCH(index_dtype,
       return (matmat_typed<V, I>(s, data, data_len, row, col, nse, dense, m, n, k, sparse_left, out))));

@@ -351,6 +351,10 @@ int bpx_event_csrmv(void* stream, const void* data, int data_dtype, int64_t data
   if (m < 0 || n < 0 || nnz < 0) return BPX_ERR_SHAPE;
   if (data_len != 1 && data_len != nnz) return BPX_ERR_SHAPE;
   if (ev_rule != BPX_RULE_POSITIVE && ev_rule != BPX_RULE_NONZERO) return BPX_ERR_ARG;
+  // (a vector of length 0 may come as a null pointer)
+  if ((!out && (transpose ? n : m) > 0) ||
+      (nnz > 0 && (!data || !indices || !indptr || (!events && (transpose ? m : n) > 0))))
+    return BPX_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   if (nnz == 0) {
     if (data_dtype == BPX_F64) return launch_fill_zero<double>(s, (double*)out, transpose ? n : m);
@@ -378,6 +382,7 @@ int bpx_event_csrmv_add(void* stream, const void* data, int data_dtype, int64_t 
   if (m < 0 || n < 0 || nnz < 0) return BPX_ERR_SHAPE;
   if (data_len != 1 && data_len != nnz) return BPX_ERR_SHAPE;
   if (ev_rule != BPX_RULE_POSITIVE && ev_rule != BPX_RULE_NONZERO) return BPX_ERR_ARG;
+  if (nnz > 0 && (!data || !indices || !indptr || (!out && n > 0) || (!events && m > 0))) return BPX_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   if (nnz == 0) return BPX_OK;
 #define EV_SWITCH_ADD(V_)                                                                                      \
@@ -402,6 +407,9 @@ int bpx_csrmv(void* stream, const void* data, int data_dtype, int64_t data_len, 
   hipStream_t s = (hipStream_t)stream;
   const bool f64 = data_dtype == BPX_F64;
   if (vec_dtype != (f64 ? BPX_F64 : BPX_F32)) return BPX_ERR_DTYPE;
+  if ((!out && (transpose ? n : m) > 0) ||
+      (nnz > 0 && (!data || !indices || !indptr || (!vector && (transpose ? m : n) > 0))))
+    return BPX_ERR_ARG;
   if (nnz == 0) {
     if (f64) return launch_fill_zero<double>(s, (double*)out, transpose ? n : m);
     return launch_fill_zero<float>(s, (float*)out, transpose ? n : m);
@@ -422,6 +430,7 @@ int bpx_event_csrmv_fixed_order(void* stream, const void* csc_data, int data_dty
   if (data_len != 1 && data_len != nnz) return BPX_ERR_SHAPE;
   hipStream_t s = (hipStream_t)stream;
   if (n == 0) return BPX_OK;
+  if (!out || !csc_colptr || (nnz > 0 && (!csc_data || !csc_rows || (!events && m > 0)))) return BPX_ERR_ARG;
   const unsigned blocks = (unsigned)((n + 255) / 256);
   const bool homo = data_len == 1;
   if (data_dtype != BPX_F32) return BPX_ERR_DTYPE;

@@ -316,6 +316,7 @@ int bpx_event_info(void* stream, const void* events, int ev_dtype, int64_t batch
   if (batch > 65535) return BPX_ERR_SHAPE;
   hipStream_t s = (hipStream_t)stream;
   if (batch == 0) return BPX_OK;
+  if (!num || (n > 0 && (!events || !ids))) return BPX_ERR_ARG;
   if (n == 0) {
     hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, s, num, batch);
     BPX_CHECK_LAUNCH();

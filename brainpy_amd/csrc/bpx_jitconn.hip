@@ -537,6 +537,7 @@ extern "C" int bpx_jitconn_mv(void* stream, const void* vec, int vec_dtype, int 
   }
   hipStream_t s = (hipStream_t)stream;
   if (num_row == 0) return BPX_OK;
+  if (!out || (num_col > 0 && !vec)) return BPX_ERR_ARG;
   const int o = outdim_parallel ? 1 : 0;
   switch (dist) {
     case BPX_DIST_HOMO: return run_jit_vm<BPX_DIST_HOMO>(vm, s, vec, w0, w1, clen, seed, num_row, num_col, o, layout, out, workspace, workspace_bytes);

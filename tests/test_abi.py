@@ -187,3 +187,21 @@ def test_argument_validation_of_bsr_entry_points():
     assert L.bpx_bcsrmm(None, None, F32, None, None, None, I32, 1, 8, 64, 40, None) == SHAPE    # n % 32
     assert L.bpx_bcsrmm(None, None, F32, None, None, None, I32, 1, 8, 64, 64, None) == ARG      # null arrays
     assert L.bpx_bcsrmm(None, None, BF16, None, None, None, I32, 0, 0, 64, 64, None) == 0       # empty: no work
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libbpx.so not built")
+def test_null_arrays_are_rejected_before_launch():
+    # an entry point with work to do and a null array returns BPX_ERR_ARG
+    # instead of launching a kernel that would read through it
+    from brainpy_amd import _lib
+    L = _lib.lib()
+    ARG = -5
+    F32, I32, BOOL = _lib.BPX_F32, _lib.BPX_I32, _lib.BPX_BOOL
+    d = ctypes.c_void_p(16)   # stand-in for a device pointer; never dereferenced
+    assert L.bpx_event_csrmv(None, d, F32, 1, d, d, I32, 10, None, BOOL, 1, 4, 5, 1, d) == ARG    # events
+    assert L.bpx_event_csrmv(None, d, F32, 1, d, d, I32, 10, d, BOOL, 1, 4, 5, 1, None) == ARG    # out
+    assert L.bpx_csrmv(None, d, F32, 1, None, d, I32, 10, d, F32, 4, 5, 0, d) == ARG              # indices
+    assert L.bpx_coomv(None, d, F32, 1, d, d, I32, 10, None, F32, 4, 5, 0, d) == ARG              # vector
+    assert L.bpx_coo_matmat(None, d, F32, 1, d, d, I32, 10, d, F32, 4, 5, 3, 1, None) == ARG      # out
+    assert L.bpx_event_info(None, None, BOOL, 1, 100, d, d, d, 1 << 20) == ARG                    # events
+    assert L.bpx_jitconn_mv(None, None, BOOL, 1, 0, 1.0, 0.0, 3, 1, 10, 10, 1, 0, d, None, 0) == ARG
