@@ -32,7 +32,8 @@ extern "C" {
 #define BPX_ERR_SHAPE (-2)      /* inconsistent sizes */
 #define BPX_ERR_WORKSPACE (-3)  /* workspace missing or too small */
 #define BPX_ERR_LAUNCH (-4)     /* a kernel launch failed */
-#define BPX_ERR_ARG (-5)        /* invalid scalar argument */
+#define BPX_ERR_ARG (-5)        /* invalid scalar argument, or a null array the call would read or write
+                                   (an array of length 0 may be null) */
 
 /* dtype codes */
 #define BPX_BOOL 0   /* 1 byte, truthy if non-zero */
