@@ -215,8 +215,14 @@ constexpr int kChunksPerBatch = 4;
 constexpr int kListCap = 64 * kChunksPerBatch;
 constexpr int kWavesPerBlock = kTiledThreads / 64;
 
-// weight -> 64-bit fixed point (w * 2^shift, exact for |w| >= 2^(23-shift))
-__device__ __forceinline__ long long fixed_of(float w, double scale) { return __double2ll_rn((double)w * scale); }
+// weight -> 64-bit fixed point: round(w * 2^shift) to nearest even, exact
+// for |w| >= 2^(23-shift).  The plan's shift keeps every term below 2^50, so
+// one fma adding 1.5 * 2^52 (whose ulp is 1) rounds the scaled weight to an
+// integer, and the difference of the two bit patterns is that integer.
+__device__ __forceinline__ long long fixed_of(float w, double scale) {
+  const double y = fma((double)w, scale, 6755399441055744.0);
+  return (long long)__double_as_longlong(y) - 0x4338000000000000ll;
+}
 
 
 // DENSE: `ev` is a dense f32 vector x (bm.sparse.csrmv with transpose): the
@@ -567,6 +573,10 @@ int bpx_tiled_csr_build(void* stream, const void* data, int data_dtype, int64_t 
     double bound = (double)mx * (double)(m > 1 ? m : 1);
     int e = bound > 0.0 ? (int)ceil(log2(bound)) : -100;
     int shift = 61 - e;
+    // every term |w| * 2^shift below 2^50 (the exec kernel's conversion is
+    // exact there); binds only for m < 2^11
+    const int cap = 50 - (mx > 0.f ? (int)ceil(log2((double)mx)) : -100);
+    if (shift > cap) shift = cap;
     if (shift > 200) shift = 200;
     plan->fx_shift = shift;
   }

@@ -131,8 +131,8 @@ int bpx_event_csrmv_add(void* stream, const void* data, int data_dtype, int64_t 
  * atomic per synaptic event: the output is cut into one tile per CU of an
  * XCD (tile_len floats, 16-bit tile-local column indices), each XCD holds
  * one copy of the output in the LDS of its CUs, and rows carry per-tile
- * segment offsets; segments are padded to multiples of 8 entries (index
- * 0xFFFF) so indices and weights stream as 16-byte vectors.  No reference counterpart (the reference re-reads plain
+ * segment offsets; segments are padded to multiples of 4 entries (index
+ * 0xFFFF) so indices and weights stream as 8- and 16-byte vectors.  No reference counterpart (the reference re-reads plain
  * CSR each call, event/_csr_matvec.py:453-500); it serves the same operator.
  * Usage: bpx_tiled_csr_layout (sizes, fills the plan's shape fields) ->
  * caller allocates `bytes` of device memory -> bpx_tiled_csr_build ->
@@ -140,9 +140,10 @@ int bpx_event_csrmv_add(void* stream, const void* data, int data_dtype, int64_t 
  * 32 KiB of LDS next to the kernel's row lists (tile_len <= 4096,
  * n <= 4096 * CUs-per-XCD = 131072 on MI355X).  Accumulation inside a tile
  * is in 64-bit fixed point (LDS integer atomics; gfx950's LDS float atomics
- * are ~20x slower): fx_shift = 61 - ceil(log2(max|w| * m)) keeps every sum
- * below 2^62, weights with |w| >= 2^(23 - fx_shift) convert exactly, and each
- * copy's tile sum is rounded to f32 once.  The build synchronises the stream
+ * are ~20x slower): fx_shift = min(61 - ceil(log2(max|w| * m)),
+ * 50 - ceil(log2 max|w|)) keeps every sum below 2^62 and every term below
+ * 2^50, weights with |w| >= 2^(23 - fx_shift) convert exactly (round to
+ * nearest even below that), and each copy's tile sum is rounded to f32 once.  The build synchronises the stream
  * once (to read max|w|) and rejects non-finite weights (BPX_ERR_ARG).  Weights: f32, bf16 (rounded to nearest even
  * at build, accumulated in f32) or homogeneous (data_len 1).
  */
@@ -154,7 +155,7 @@ typedef struct {
   int32_t tile_len;     /* T: floats per tile */
   int32_t weight_dtype; /* BPX_F32, BPX_BF16 or BPX_TILED_HOMO */
   int32_t fx_shift;     /* LDS accumulators hold sum(w) * 2^fx_shift as int64 */
-  int64_t capacity;     /* element slots incl. padding (segments padded to 8) */
+  int64_t capacity;     /* element slots incl. padding (segments padded to 4) */
   int64_t rowtile_offset, aux_offset, lidx_offset, weights_offset, homo_offset; /* byte offsets in mem */
   void* mem;            /* device memory holding the plan (set by build) */
 } bpx_tiled_csr;
