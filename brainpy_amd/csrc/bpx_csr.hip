@@ -165,8 +165,13 @@ __device__ __forceinline__ uint32_t nz_flags4(uint32_t w) {
   return ((t * 0x00204081u) >> 21) & 0xFu;
 }
 
+// 1024-thread blocks, 2 per CU: a CU packs the event bitmask twice per call
+// (each block reads the whole event vector) instead of once per 256-thread
+// block, with the same 32 waves of rows in flight
+constexpr int kGatherBitsThreads = 1024;
+
 template <int G, int U, bool HOMO, typename I, typename V>
-__global__ __launch_bounds__(256) void k_csr_gather_bits(const V* __restrict__ data,
+__global__ __launch_bounds__(kGatherBitsThreads) void k_csr_gather_bits(const V* __restrict__ data,
                                                          const I* __restrict__ indices,
                                                          const I* __restrict__ indptr, int64_t nnz,
                                                          const uint8_t* __restrict__ ev, int64_t m, int64_t n,
@@ -174,7 +179,7 @@ __global__ __launch_bounds__(256) void k_csr_gather_bits(const V* __restrict__ d
   using A = typename acc_of<V>::type;
   extern __shared__ uint32_t s_ev[];
   const int64_t nw = (n + 31) >> 5;
-  for (int64_t w = threadIdx.x; w < nw; w += 256) {
+  for (int64_t w = threadIdx.x; w < nw; w += kGatherBitsThreads) {
     const int64_t c0 = w << 5;
     uint32_t bits = 0;
     if (c0 + 32 <= n) {
@@ -190,7 +195,7 @@ __global__ __launch_bounds__(256) void k_csr_gather_bits(const V* __restrict__ d
   }
   __syncthreads();
   const A homo_w = HOMO ? (A)to_f32(data[0]) : A(0);
-  constexpr int NG = 256 / G;
+  constexpr int NG = kGatherBitsThreads / G;
   const int lane = (int)threadIdx.x % G;
   const int64_t groups = (m + NG - 1) / NG;
   for (int64_t gi = blockIdx.x; gi < groups; gi += gridDim.x) {
@@ -259,8 +264,8 @@ static int64_t gather_bits_grid() {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
       cus = 256;
     const char* e = getenv("BPX_GATHER_BLOCKS_PER_CU");
-    const int per = e ? atoi(e) : 8;
-    return (int64_t)cus * (per > 0 ? per : 8);
+    const int per = e ? atoi(e) : 2;
+    return (int64_t)cus * (per > 0 ? per : 2);
   }();
   return g;
 }
@@ -293,7 +298,8 @@ static int launch_csr(hipStream_t s, int transpose, const void* data, const void
     if (blocks > 0 && bits) {
       int64_t grid = gather_bits_grid();
       if (grid > blocks) grid = blocks;
-      BPX_LAUNCH(KID_EVENT_GATHER, (k_csr_gather_bits<G, U, HOMO, I, V>), dim3((unsigned)grid), dim3(256),
+      BPX_LAUNCH(KID_EVENT_GATHER, (k_csr_gather_bits<G, U, HOMO, I, V>), dim3((unsigned)grid),
+                 dim3(kGatherBitsThreads),
                  (size_t)nw * 4, s, (const V*)data, (const I*)indices, (const I*)indptr, nnz,
                  (const uint8_t*)vec, m, n, (A*)out);
     } else if (blocks > 0) {
