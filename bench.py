@@ -273,11 +273,13 @@ def bench_forms(_lib, torch, steps=30, warmup=3):
     w = torch.empty(nnz, dtype=torch.float32, device=dev).normal_(0.5, 0.1, generator=g)
     x = torch.rand(n, generator=g, device=dev)
     ev = torch.rand(n, generator=g, device=dev) < 0.1
+    ev_f = ev.to(torch.float32)
     out = torch.empty(n, dtype=torch.float32, device=dev)
     res = []
     legs = (("sparse.csrmv", 0, x, _lib.BPX_F32, _lib.KID["csr_gather"], 8),
             ("sparse.csrmv", 1, x, _lib.BPX_F32, _lib.KID["csr_scatter"], 8),
-            ("event.csrmv", 0, ev, _lib.BPX_BOOL, _lib.KID["event_gather"], 8))
+            ("event.csrmv", 0, ev, _lib.BPX_BOOL, _lib.KID["event_gather"], 8),
+            ("event.csrmv", 0, ev_f, _lib.BPX_F32, _lib.KID["event_gather"], 8))
     for name, transpose, vec, vdt, kid, per in legs:
         def call():
             if name == "sparse.csrmv":
@@ -290,11 +292,13 @@ def bench_forms(_lib, torch, steps=30, warmup=3):
                 raise RuntimeError(f"{name} failed ({rc})")
         wall, kern = _time_calls(_lib, torch, call, kid, steps, warmup)
         alg = nnz * per + (n + 1) * 4 + n * 4 + n * (1 if vdt == _lib.BPX_BOOL else 4)
-        r = {"op": name, "transpose": bool(transpose), "nnz": nnz, "us_per_call": wall * 1e6,
+        r = {"op": name, "transpose": bool(transpose), "events": "bool" if vdt == _lib.BPX_BOOL else "f32",
+             "nnz": nnz, "us_per_call": wall * 1e6,
              "kernel_us": kern * 1e6, "alg_bytes": alg, "gbs": alg / kern / 1e9,
              "frac": alg / kern / 1e9 / HBM_PEAK_GBS, "synapses_per_s": nnz / wall}
         res.append(r)
-        print(f"# {name:12s} transpose={transpose} {r['us_per_call']:8.1f} us/call kernel {r['kernel_us']:8.1f} us "
+        tag = f"{name} ({'bool' if vdt == _lib.BPX_BOOL else 'f32'} vector)"
+        print(f"# {tag:26s} transpose={transpose} {r['us_per_call']:8.1f} us/call kernel {r['kernel_us']:8.1f} us "
               f"{r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%) {r['synapses_per_s']:.3e} synapses/s",
               file=sys.stderr, flush=True)
     # the transpose with the plan for a static matrix (TiledCSR.matvec): all

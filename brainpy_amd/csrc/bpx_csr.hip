@@ -170,28 +170,68 @@ __device__ __forceinline__ uint32_t nz_flags4(uint32_t w) {
 // block, with the same 32 waves of rows in flight
 constexpr int kGatherBitsThreads = 1024;
 
-template <int G, int U, bool HOMO, typename I, typename V>
-__global__ __launch_bounds__(kGatherBitsThreads) void k_csr_gather_bits(const V* __restrict__ data,
+template <int G, int U, bool HOMO, typename I, typename V, typename E>
+__global__ __launch_bounds__(kGatherBitsThreads, 2) void k_csr_gather_bits(const V* __restrict__ data,
                                                          const I* __restrict__ indices,
                                                          const I* __restrict__ indptr, int64_t nnz,
-                                                         const uint8_t* __restrict__ ev, int64_t m, int64_t n,
+                                                         const E* __restrict__ ev, int rule, int64_t m, int64_t n,
                                                          typename acc_of<V>::type* __restrict__ out) {
   using A = typename acc_of<V>::type;
   extern __shared__ uint32_t s_ev[];
   const int64_t nw = (n + 31) >> 5;
-  for (int64_t w = threadIdx.x; w < nw; w += kGatherBitsThreads) {
-    const int64_t c0 = w << 5;
-    uint32_t bits = 0;
-    if (c0 + 32 <= n) {
-      const uint4 a = *reinterpret_cast<const uint4*>(ev + c0);
-      const uint4 b = *reinterpret_cast<const uint4*>(ev + c0 + 16);
-      const uint32_t q[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  if constexpr (sizeof(E) == 1) {
+    // bool events (16-byte aligned): a thread packs 32 events from two
+    // 16-byte loads
+    // every block reads the whole event vector; each starts at its own
+    // offset so the blocks' first reads of a line do not all miss at once
+    const int64_t rot = ((int64_t)blockIdx.x * 97) % nw;
+    for (int64_t wr = threadIdx.x; wr < nw; wr += kGatherBitsThreads) {
+      const int64_t w = wr + rot < nw ? wr + rot : wr + rot - nw;
+      const int64_t c0 = w << 5;
+      uint32_t bits = 0;
+      if (c0 + 32 <= n) {
+        const uint4 a = *reinterpret_cast<const uint4*>(ev + c0);
+        const uint4 b = *reinterpret_cast<const uint4*>(ev + c0 + 16);
+        const uint32_t q[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-      for (int k = 0; k < 8; ++k) bits |= nz_flags4(q[k]) << (4 * k);
-    } else {
-      for (int64_t c = c0; c < n; ++c) bits |= (ev[c] != 0 ? 1u : 0u) << (c - c0);
+        for (int k = 0; k < 8; ++k) bits |= nz_flags4(q[k]) << (4 * k);
+      } else {
+        for (int64_t c = c0; c < n; ++c) bits |= (ev[c] != 0 ? 1u : 0u) << (c - c0);
+      }
+      s_ev[w] = bits;
     }
-    s_ev[w] = bits;
+  } else {
+    // float events (tested with the operator's rule, never multiplied): a
+    // wave tests 64 consecutive events per load, several loads in flight,
+    // and a ballot gives two bitmask words
+    constexpr int kIn = 8;
+    const int lane = (int)(threadIdx.x & 63);
+    const int64_t wave = threadIdx.x >> 6;
+    constexpr int64_t kWaves = kGatherBitsThreads / 64;
+    const int64_t nchunk = (n + 63) >> 6;
+    const int64_t rot = ((int64_t)blockIdx.x * 97) % nchunk;   // as above
+    for (int64_t c = wave; c < nchunk; c += kWaves * kIn) {
+      // all loads first (unconditional, in bounds), then the tests: a
+      // short-circuit test around each load would serialise them
+      E v[kIn];
+      int64_t cc[kIn];
+#pragma unroll
+      for (int k = 0; k < kIn; ++k) {
+        const int64_t cr = c + kWaves * k;
+        cc[k] = cr < nchunk ? (cr + rot < nchunk ? cr + rot : cr + rot - nchunk) : nchunk;
+        const int64_t i = (cc[k] << 6) + lane;
+        v[k] = ev[i < n ? i : 0];
+      }
+#pragma unroll
+      for (int k = 0; k < kIn; ++k) {
+        const int64_t i = (cc[k] << 6) + lane;
+        const bool on = (i < n) & ev_true(v[k], rule);
+        const uint64_t mask = __ballot(on);
+        const int64_t cw = cc[k] << 1;                 // first of the chunk's two words
+        if (lane == 0 && cw < nw) s_ev[cw] = (uint32_t)mask;
+        if (lane == 1 && cw + 1 < nw) s_ev[cw + 1] = (uint32_t)(mask >> 32);
+      }
+    }
   }
   __syncthreads();
   const A homo_w = HOMO ? (A)to_f32(data[0]) : A(0);
@@ -293,15 +333,16 @@ static int launch_csr(hipStream_t s, int transpose, const void* data, const void
     constexpr int NG = 256 / G;
     const int64_t blocks = (m + NG - 1) / NG;
     const int64_t nw = (n + 31) / 32;
-    // bool events whose bitmask fits 64 KB of LDS (n <= 524,288), 16-byte aligned
-    const bool bits = EVENT && sizeof(E) == 1 && nw * 4 <= 65536 && ((uintptr_t)vec & 15u) == 0 && n > 0;
+    // events whose bitmask fits 64 KB of LDS (n <= 524,288); bool events
+    // 16-byte aligned
+    const bool bits = EVENT && nw * 4 <= 65536 && n > 0 && (sizeof(E) != 1 || ((uintptr_t)vec & 15u) == 0);
     if (blocks > 0 && bits) {
       int64_t grid = gather_bits_grid();
       if (grid > blocks) grid = blocks;
-      BPX_LAUNCH(KID_EVENT_GATHER, (k_csr_gather_bits<G, U, HOMO, I, V>), dim3((unsigned)grid),
+      BPX_LAUNCH(KID_EVENT_GATHER, (k_csr_gather_bits<G, U, HOMO, I, V, E>), dim3((unsigned)grid),
                  dim3(kGatherBitsThreads),
                  (size_t)nw * 4, s, (const V*)data, (const I*)indices, (const I*)indptr, nnz,
-                 (const uint8_t*)vec, m, n, (A*)out);
+                 (const E*)vec, rule, m, n, (A*)out);
     } else if (blocks > 0) {
       BPX_LAUNCH(EVENT ? KID_EVENT_GATHER : KID_CSR_GATHER, (k_csr_gather<G, U, EVENT, HOMO, I, V, E>),
                  dim3((unsigned)blocks), dim3(256), 0, s, (const V*)data, (const I*)indices,

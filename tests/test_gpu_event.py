@@ -67,6 +67,37 @@ def test_event_rules_and_taichi_twin():
                                                   rule=orc.RULE_NONZERO))
 
 
+@pytest.mark.parametrize("n", [200, 4099, 70000])
+@pytest.mark.parametrize("evdt", [np.float32, np.float64])
+def test_event_gather_float_events_rules(n, evdt):
+    # transpose=False with float events: the gather tests each event with
+    # the operator's rule (csrmv: > 0, the Numba kernels; csrmv_taichi: != 0)
+    # and never multiplies by it, so signed values, -0.0 and NaN select rows
+    # exactly as the oracle does (homo weight 1: exact integer sums); n not a
+    # multiple of 32 or 64 leaves partial bitmask words
+    m = 300
+    indices, indptr = fixed_prob_csr(m, n, min(0.5, 40.0 / n), 6)
+    rng = np.random.default_rng(n)
+    r = rng.random(n)
+    ev = np.zeros(n, evdt)
+    ev[r < 0.10] = -1.5
+    ev[(r >= 0.10) & (r < 0.25)] = 2.0
+    ev[(r >= 0.25) & (r < 0.26)] = -0.0
+    ev[r > 0.995] = np.nan
+    one = np.float32(1.0)
+    a = bm.event.csrmv(one, indices, indptr, ev, shape=(m, n), transpose=False).cpu().numpy()
+    np.testing.assert_array_equal(a, orc.event_csrmv(one, indices, indptr, ev, (m, n), False,
+                                                     rule=orc.RULE_POSITIVE))
+    b = bm.event.csrmv_taichi(one, indices, indptr, ev, shape=(m, n), transpose=False)[0].cpu().numpy()
+    np.testing.assert_array_equal(b, orc.event_csrmv(one, indices, indptr, ev, (m, n), False,
+                                                     rule=orc.RULE_NONZERO))
+    w = rng.normal(0.5, 0.1, indices.size).astype(np.float32)
+    y = bm.event.csrmv(w, indices, indptr, ev, shape=(m, n), transpose=False).cpu().numpy()
+    D = dense_of(w, indices, indptr, (m, n))
+    ref = D @ np.where(ev > 0, 1.0, 0.0)
+    _tol_check(y, ref, np.abs(D).sum(axis=1).max())
+
+
 def test_f64_weights():
     indices, indptr = random_csr(120, 90, 0.2, 6)
     w = np.random.default_rng(1).normal(0.5, 0.1, indices.size)
