@@ -183,6 +183,29 @@ __device__ __forceinline__ float repeat_sum_f(uint32_t c, float w) {
   return r;
 }
 
+// copy of the packed event bitmask into LDS by a whole block: each thread's
+// loads are issued in batches before their LDS stores (a plain copy loop
+// waits for every load), and each block starts at its own offset so the
+// blocks' first reads of a line do not all miss L2 together
+__device__ __forceinline__ void copy_bits_to_lds(const uint32_t* __restrict__ gbits, int64_t nwords,
+                                                 uint32_t* s_bits) {
+  constexpr int kB = 16;
+  const int64_t rot = ((int64_t)blockIdx.x * 997) % (nwords > 0 ? nwords : 1);
+  for (int64_t base = threadIdx.x; base < nwords; base += (int64_t)kJitThreads * kB) {
+    uint32_t v[kB];
+    int64_t kk[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int64_t k = base + (int64_t)u * kJitThreads;
+      kk[u] = k < nwords ? (k + rot < nwords ? k + rot : k + rot - nwords) : -1;
+      v[u] = gbits[kk[u] >= 0 ? kk[u] : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < kB; ++u)
+      if (kk[u] >= 0) s_bits[kk[u]] = v[u];
+  }
+}
+
 template <int DIST, bool Q24>
 __global__ __launch_bounds__(kJitThreads) void k_jit_lane32_outdim_bits(const uint32_t* __restrict__ gbits,
                                                                         int64_t nwords, float w0, float w1, DivU32 dv,
@@ -190,7 +213,7 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_lane32_outdim_bits(const ui
                                                                         int64_t num_col, uint32_t step,
                                                                         float* __restrict__ out) {
   extern __shared__ uint32_t s_bits[];
-  for (int64_t k = threadIdx.x; k < nwords; k += kJitThreads) s_bits[k] = gbits[k];
+  copy_bits_to_lds(gbits, nwords, s_bits);
   __syncthreads();
   const uint32_t l = threadIdx.x & 31;
   const int half = threadIdx.x >> 5;            // 32 halves of 32 lanes
@@ -272,7 +295,7 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
                                                                 uint32_t seed, int64_t num_row, int64_t num_col,
                                                                 uint32_t step, float* __restrict__ out) {
   extern __shared__ uint32_t s_bits[];
-  for (int64_t k = threadIdx.x; k < nwords; k += kJitThreads) s_bits[k] = gbits[k];
+  copy_bits_to_lds(gbits, nwords, s_bits);
   __syncthreads();
   constexpr bool HITS = DIST == BPX_DIST_HOMO && FIN;
   const int64_t nthreads = (int64_t)gridDim.x * kJitThreads;
