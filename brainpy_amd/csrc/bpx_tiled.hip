@@ -460,10 +460,16 @@ struct TiledShape {
 };
 
 
+static int max_tile_len();
+
+// one tile per CU of an XCD, or more tiles (up to kMaxTiles) when a tile
+// would not fit the block's LDS
 static TiledShape tiled_shape(int64_t n) {
   int cus = device_cus();
   int P = cus / 8;
   if (P < 1) P = 1;
+  const int64_t tmax = max_tile_len();
+  if (tmax > 0 && (n + P - 1) / P > tmax) P = (int)((n + tmax - 1) / tmax);
   if (P > kMaxTiles) P = kMaxTiles;
   int64_t T = (n + P - 1) / P;
   if (T < 1) T = 1;

@@ -137,8 +137,9 @@ int bpx_event_csrmv_add(void* stream, const void* data, int data_dtype, int64_t 
  * Usage: bpx_tiled_csr_layout (sizes, fills the plan's shape fields) ->
  * caller allocates `bytes` of device memory -> bpx_tiled_csr_build ->
  * bpx_event_csrmv_tiled per call.  Requires nnz < 2^31 and a tile that fits
- * 32 KiB of LDS next to the kernel's row lists (tile_len <= 4096,
- * n <= 4096 * CUs-per-XCD = 131072 on MI355X).  Accumulation inside a tile
+ * 32 KiB of LDS next to the kernel's row lists (tile_len <= 4096): one tile
+ * per CU of an XCD up to n = 4096 * CUs-per-XCD = 131072 on MI355X, then up
+ * to 64 tiles per copy, n <= 262144.  Accumulation inside a tile
  * is in 64-bit fixed point (LDS integer atomics; gfx950's LDS float atomics
  * are ~20x slower): fx_shift = min(61 - ceil(log2(max|w| * m)),
  * 50 - ceil(log2 max|w|)) keeps every sum below 2^62 and every term below
@@ -151,7 +152,7 @@ int bpx_event_csrmv_add(void* stream, const void* data, int data_dtype, int64_t 
 #define BPX_TILED_MAX_TILE 32768
 typedef struct {
   int64_t m, n, nnz;
-  int32_t tiles;        /* P: output tiles per copy (= CUs per XCD) */
+  int32_t tiles;        /* P: output tiles per copy (CUs per XCD, more for n > 131072) */
   int32_t tile_len;     /* T: floats per tile */
   int32_t weight_dtype; /* BPX_F32, BPX_BF16 or BPX_TILED_HOMO */
   int32_t fx_shift;     /* LDS accumulators hold sum(w) * 2^fx_shift as int64 */

@@ -234,7 +234,8 @@ def test_sparse_converters():
     np.testing.assert_array_equal(ptr2.cpu().numpy(), indptr)
 
 
-@pytest.mark.parametrize("shape", [(100, 200), (1000, 3000), (3000, 50), (257, 129), (5000, 100000)])
+@pytest.mark.parametrize("shape", [(100, 200), (1000, 3000), (3000, 50), (257, 129), (5000, 100000),
+                                   (2000, 200000), (300, 262144)])
 @pytest.mark.parametrize("wkind", ["homo", "f32", "bf16"])
 @pytest.mark.parametrize("sort", [True, False])
 def test_tiled_plan_vs_oracle(shape, wkind, sort):
@@ -286,6 +287,20 @@ def test_tiled_plan_dense_matvec(shape, wkind, xscale):
     if wkind == "f32":
         y2 = bm.sparse.csrmv(data, indices, indptr, x, shape=shape, transpose=True).cpu().numpy()
         assert np.all(np.abs(y2 - y) <= 2e-5 * mag + 1e-30)
+
+
+def test_tiled_plan_output_limit():
+    # more than 32 tiles per copy once a tile would not fit the LDS budget:
+    # 64 tiles of 4096 outputs hold n = 262,144; one more output is refused
+    from brainpy_amd._lib import BPXError
+    idx = np.array([0, 262143], np.int32)
+    ptr = np.array([0, 2], np.int32)
+    plan = bm.event.TiledCSR(np.float32(1.0), idx, ptr, shape=(1, 262144))
+    assert plan.plan.tiles == 64 and plan.plan.tile_len == 4096
+    y = plan.csrmv(np.ones(1, bool)).cpu().numpy()
+    assert y[0] == 1.0 and y[262143] == 1.0 and y.sum() == 2.0
+    with pytest.raises(BPXError):
+        bm.event.TiledCSR(np.float32(1.0), idx, ptr, shape=(1, 262145))
 
 
 def test_tiled_plan_dense_matvec_accumulate_and_nonfinite():
