@@ -519,8 +519,10 @@ def _percentile(x, q):
 def bench_coba(_lib, torch, steps=1000):
     """C1: 4000-neuron E/I COBA network (3200 E + 800 I, FixedProb p = 0.02,
     we = 0.6, wi = 6.7, input 20 mV, V0 ~ U[-60, -50], dt = 0.1 ms, 1000 steps,
-    LifRefLTC, exp_euler).  GPU: the batched step kernel on this one network
-    (CSR connectivity), HIP-graph captured; plain op-by-op GPU path; CPU: the
+    LifRefLTC, exp_euler).  GPU: the per-step network kernel and the
+    multi-step one (one block, 100 steps per launch) on this one network (CSR
+    connectivity), HIP-graph captured; the multi-step kernel as one launch
+    of 1000 steps; eager per-step launches; plain op-by-op GPU path; CPU: the
     oracle's serial restatement (1 thread)."""
     from brainpy_amd.dyn import EICOBANet
     from brainpy_amd.dyn.batched import BatchedCOBA, combined_csr
@@ -528,25 +530,37 @@ def bench_coba(_lib, torch, steps=1000):
     net = EICOBANet(seed=0, mode="atomic")
     csrs = {k: (net.csr[k][1], net.csr[k][2]) for k in ("E2E", "E2I", "I2E", "I2I")}
     spec = dict(n_exc=3200, n_inh=800, seed=0, w_exc=0.6, w_inh=6.7, I_ext=20.0, csr=csrs)
-    b = BatchedCOBA([spec], V_init=net.V0)
-    b.capture(100)
-    b.replay()
-    torch.cuda.synchronize()
-    b.reset()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps // 100):
+    def graph_run(fused):
+        b = BatchedCOBA([spec], V_init=net.V0)
+        b.capture(100, fused=fused)
         b.replay()
-    torch.cuda.synchronize()
-    t_graph = time.perf_counter() - t0
-    spikes = int(b.spike_count.sum())
+        torch.cuda.synchronize()
+        b.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps // 100):
+            b.replay()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, int(b.spike_count.sum()), b.V.cpu().numpy()
+
+    # one launch per step, and one multi-step launch per 100 steps (one
+    # block, counts in LDS), both HIP-graph captured
+    t_graph, spikes, V_steps = graph_run(False)
+    t_graph_multi, spikes_multi, V_multi = graph_run(True)
+    assert spikes == spikes_multi and np.array_equal(V_multi, V_steps)
     events = spikes * 80          # every neuron has 64 + 16 targets
     b2 = BatchedCOBA([spec], V_init=net.V0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    b2.run(steps)
+    b2.run(steps, fused=False)
     torch.cuda.synchronize()
     t_eager = time.perf_counter() - t0
+    b3 = BatchedCOBA([spec], V_init=net.V0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b3.run(steps, fused=True)
+    torch.cuda.synchronize()
+    t_one_launch = time.perf_counter() - t0
     net.reset()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -559,10 +573,12 @@ def bench_coba(_lib, torch, steps=1000):
                                      (-60.0, -50.0, -60.0, 20.0, 5.0, 1.0), raster=False)
     t_cpu = time.perf_counter() - t0
     r = {"workload": "coba_4000_p0.02_1000steps", "steps": steps, "spikes": spikes, "cpu_spikes": int(cpu_spikes),
-         "synaptic_events": events, "gpu_graph_s": t_graph, "gpu_eager_s": t_eager, "gpu_plain_opbyop_s": t_plain,
+         "synaptic_events": events, "gpu_graph_s": t_graph, "gpu_graph_multistep_s": t_graph_multi,
+         "gpu_one_launch_s": t_one_launch, "gpu_eager_s": t_eager, "gpu_plain_opbyop_s": t_plain,
          "cpu_1thread_s": t_cpu, "gpu_steps_per_s": steps / t_graph, "gpu_events_per_s": events / t_graph,
          "cpu_events_per_s": events / t_cpu, "mean_rate_hz": spikes / 4000 / (steps * 1e-4)}
-    print(f"# coba C1 graph {t_graph*1e3:.2f} ms eager {t_eager*1e3:.2f} ms plain {t_plain*1e3:.2f} ms "
+    print(f"# coba C1 graph {t_graph*1e3:.2f} ms (multi-step launches {t_graph_multi*1e3:.2f} ms, one launch "
+          f"{t_one_launch*1e3:.2f} ms) eager {t_eager*1e3:.2f} ms plain {t_plain*1e3:.2f} ms "
           f"cpu {t_cpu*1e3:.1f} ms spikes {spikes} (cpu {cpu_spikes})", file=sys.stderr, flush=True)
     return r
 

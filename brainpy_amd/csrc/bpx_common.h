@@ -94,6 +94,7 @@ enum KernelId {
   KID_TILED = 11,
   KID_COO = 12,
   KID_BSR = 13,
+  KID_NET_RUN = 14,
   KID_COUNT = 16
 };
 

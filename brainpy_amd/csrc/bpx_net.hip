@@ -164,6 +164,164 @@ __global__ __launch_bounds__(T) void k_net_step(NetArgs a) {
   }
 }
 
+// Multi-step form: one block of kRunThreads threads per network advances it
+// K steps in one launch.  Each thread keeps kRunNpt neurons' state
+// (V, t_last, g_exc, g_inh) in registers; the count ring lives in LDS with
+// the excitatory and inhibitory counts of a neuron packed in one word (low
+// and high 16 bits; exact while no neuron receives 65536 synapses of one
+// kind, which the host checks), so delivery is LDS atomics and the two
+// phases of a step are separated by block barriers instead of launches.
+// The same integer counts and the same update as k_net_step: the spike
+// raster, state and counters equal K launches of the step kernel.
+constexpr int kRunThreads = 1024;
+constexpr int kRunNpt = 8;                 // networks up to 8192 neurons
+
+template <int CONN>
+__global__ __launch_bounds__(kRunThreads) void k_net_run(NetArgs a, uint8_t* raster, int64_t n_steps, int cap) {
+  extern __shared__ uint32_t s_mem[];
+  __shared__ int s_nspk[2];
+  __shared__ unsigned long long s_events;
+  const int nb = a.n_buf;
+  uint32_t* s_cnt = s_mem;                               // [n_buf][cap] packed counts
+  int* s_spk = (int*)(s_mem + (int64_t)nb * cap);        // [cap] this step's spiking neurons
+  const int net = blockIdx.x;
+  const bpx_net_desc d = a.nets[net];
+  const int N = d.n_exc + d.n_inh;
+  if (N > cap) return;                                   // (the host checks every network fits)
+  const int64_t total = a.total;
+  const int tid = threadIdx.x;
+  for (int b = 0; b < nb; ++b) {
+    const uint32_t* src = a.counts + (int64_t)b * 2 * total + d.off;
+    for (int j = tid; j < N; j += kRunThreads) s_cnt[b * cap + j] = src[j] | (src[total + j] << 16);
+  }
+  if (tid == 0) {
+    s_nspk[0] = 0;
+    s_nspk[1] = 0;
+    s_events = 0ull;
+  }
+  float V[kRunNpt], tl[kRunNpt], ge[kRunNpt], gi[kRunNpt];
+#pragma unroll
+  for (int u = 0; u < kRunNpt; ++u) {
+    const int j = tid + u * kRunThreads;
+    if (j < N) {
+      V[u] = a.V[d.off + j];
+      tl[u] = a.t_last[d.off + j];
+      ge[u] = a.g_exc[d.off + j];
+      gi[u] = a.g_inh[d.off + j];
+    }
+  }
+  int dly[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) dly[k] = d.delay[k] < 1 ? 1 : (d.delay[k] > nb - 1 ? nb - 1 : d.delay[k]);
+  const float tau_g[2] = {a.tau_e, a.tau_i};
+  const float E_g[2] = {a.E_e, a.E_i};
+  const float c_g[2] = {a.c_e, a.c_i};
+  const int64_t step0 = a.d_step ? *a.d_step + a.step : a.step;
+  const int grp = tid >> 5;
+  const uint32_t l = tid & 31;
+  unsigned long long ev = 0ull;
+  unsigned int spk_total = 0u;
+  __syncthreads();
+  for (int64_t k = 0; k < n_steps; ++k) {
+    const int64_t step = step0 + k;
+    const float t = (float)(step - d.t0_step) * a.dt;
+    uint32_t* cur = s_cnt + (step % nb) * cap;
+    int* nsp = s_nspk + (k & 1);
+    uint8_t* rs = raster ? raster + k * total + d.off : nullptr;
+    const bool last = k == n_steps - 1;
+#pragma unroll
+    for (int u = 0; u < kRunNpt; ++u) {
+      const int j = tid + u * kRunThreads;
+      if (j < N) {
+        const uint32_t c = cur[j];
+        cur[j] = 0u;
+        float g[2] = {ge[u], gi[u]};
+        if (c & 0xffffu) g[0] = g[0] + repeat_sum(c & 0xffffu, d.w_exc);
+        if (c >> 16) g[1] = g[1] + repeat_sum(c >> 16, d.w_inh);
+        const bool sp = lif_update<2>(V[u], tl[u], g, tau_g, E_g, c_g, d.I_ext, a.p, t, a.dt);
+        ge[u] = g[0];
+        gi[u] = g[1];
+        if (rs) rs[j] = sp ? 1 : 0;
+        if (last && a.spikes) a.spikes[d.off + j] = sp ? 1 : 0;
+        if (sp) s_spk[atomicAdd(nsp, 1)] = j;
+      }
+    }
+    __syncthreads();
+    const int nspk = *nsp;
+    if (tid == 0) {
+      s_nspk[(k + 1) & 1] = 0;                           // last read before this step's first barrier
+      spk_total += (unsigned int)nspk;
+    }
+    for (int e = grp; e < nspk; e += kRunThreads / 32) {
+      const int p = s_spk[e];
+      const bool exc = p < d.n_exc;
+      const uint32_t inc = exc ? 1u : 0x10000u;          // excitatory or inhibitory conductance
+      if constexpr (CONN == BPX_NET_JIT) {
+        const int col = exc ? p : p - d.n_exc;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int kk = exc ? q : 2 + q;                // E2E, E2I, I2E, I2I
+          const int64_t num_row = q == 0 ? d.n_exc : d.n_inh;
+          if (num_row <= 0) continue;
+          uint32_t* cnt = s_cnt + ((step + dly[kk]) % nb) * cap + (q == 0 ? 0 : d.n_exc);
+          const DivU32 dv = {d.div[kk][0], d.div[kk][1], d.div[kk][2], d.div[kk][3]};
+          const uint32_t rstep = (uint32_t)(((num_row + 1) >> 5) > 1 ? ((num_row + 1) >> 5) : 1);
+          uint32_t r = rstep * l - 1u;
+          uint32_t end = r + rstep;
+          if ((int64_t)end > num_row) end = (uint32_t)num_row;
+          Lfsr88 key(d.seed[kk] + (uint32_t)(col * 32) + l);
+          r += (uint32_t)rand_inc(key, dv);
+          while (r < end) {
+            atomicAdd(cnt + r, inc);
+            ++ev;
+            r += (uint32_t)rand_inc(key, dv);
+          }
+        }
+      } else {
+        uint32_t* cnt_e = s_cnt + ((step + dly[exc ? 0 : 2]) % nb) * cap;
+        uint32_t* cnt_i = s_cnt + ((step + dly[exc ? 1 : 3]) % nb) * cap;
+        const int32_t s = d.indptr[p], t2 = d.indptr[p + 1];
+        for (int32_t q = s + (int32_t)l; q < t2; q += 32) {
+          const int32_t tgt = d.indices[q];
+          if ((uint32_t)tgt < (uint32_t)N) {
+            atomicAdd((tgt < d.n_exc ? cnt_e : cnt_i) + tgt, inc);
+            ++ev;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < kRunNpt; ++u) {
+    const int j = tid + u * kRunThreads;
+    if (j < N) {
+      a.V[d.off + j] = V[u];
+      a.t_last[d.off + j] = tl[u];
+      a.g_exc[d.off + j] = ge[u];
+      a.g_inh[d.off + j] = gi[u];
+    }
+  }
+  for (int b = 0; b < nb; ++b) {
+    uint32_t* dst = a.counts + (int64_t)b * 2 * total + d.off;
+    for (int j = tid; j < N; j += kRunThreads) {
+      const uint32_t c = s_cnt[b * cap + j];
+      dst[j] = c & 0xffffu;
+      dst[total + j] = c >> 16;
+    }
+  }
+  if (a.events || a.spike_count) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ev += __shfl_xor(ev, o, 64);
+    if ((tid & 63) == 0 && ev) atomicAdd(&s_events, ev);
+    __syncthreads();
+    if (tid == 0) {
+      if (a.events && s_events) atomicAdd(a.events + net, s_events);
+      if (a.spike_count && spk_total) atomicAdd(a.spike_count + net, spk_total);
+    }
+  }
+}
+
 __global__ void k_net_advance(int64_t* d_step, int64_t delta) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *d_step += delta;
 }
@@ -172,31 +330,10 @@ __global__ void k_net_advance(int64_t* d_step, int64_t delta) {
 
 using namespace bpx;
 
-extern "C" {
-
-void bpx_net_prepare(bpx_net_desc* d) {
-  // divisor magic of each projection's clen (jitconn)
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t c = d->clen[k] > 0 ? (uint32_t)d->clen[k] : 1u;
-    const DivU32 dv = DivU32::make(c);
-    d->div[k][0] = dv.d;
-    d->div[k][1] = dv.m;
-    d->div[k][2] = dv.s1;
-    d->div[k][3] = dv.s2;
-  }
-}
-
-int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, int n_tiles, int tile_size,
-                 int conn_kind, int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts, int n_buf,
-                 uint8_t* spikes, unsigned long long* events, unsigned int* spike_count,
-                 const bpx_net_params* prm, int64_t step, const int64_t* d_step) {
-  if (!nets || !tiles || !prm) return BPX_ERR_ARG;
-  if (n_buf < 2 || (!d_step && step < 0)) return BPX_ERR_ARG;
-  if (n_tiles < 0 || total < 0) return BPX_ERR_SHAPE;
-  if (conn_kind != BPX_NET_JIT && conn_kind != BPX_NET_CSR) return BPX_ERR_ARG;
-  if (tile_size != 64 && tile_size != 128 && tile_size != 256 && tile_size != 512 && tile_size != 1024)
-    return BPX_ERR_ARG;
-  if (n_tiles == 0) return BPX_OK;
+static NetArgs net_args(const bpx_net_desc* nets, const int32_t* tiles, int64_t total, float* V, float* t_last,
+                        float* g_exc, float* g_inh, uint32_t* counts, int n_buf, uint8_t* spikes,
+                        unsigned long long* events, unsigned int* spike_count, const bpx_net_params* prm,
+                        int64_t step, const int64_t* d_step) {
   NetArgs a;
   a.nets = nets;
   a.tiles = tiles;
@@ -227,6 +364,43 @@ int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, i
   a.dt = prm->dt;
   a.step = step;
   a.d_step = d_step;
+  return a;
+}
+
+// LDS words of the multi-step kernel per network neuron: n_buf packed count
+// words and one spike-list entry; the block stays within 64 KiB
+static int64_t run_max_neurons(int n_buf) {
+  const int64_t lim = (65536 - 64) / (4 * ((int64_t)n_buf + 1));
+  return lim < (int64_t)kRunNpt * kRunThreads ? lim : (int64_t)kRunNpt * kRunThreads;
+}
+
+extern "C" {
+
+void bpx_net_prepare(bpx_net_desc* d) {
+  // divisor magic of each projection's clen (jitconn)
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t c = d->clen[k] > 0 ? (uint32_t)d->clen[k] : 1u;
+    const DivU32 dv = DivU32::make(c);
+    d->div[k][0] = dv.d;
+    d->div[k][1] = dv.m;
+    d->div[k][2] = dv.s1;
+    d->div[k][3] = dv.s2;
+  }
+}
+
+int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, int n_tiles, int tile_size,
+                 int conn_kind, int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts, int n_buf,
+                 uint8_t* spikes, unsigned long long* events, unsigned int* spike_count,
+                 const bpx_net_params* prm, int64_t step, const int64_t* d_step) {
+  if (!nets || !tiles || !prm) return BPX_ERR_ARG;
+  if (n_buf < 2 || (!d_step && step < 0)) return BPX_ERR_ARG;
+  if (n_tiles < 0 || total < 0) return BPX_ERR_SHAPE;
+  if (conn_kind != BPX_NET_JIT && conn_kind != BPX_NET_CSR) return BPX_ERR_ARG;
+  if (tile_size != 64 && tile_size != 128 && tile_size != 256 && tile_size != 512 && tile_size != 1024)
+    return BPX_ERR_ARG;
+  if (n_tiles == 0) return BPX_OK;
+  const NetArgs a = net_args(nets, tiles, total, V, t_last, g_exc, g_inh, counts, n_buf, spikes, events,
+                             spike_count, prm, step, d_step);
   hipStream_t s = (hipStream_t)stream;
 #define NET_GO(T_)                                                                                     \
   do {                                                                                                 \
@@ -243,6 +417,33 @@ int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, i
     default: NET_GO(1024); break;
   }
 #undef NET_GO
+  BPX_CHECK_LAUNCH();
+  return BPX_OK;
+}
+
+int64_t bpx_net_run_max_neurons(int n_buf) { return n_buf < 2 ? 0 : run_max_neurons(n_buf); }
+
+int bpx_net_run(void* stream, const bpx_net_desc* nets, int n_nets, int max_neurons, int max_in_degree,
+                int conn_kind, int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts,
+                int n_buf, uint8_t* raster, uint8_t* spikes, unsigned long long* events, unsigned int* spike_count,
+                const bpx_net_params* prm, int64_t step, const int64_t* d_step, int64_t n_steps) {
+  if (!nets || !prm) return BPX_ERR_ARG;
+  if (n_buf < 2 || (!d_step && step < 0) || n_steps < 0) return BPX_ERR_ARG;
+  if (n_nets < 0 || total < 0) return BPX_ERR_SHAPE;
+  if (conn_kind != BPX_NET_JIT && conn_kind != BPX_NET_CSR) return BPX_ERR_ARG;
+  if (n_nets == 0 || n_steps == 0) return BPX_OK;
+  if (max_neurons < 1 || max_neurons > run_max_neurons(n_buf)) return BPX_ERR_SHAPE;
+  if (max_in_degree < 0 || max_in_degree > 65535) return BPX_ERR_SHAPE;
+  const NetArgs a = net_args(nets, nullptr, total, V, t_last, g_exc, g_inh, counts, n_buf, spikes, events,
+                             spike_count, prm, step, d_step);
+  const size_t shm = (size_t)4 * ((size_t)n_buf + 1) * (size_t)max_neurons;
+  hipStream_t s = (hipStream_t)stream;
+  if (conn_kind == BPX_NET_JIT)
+    BPX_LAUNCH(KID_NET_RUN, (k_net_run<BPX_NET_JIT>), dim3((unsigned)n_nets), dim3(kRunThreads), shm, s, a, raster,
+               n_steps, max_neurons);
+  else
+    BPX_LAUNCH(KID_NET_RUN, (k_net_run<BPX_NET_CSR>), dim3((unsigned)n_nets), dim3(kRunThreads), shm, s, a, raster,
+               n_steps, max_neurons);
   BPX_CHECK_LAUNCH();
   return BPX_OK;
 }

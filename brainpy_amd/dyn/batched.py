@@ -1,5 +1,6 @@
 """Batched E/I COBA networks: many independent networks advanced by one
-launch per step (``bpx_net_step``).
+launch per step (``bpx_net_step``), or, for networks of a few thousand
+neurons, by one launch for many steps (``bpx_net_run``).
 
 Each network is the ``EICOBA_PostAlign`` model of
 examples/dynamics_simulation/COBA.py:58-109 (LifRefLTC or LifRef neurons,
@@ -56,6 +57,14 @@ def _register():
         L.bpx_net_step.restype = ctypes.c_int
         L.bpx_net_advance.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
         L.bpx_net_advance.restype = ctypes.c_int
+        L.bpx_net_run_max_neurons.argtypes = [ctypes.c_int]
+        L.bpx_net_run_max_neurons.restype = ctypes.c_int64
+        L.bpx_net_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(NetParams), ctypes.c_int64,
+                                  ctypes.c_void_p, ctypes.c_int64]
+        L.bpx_net_run.restype = ctypes.c_int
         L._bpx_net_registered = True
     return L
 
@@ -137,6 +146,7 @@ class BatchedCOBA:
         self._keep = []
         off = 0
         max_delay = 1
+        in_deg = 0
         for k, sp in enumerate(specs):
             d = self.descs[k]
             ne, ni = int(sp["n_exc"]), int(sp["n_inh"])
@@ -153,8 +163,13 @@ class BatchedCOBA:
                 for q in range(4):
                     d.seed[q] = (base * 4 + q) & 0xFFFFFFFF
                     d.clen[q] = c
+                in_deg = max(in_deg, ne, ni)    # each projection draws a pair at most once
             else:
                 idx, ptr = combined_csr(sp["csr"], ne, ni)
+                for rows in (idx[ptr[0]:ptr[ne]], idx[ptr[ne]:ptr[N]]):
+                    rows = rows[(rows >= 0) & (rows < N)]
+                    if rows.size:
+                        in_deg = max(in_deg, int(np.bincount(rows, minlength=N).max()))
                 ti = torch.from_numpy(idx).to(self.dev)
                 tp = torch.from_numpy(ptr).to(self.dev)
                 self._keep += [ti, tp]
@@ -169,6 +184,10 @@ class BatchedCOBA:
             off += N
         self.total = off
         self.n_buf = max_delay + 1             # count buffers: one per step of the longest delay, plus one
+        # the multi-step launch (one block per network, counts in LDS) serves
+        # batches whose networks and in-degrees fit it
+        self.max_in_degree = in_deg
+        self.fused = (max(self.sizes) <= L.bpx_net_run_max_neurons(self.n_buf) and in_deg <= 65535)
         self.d_descs = torch.frombuffer(bytearray(bytes(self.descs)), dtype=torch.uint8).to(self.dev)
         self.set_tile(tile if tile is not None else _TUNED.get(self._tune_key(), TILE))
         self.params = NetParams(V_rest=V_rest, V_th=V_th, V_reset=V_reset, tau=tau, tau_ref=tau_ref, R=R,
@@ -293,18 +312,48 @@ class BatchedCOBA:
         _lib.check(rc, "bpx_net_step")
         self.i += 1
 
-    def run(self, steps, raster=False):
-        """Advance `steps` steps; with `raster` returns uint8 [steps, total]."""
+    def run_fused(self, steps, raster=None, device_offset=None):
+        """Advance `steps` steps in one launch (``bpx_net_run``: one block per
+        network); same results as `steps` calls of :meth:`step`.  `raster`
+        (uint8 [steps, total]) receives every step's spike flags; without it
+        a recording batch keeps the last step's flags in ``spikes``, as
+        :meth:`step` does."""
+        if not self.fused:
+            raise ValueError("the multi-step launch needs networks of at most "
+                             f"{_lib.lib().bpx_net_run_max_neurons(self.n_buf)} neurons and in-degrees below 65536")
+        if raster is not None and (raster.dtype != torch.uint8 or not raster.is_contiguous()
+                                   or tuple(raster.shape) != (steps, self.total)):
+            raise ValueError(f"raster must be a contiguous uint8 tensor of shape ({steps}, {self.total})")
+        L = _lib.lib()
+        dev = device_offset is not None
+        rc = L.bpx_net_run(_lib.stream_ptr(), _lib.ptr(self.d_descs), len(self.specs), max(self.sizes),
+                           self.max_in_degree, self.kind, self.total, _lib.ptr(self.V), _lib.ptr(self.t_last),
+                           _lib.ptr(self.g_exc), _lib.ptr(self.g_inh), _lib.ptr(self.counts), self.n_buf,
+                           _lib.ptr(raster), _lib.ptr(self.spikes) if (self.record and raster is None) else None, _lib.ptr(self.events),
+                           _lib.ptr(self.spike_count), ctypes.byref(self.params),
+                           device_offset if dev else self.i, _lib.ptr(self.d_step) if dev else None, steps)
+        _lib.check(rc, "bpx_net_run")
+        self.i += steps
+
+    def run(self, steps, raster=False, fused=None):
+        """Advance `steps` steps; with `raster` returns uint8 [steps, total].
+        `fused` (default: when the batch fits) runs them as one launch."""
         out = torch.empty((steps, self.total), dtype=torch.uint8, device=self.dev) if raster else None
+        if (self.fused if fused is None else fused) and steps > 0:
+            self.run_fused(steps, out)
+            return out
         for k in range(steps):
             self.step(out[k] if raster else None)
         return out
 
-    def capture(self, steps):
+    def capture(self, steps, fused=False):
         """Capture `steps` steps in one HIP graph (one batched launch per step
-        at offsets 0..steps-1 from the device base step, then one launch that
-        advances the base by `steps`); replay() advances the batch by `steps`
-        steps per call, continuing in time."""
+        at offsets 0..steps-1 from the device base step, or with `fused` one
+        multi-step launch, then one launch that advances the base by
+        `steps`); replay() advances the batch by `steps` steps per call,
+        continuing in time.  Replayed, the per-step launches are as fast as
+        the multi-step launch, whose single block per network is bound by
+        the neuron update's arithmetic on one compute unit."""
         self.d_step.fill_(self.i)
         torch.cuda.synchronize()
         i0 = self.i
@@ -313,8 +362,11 @@ class BatchedCOBA:
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
             with torch.cuda.graph(g, stream=s):
-                for k in range(steps):
-                    self.step(device_offset=k)
+                if fused:
+                    self.run_fused(steps, device_offset=0)
+                else:
+                    for k in range(steps):
+                        self.step(device_offset=k)
                 _lib.check(_lib.lib().bpx_net_advance(_lib.stream_ptr(), _lib.ptr(self.d_step), steps),
                            "bpx_net_advance")
         torch.cuda.current_stream().wait_stream(s)

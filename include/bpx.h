@@ -288,6 +288,22 @@ int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, i
                  uint8_t* spikes, unsigned long long* events, unsigned int* spike_count,
                  const bpx_net_params* params, int64_t step, const int64_t* d_step);
 int bpx_net_advance(void* stream, int64_t* d_step, int64_t delta);
+/* Multi-step form of bpx_net_step (same reference counterpart): advances
+ * every network of `nets` (one block each, n_nets of them) n_steps steps in
+ * one launch, from step `step` (or *d_step + step), with the same results as
+ * n_steps calls of bpx_net_step: spike raster, state, counts, events and
+ * spike counts.  For small networks, where the per-step launch dominates:
+ * every network has at most max_neurons <= bpx_net_run_max_neurons(n_buf)
+ * neurons (5456 at n_buf = 2), and no neuron receives more than
+ * max_in_degree <= 65535 synapses from one population (counts are kept as
+ * 16-bit halves in LDS); larger values return BPX_ERR_SHAPE.  raster
+ * (nullable) receives [n_steps][total] spike flags, spikes (nullable) the
+ * flags of the last step. */
+int64_t bpx_net_run_max_neurons(int n_buf);
+int bpx_net_run(void* stream, const bpx_net_desc* nets, int n_nets, int max_neurons, int max_in_degree,
+                int conn_kind, int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts,
+                int n_buf, uint8_t* raster, uint8_t* spikes, unsigned long long* events, unsigned int* spike_count,
+                const bpx_net_params* params, int64_t step, const int64_t* d_step, int64_t n_steps);
 
 /* ---------------------------------------------------------------------------
  * Event-driven STDP weight updates.

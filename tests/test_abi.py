@@ -205,3 +205,31 @@ def test_null_arrays_are_rejected_before_launch():
     assert L.bpx_coo_matmat(None, d, F32, 1, d, d, I32, 10, d, F32, 4, 5, 3, 1, None) == ARG      # out
     assert L.bpx_event_info(None, None, BOOL, 1, 100, d, d, d, 1 << 20) == ARG                    # events
     assert L.bpx_jitconn_mv(None, None, BOOL, 1, 0, 1.0, 0.0, 3, 1, 10, 10, 1, 0, d, None, 0) == ARG
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libbpx.so not built")
+def test_argument_validation_of_net_run():
+    # the multi-step network launch: LDS limit per neuron and per in-degree,
+    # rejected before launch; no steps or no networks is no work
+    from brainpy_amd.dyn.batched import NetParams, _register
+    L = _register()
+    SHAPE, ARG = -2, -5
+    assert L.bpx_net_run_max_neurons(2) == 5456       # (65536 - 64) / (4 * (n_buf + 1))
+    assert L.bpx_net_run_max_neurons(26) == 606
+    assert L.bpx_net_run_max_neurons(1) == 0
+    d = ctypes.c_void_p(16)   # never dereferenced
+    prm = NetParams()
+
+    def run(nets=d, n_nets=1, max_n=4000, in_deg=80, n_buf=2, steps=10, p=ctypes.byref(prm)):
+        return L.bpx_net_run(None, nets, n_nets, max_n, in_deg, 1, 4000, d, d, d, d, d, n_buf, None, None, None,
+                             None, p, 0, None, steps)
+
+    assert run(nets=None) == ARG
+    assert run(p=None) == ARG
+    assert run(n_buf=1) == ARG
+    assert run(steps=-1) == ARG
+    assert run(max_n=5457) == SHAPE
+    assert run(max_n=0) == SHAPE
+    assert run(in_deg=65536) == SHAPE
+    assert run(steps=0) == 0
+    assert run(n_nets=0) == 0

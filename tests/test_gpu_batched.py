@@ -70,27 +70,30 @@ def _oracle_jit(batch, k, steps, delays=(1, 1, 1, 1)):
     return raster
 
 
+@pytest.mark.parametrize("fused", [False, True])
 @pytest.mark.parametrize("ltc", [True, False])
-def test_jit_batch_vs_oracle(ltc):
+def test_jit_batch_vs_oracle(ltc, fused):
     batch = BatchedCOBA(_jit_specs([1000, 1500, 4000]), ltc=ltc)
-    raster = batch.run(100, raster=True).cpu().numpy()
+    assert batch.fused
+    raster = batch.run(100, raster=True, fused=fused).cpu().numpy()
     for k in range(3):
         ref = _oracle_jit(batch, k, 100)
         assert ref.sum() > 0
         np.testing.assert_array_equal(raster[:, batch.net_slice(k)], ref)
 
 
-def test_alone_equals_in_mix_and_graph():
+@pytest.mark.parametrize("fused", [False, True])
+def test_alone_equals_in_mix_and_graph(fused):
     specs = _jit_specs([1200, 3000, 800, 5000, 2000], seed0=7)
     mix = BatchedCOBA(specs)
-    r_mix = mix.run(60, raster=True).cpu().numpy()
+    r_mix = mix.run(60, raster=True, fused=fused).cpu().numpy()
     for k in (0, 3):
         alone = BatchedCOBA([specs[k]], V_init=mix.V0[mix.net_slice(k)])
-        r_alone = alone.run(60, raster=True).cpu().numpy()
+        r_alone = alone.run(60, raster=True, fused=fused).cpu().numpy()
         np.testing.assert_array_equal(r_mix[:, mix.net_slice(k)], r_alone)
     # graph capture: 3 replays of 20 steps == 60 eager steps
     g = BatchedCOBA(specs)
-    g.capture(20)
+    g.capture(20, fused=fused)
     for _ in range(3):
         g.replay()
         torch.cuda.synchronize()
@@ -110,18 +113,18 @@ def test_tile_sizes_same_raster_and_autotune_restores(kind):
         csrs = {name: (net.csr[name][1], net.csr[name][2]) for name in ("E2E", "E2I", "I2E", "I2I")}
         specs = [dict(n_exc=net.n_exc, n_inh=net.n_inh, seed=5, w_exc=0.6, w_inh=6.7, I_ext=20.0, csr=csrs)]
     ref = BatchedCOBA(specs, tile=256)
-    r_ref = ref.run(50, raster=True).cpu().numpy()
+    r_ref = ref.run(50, raster=True, fused=False).cpu().numpy()
     assert r_ref.sum() > 0
     for tile in (64, 128, 512, 1024):
         b = BatchedCOBA(specs, tile=tile, V_init=ref.V0)
-        np.testing.assert_array_equal(b.run(50, raster=True).cpu().numpy(), r_ref)
+        np.testing.assert_array_equal(b.run(50, raster=True, fused=False).cpu().numpy(), r_ref)
     # autotune runs candidates from a snapshot and restores it: 20 steps,
     # tune, 30 more steps == 50 steps
     b = BatchedCOBA(specs, V_init=ref.V0)
-    r1 = b.run(20, raster=True).cpu().numpy()
+    r1 = b.run(20, raster=True, fused=False).cpu().numpy()
     times = b.autotune(steps=5, repeats=1)
     assert set(times) == {64, 128, 256, 512, 1024} and b.tile == min(times, key=times.get)
-    r2 = b.run(30, raster=True).cpu().numpy()
+    r2 = b.run(30, raster=True, fused=False).cpu().numpy()
     np.testing.assert_array_equal(np.concatenate([r1, r2]), r_ref)
     np.testing.assert_array_equal(b.V.cpu().numpy(), ref.V.cpu().numpy())
     assert int(b.events.sum()) == int(ref.events.sum())
@@ -137,12 +140,14 @@ def test_bf16_weights_rounded():
     b.run(20)
 
 
-def test_csr_network_vs_eicoba_oracle():
+@pytest.mark.parametrize("fused", [False, True])
+def test_csr_network_vs_eicoba_oracle(fused):
     net = EICOBANet(seed=5, mode="fixed_order")
     csrs = {name: (net.csr[name][1], net.csr[name][2]) for name in ("E2E", "E2I", "I2E", "I2I")}
     spec = dict(n_exc=net.n_exc, n_inh=net.n_inh, seed=5, w_exc=0.6, w_inh=6.7, I_ext=20.0, csr=csrs)
     b = BatchedCOBA([spec], V_init=net.V0)
-    raster = b.run(100, raster=True).cpu().numpy()
+    assert b.fused
+    raster = b.run(100, raster=True, fused=fused).cpu().numpy()
     ref = net.run(100).cpu().numpy()
     assert ref.sum() > 0
     np.testing.assert_array_equal(raster, ref)
@@ -159,8 +164,9 @@ def test_coba4000_fixture_first_100_steps():
     np.testing.assert_array_equal(net.run(100).cpu().numpy(), ref)
     csrs = {name: (net.csr[name][1], net.csr[name][2]) for name in ("E2E", "E2I", "I2E", "I2I")}
     spec = dict(n_exc=3200, n_inh=800, seed=0, w_exc=0.6, w_inh=6.7, I_ext=20.0, csr=csrs)
-    b = BatchedCOBA([spec], V_init=net.V0)
-    np.testing.assert_array_equal(b.run(100, raster=True).cpu().numpy(), ref)
+    for fused in (False, True):
+        b = BatchedCOBA([spec], V_init=net.V0)
+        np.testing.assert_array_equal(b.run(100, raster=True, fused=fused).cpu().numpy(), ref)
 
 
 def test_plain_path_equals_batched():
@@ -299,3 +305,48 @@ def test_csr_delays_vs_oracle_and_graph():
         g.replay()
     torch.cuda.synchronize()
     np.testing.assert_array_equal(g.V.cpu().numpy(), b.V.cpu().numpy())
+
+
+def _state(b):
+    return [t.cpu().numpy().copy() for t in (b.V, b.t_last, b.g_exc, b.g_inh, b.counts, b.events, b.spike_count,
+                                             b.spikes)]
+
+
+@pytest.mark.parametrize("kind", ["jit", "csr"])
+def test_fused_run_equals_step_launches(kind):
+    # the multi-step launch (one block per network, counts in LDS) leaves the
+    # same raster, state, count ring, event and spike totals as one launch per
+    # step, with delays, interleaved with step launches, and under capture
+    if kind == "jit":
+        specs = _jit_specs([600, 2700, 1300], seed0=41)
+        specs[0]["delay"] = [0.3, 0.1, 0.2, 0.0]      # 3, 1, 2, 1 steps
+    else:
+        net = EICOBANet(seed=3, mode="fixed_order")
+        csrs = {name: (net.csr[name][1], net.csr[name][2]) for name in ("E2E", "E2I", "I2E", "I2I")}
+        specs = [dict(n_exc=net.n_exc, n_inh=net.n_inh, seed=3, w_exc=0.6, w_inh=6.7, I_ext=20.0, csr=csrs,
+                      delay=[0.2, 0.1, 0.0, 0.2])]     # 2, 1, 1, 2 steps
+    ref = BatchedCOBA(specs, record=True)
+    assert ref.n_buf == (4 if kind == "jit" else 3) and ref.fused
+    r_ref = ref.run(90, raster=True, fused=False).cpu().numpy()
+    assert r_ref.sum() > 0
+    b = BatchedCOBA(specs, record=True, V_init=ref.V0)
+    parts = [b.run(30, raster=True, fused=True), b.run(20, raster=True, fused=False),
+             b.run(40, raster=True, fused=True)]
+    np.testing.assert_array_equal(torch.cat(parts).cpu().numpy(), r_ref)
+    for x, y in zip(_state(b), _state(ref)):
+        np.testing.assert_array_equal(x, y)
+    assert b.i == ref.i == 90
+    # without a raster a recording batch keeps the last step's flags
+    ref.run(7, fused=False)
+    b.run(7, fused=True)
+    for x, y in zip(_state(b), _state(ref)):
+        np.testing.assert_array_equal(x, y)
+    # captured multi-step launch: 3 replays of 30 steps
+    g = BatchedCOBA(specs, record=True, V_init=ref.V0)
+    g.capture(30, fused=True)
+    for _ in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+    g.run(7, fused=False)
+    for x, y in zip(_state(g), _state(ref)):
+        np.testing.assert_array_equal(x, y)
