@@ -350,3 +350,23 @@ def test_fused_run_equals_step_launches(kind):
     g.run(7, fused=False)
     for x, y in zip(_state(g), _state(ref)):
         np.testing.assert_array_equal(x, y)
+
+
+def test_fused_run_at_the_lds_limit_and_fallback():
+    # a network of exactly bpx_net_run_max_neurons(2) neurons runs fused; a
+    # batch with a larger network keeps per-step launches
+    from brainpy_amd import _lib
+    cap = int(_lib.lib().bpx_net_run_max_neurons(2))
+    assert cap == 5456
+    specs = _jit_specs([cap, 900], seed0=77)
+    b = BatchedCOBA(specs)
+    assert b.fused and b.n_buf == 2
+    ref = BatchedCOBA(specs, V_init=b.V0)
+    r = b.run(40, raster=True, fused=True).cpu().numpy()
+    np.testing.assert_array_equal(r, ref.run(40, raster=True, fused=False).cpu().numpy())
+    assert r.sum() > 0
+    big = BatchedCOBA(_jit_specs([cap + 1, 900], seed0=77))
+    assert not big.fused
+    with pytest.raises(ValueError):
+        big.run_fused(5)
+    big.run(5)      # per-step launches
