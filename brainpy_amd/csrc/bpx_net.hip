@@ -399,6 +399,7 @@ int bpx_net_step(void* stream, const bpx_net_desc* nets, const int32_t* tiles, i
   if (tile_size != 64 && tile_size != 128 && tile_size != 256 && tile_size != 512 && tile_size != 1024)
     return BPX_ERR_ARG;
   if (n_tiles == 0) return BPX_OK;
+  if (!V || !t_last || !g_exc || !g_inh || !counts) return BPX_ERR_ARG;
   const NetArgs a = net_args(nets, tiles, total, V, t_last, g_exc, g_inh, counts, n_buf, spikes, events,
                              spike_count, prm, step, d_step);
   hipStream_t s = (hipStream_t)stream;
@@ -432,6 +433,7 @@ int bpx_net_run(void* stream, const bpx_net_desc* nets, int n_nets, int max_neur
   if (n_nets < 0 || total < 0) return BPX_ERR_SHAPE;
   if (conn_kind != BPX_NET_JIT && conn_kind != BPX_NET_CSR) return BPX_ERR_ARG;
   if (n_nets == 0 || n_steps == 0) return BPX_OK;
+  if (!V || !t_last || !g_exc || !g_inh || !counts) return BPX_ERR_ARG;
   if (max_neurons < 1 || max_neurons > run_max_neurons(n_buf)) return BPX_ERR_SHAPE;
   if (max_in_degree < 0 || max_in_degree > 65535) return BPX_ERR_SHAPE;
   const NetArgs a = net_args(nets, nullptr, total, V, t_last, g_exc, g_inh, counts, n_buf, spikes, events,

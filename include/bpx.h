@@ -256,7 +256,8 @@ float bpx_expon_coef(float tau, float dt);
  * k = E2E, E2I, I2E, I2I with matrix shape (post, pre); CSR networks give one
  * row per neuron listing local post ids (E targets of E neurons feed the
  * excitatory conductance, targets of I neurons the inhibitory one).
- * spikes/events/spike_count are optional outputs.
+ * spikes/events/spike_count are optional outputs; null state arrays (V,
+ * t_last, g_exc, g_inh, counts) with work to do return BPX_ERR_ARG.
  */
 #define BPX_NET_JIT 0
 #define BPX_NET_CSR 1

@@ -220,8 +220,8 @@ def test_argument_validation_of_net_run():
     d = ctypes.c_void_p(16)   # never dereferenced
     prm = NetParams()
 
-    def run(nets=d, n_nets=1, max_n=4000, in_deg=80, n_buf=2, steps=10, p=ctypes.byref(prm)):
-        return L.bpx_net_run(None, nets, n_nets, max_n, in_deg, 1, 4000, d, d, d, d, d, n_buf, None, None, None,
+    def run(nets=d, n_nets=1, max_n=4000, in_deg=80, n_buf=2, steps=10, p=ctypes.byref(prm), V=d):
+        return L.bpx_net_run(None, nets, n_nets, max_n, in_deg, 1, 4000, V, d, d, d, d, n_buf, None, None, None,
                              None, p, 0, None, steps)
 
     assert run(nets=None) == ARG
@@ -231,5 +231,9 @@ def test_argument_validation_of_net_run():
     assert run(max_n=5457) == SHAPE
     assert run(max_n=0) == SHAPE
     assert run(in_deg=65536) == SHAPE
+    assert run(V=None) == ARG
     assert run(steps=0) == 0
     assert run(n_nets=0) == 0
+    tiles = ctypes.c_void_p(32)
+    assert L.bpx_net_step(None, d, tiles, 16, 256, 1, 4000, None, d, d, d, d, 2, None, None, None,
+                          ctypes.byref(prm), 0, None) == ARG                       # null state
