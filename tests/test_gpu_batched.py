@@ -370,3 +370,28 @@ def test_fused_run_at_the_lds_limit_and_fallback():
     with pytest.raises(ValueError):
         big.run_fused(5)
     big.run(5)      # per-step launches
+
+
+def test_coba4000_5000_steps_every_path_vs_oracle():
+    # long horizon: the 4000-neuron network for 5000 steps (0.5 s of model
+    # time), per-step launches, the multi-step launch and a captured graph of
+    # multi-step launches against the serial CPU restatement, spike for spike
+    from brainpy_amd.dyn.batched import combined_csr
+    net = EICOBANet(seed=0, mode="atomic")
+    csrs = {name: (net.csr[name][1], net.csr[name][2]) for name in ("E2E", "E2I", "I2E", "I2I")}
+    spec = dict(n_exc=3200, n_inh=800, seed=0, w_exc=0.6, w_inh=6.7, I_ext=20.0, csr=csrs)
+    idx, ptr = combined_csr(csrs, 3200, 800)
+    steps = 5000
+    total, ref = orc.coba_net_run(3200, 800, ptr, idx, 0.6, 6.7, 20.0, net.V0, steps, PARAMS)
+    assert total > 20000
+    for fused in (False, True):
+        b = BatchedCOBA([spec], V_init=net.V0)
+        np.testing.assert_array_equal(b.run(steps, raster=True, fused=fused).cpu().numpy(), ref)
+        assert int(b.spike_count.sum()) == total
+    g = BatchedCOBA([spec], V_init=net.V0, record=True)
+    g.capture(500, fused=True)
+    for _ in range(steps // 500):
+        g.replay()
+    torch.cuda.synchronize()
+    assert int(g.spike_count.sum()) == total
+    np.testing.assert_array_equal(g.spikes.cpu().numpy(), ref[-1])
