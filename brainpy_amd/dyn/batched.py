@@ -119,6 +119,21 @@ def combined_csr(csrs, n_exc, n_inh):
     return np.concatenate(rows).astype(np.int32), indptr.astype(np.int32)
 
 
+def max_in_degree(indices, indptr, n_exc, n_inh):
+    """Largest number of synapses one neuron receives from one population
+    (E rows, then I rows) of a combined network CSR, duplicates counted and
+    out-of-range targets (which delivery skips) ignored: the largest count a
+    neuron can take in one step."""
+    N = n_exc + n_inh
+    best = 0
+    for rows in (indices[indptr[0]:indptr[n_exc]], indices[indptr[n_exc]:indptr[N]]):
+        rows = np.asarray(rows, np.int64)
+        rows = rows[(rows >= 0) & (rows < N)]
+        if rows.size:
+            best = max(best, int(np.bincount(rows, minlength=N).max()))
+    return best
+
+
 class BatchedCOBA:
     """A batch of independent COBA networks sharing neuron parameters.
 
@@ -166,10 +181,7 @@ class BatchedCOBA:
                 in_deg = max(in_deg, ne, ni)    # each projection draws a pair at most once
             else:
                 idx, ptr = combined_csr(sp["csr"], ne, ni)
-                for rows in (idx[ptr[0]:ptr[ne]], idx[ptr[ne]:ptr[N]]):
-                    rows = rows[(rows >= 0) & (rows < N)]
-                    if rows.size:
-                        in_deg = max(in_deg, int(np.bincount(rows, minlength=N).max()))
+                in_deg = max(in_deg, max_in_degree(idx, ptr, ne, ni))
                 ti = torch.from_numpy(idx).to(self.dev)
                 tp = torch.from_numpy(ptr).to(self.dev)
                 self._keep += [ti, tp]
