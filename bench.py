@@ -583,9 +583,48 @@ def bench_coba(_lib, torch, steps=1000):
     return r
 
 
-def _c4_specs(n_nets=256, seed=2024):
+def bench_small_batch(torch, n_nets=512, steps=1000):
+    """A batch of 512 small jitconn networks (N log-uniform in [1k, 5k],
+    in-degree 80, FP32 weights), 1000 steps in HIP graphs of 100 steps: one
+    launch per step for all networks against one multi-step launch (one
+    block per network, counts in LDS) per 100 steps; same spike totals."""
+    from brainpy_amd.dyn.batched import BatchedCOBA
+    specs = _c4_specs(n_nets, seed=77, lo=1000, hi=5000)
+    out = {}
+    for fused in (False, True):
+        b = BatchedCOBA(specs)
+        assert b.fused
+        b.capture(100, fused=fused)
+        b.replay()
+        torch.cuda.synchronize()
+        b.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps // 100):
+            b.replay()
+        torch.cuda.synchronize()
+        out[fused] = (time.perf_counter() - t0, int(b.spike_count.sum()), int(b.events.sum()), b.total)
+        del b
+    (t_step, sp, ev, total), (t_multi, sp2, ev2, _) = out[False], out[True]
+    assert sp == sp2 and ev == ev2
+    b = BatchedCOBA(specs)
+    tune = b.autotune(steps=200)     # tiles and the multi-step launch, from the fresh state (restored)
+    picks_fused = bool(b.use_fused)
+    del b
+    r = {"workload": f"batched_coba_{n_nets}nets_1k-5k_jit80_{steps}steps", "networks": n_nets,
+         "total_neurons": total, "spikes": sp, "synaptic_events": ev, "per_step_launch_graph_s": t_step,
+         "multi_step_launch_graph_s": t_multi, "speedup": t_step / t_multi,
+         "autotune_us_per_step": {str(k): v for k, v in tune.items()}, "autotune_picks_multi_step": picks_fused,
+         "us_per_step": {"per_step_launches": t_step / steps * 1e6, "multi_step": t_multi / steps * 1e6}}
+    print(f"# small batch {n_nets} nets ({total} neurons): per-step launches {t_step*1e3:.1f} ms, multi-step "
+          f"{t_multi*1e3:.1f} ms -> x{t_step/t_multi:.2f}; autotune picks "
+          f"{'multi-step' if picks_fused else 'per-step'}", file=sys.stderr, flush=True)
+    return r
+
+
+def _c4_specs(n_nets=256, seed=2024, lo=1000, hi=64000):
     rng = np.random.Generator(np.random.PCG64(seed))
-    sizes = np.exp(rng.uniform(np.log(1000), np.log(64000), n_nets)).astype(np.int64)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_nets)).astype(np.int64)
     specs = []
     for k, N in enumerate(sizes):
         ne = int(round(N * 0.8))
@@ -811,7 +850,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the jitconn and network legs")
-    ap.add_argument("--only", choices=["jitconn", "batched", "coba", "served", "coo", "forms", "conn", "stdp", "bsr"],
+    ap.add_argument("--only", choices=["jitconn", "batched", "coba", "smallbatch", "served", "coo", "forms", "conn", "stdp",
+                                       "bsr"],
                     default=None,
                     help="run only this leg (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -831,7 +871,8 @@ def main():
     if args.only:
         fn = {"jitconn": lambda: bench_jitconn(L, _lib, torch, max(10, args.steps // 10), 3),
               "batched": lambda: bench_batched(_lib, torch, plain_sample=2, plain_steps=20),
-              "coba": lambda: bench_coba(_lib, torch), "served": lambda: bench_served(_lib, torch),
+              "coba": lambda: bench_coba(_lib, torch), "smallbatch": lambda: bench_small_batch(torch),
+              "served": lambda: bench_served(_lib, torch),
               "coo": lambda: bench_coo(_lib, torch), "forms": lambda: bench_forms(_lib, torch),
               "conn": lambda: bench_conn(_lib, torch), "stdp": lambda: bench_stdp(_lib, torch),
               "bsr": lambda: bench_bsr(_lib, torch)}[args.only]
@@ -1025,6 +1066,7 @@ def main():
     if not args.no_extra:
         extra["jitconn"] = bench_jitconn(L, _lib, torch, max(10, args.steps // 10), 3, cpu=not args.no_cpu)
         extra["coba"] = bench_coba(_lib, torch)
+        extra["small_batch"] = bench_small_batch(torch)
         extra["batched"] = bench_batched(_lib, torch)
         extra["served"] = bench_served(_lib, torch)
         extra["coo"] = bench_coo(_lib, torch)

@@ -174,4 +174,4 @@ def timer_read(kernel_id):
 
 
 KID = dict(event_scatter=0, event_gather=1, csr_scatter=2, csr_gather=3, jit_outdim=4, jit_scatter=5, info=6,
-           lif=7, fixed_order=8, copy=9, net=10, tiled=11, coo=12, bsr=13)
+           lif=7, fixed_order=8, copy=9, net=10, tiled=11, coo=12, bsr=13, net_run=14)

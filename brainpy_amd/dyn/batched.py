@@ -28,6 +28,7 @@ NET_JIT, NET_CSR = 0, 1
 TILE = 256                                  # default neurons per block of the step launch
 TILE_CHOICES = (64, 128, 256, 512, 1024)
 _TUNED = {}                                 # (kind, networks, size class) -> tile chosen by autotune()
+_TUNED_FUSED = {}                           # same key -> autotune() found the multi-step launch faster
 
 
 class NetDesc(ctypes.Structure):
@@ -202,6 +203,7 @@ class BatchedCOBA:
         self.fused = (max(self.sizes) <= L.bpx_net_run_max_neurons(self.n_buf) and in_deg <= 65535)
         self.d_descs = torch.frombuffer(bytearray(bytes(self.descs)), dtype=torch.uint8).to(self.dev)
         self.set_tile(tile if tile is not None else _TUNED.get(self._tune_key(), TILE))
+        self.use_fused = self.fused and _TUNED_FUSED.get(self._tune_key(), False)
         self.params = NetParams(V_rest=V_rest, V_th=V_th, V_reset=V_reset, tau=tau, tau_ref=tau_ref, R=R,
                                 tau_e=tau_e, tau_i=tau_i, E_e=E_e, E_i=E_i, c_e=float(expon_coef(tau_e, dt)),
                                 c_i=float(expon_coef(tau_i, dt)), dt=dt, ltc=int(ltc))
@@ -247,7 +249,10 @@ class BatchedCOBA:
         fresh state the first steps carry the synchronised initial spikes
         (delivery-heavy, favouring larger tiles) and later steps the steady
         state.  The choice is kept for later batches of the same kind, count
-        and size.  Returns {tile: kernel microseconds per step}."""
+        and size.  A batch that fits the multi-step launch also times it
+        over the same window (key "fused"); when it is faster, ``capture``
+        uses it by default.  Returns {tile or "fused": kernel microseconds
+        per step}."""
         state = [self.V, self.t_last, self.g_exc, self.g_inh, self.counts, self.spikes, self.events,
                  self.spike_count, self.d_step]
         saved = [t.clone() for t in state]
@@ -272,12 +277,32 @@ class BatchedCOBA:
                 ms, launches = _lib.timer_read(_lib.KID["net"])
                 best = min(best, ms * 1e3 / max(launches, 1))
             times[tile] = best
+        if self.fused:
+            # the multi-step launch over the same window, per step
+            best = float("inf")
+            for _ in range(repeats):
+                for t, s in zip(state, saved):
+                    t.copy_(s)
+                self.i = i0
+                torch.cuda.synchronize()
+                _lib.timer_reset()
+                _lib.timer_enable(True)
+                try:
+                    self.run_fused(steps)
+                    torch.cuda.synchronize()
+                finally:
+                    _lib.timer_enable(False)
+                ms, _ = _lib.timer_read(_lib.KID["net_run"])
+                best = min(best, ms * 1e3 / steps)
+            times["fused"] = best
         for t, s in zip(state, saved):
             t.copy_(s)
         self.i = i0
-        choice = min(times, key=times.get)
+        choice = min((k for k in times if k != "fused"), key=times.get)
         self.set_tile(choice)
+        self.use_fused = self.fused and times["fused"] < times[choice]
         _TUNED[self._tune_key()] = choice
+        _TUNED_FUSED[self._tune_key()] = self.use_fused
         self.tune_times = times
         return times
 
@@ -358,14 +383,18 @@ class BatchedCOBA:
             self.step(out[k] if raster else None)
         return out
 
-    def capture(self, steps, fused=False):
+    def capture(self, steps, fused=None):
         """Capture `steps` steps in one HIP graph (one batched launch per step
         at offsets 0..steps-1 from the device base step, or with `fused` one
         multi-step launch, then one launch that advances the base by
         `steps`); replay() advances the batch by `steps` steps per call,
-        continuing in time.  Replayed, the per-step launches are as fast as
-        the multi-step launch, whose single block per network is bound by
-        the neuron update's arithmetic on one compute unit."""
+        continuing in time.  `fused` defaults to the autotuner's finding
+        (per-step launches until it has run): replayed, the per-step
+        launches of one network are as fast as the multi-step launch, whose
+        single block per network is bound by the neuron update's arithmetic
+        on one compute unit, while a batch of hundreds of small networks
+        runs faster as multi-step launches."""
+        fused = self.use_fused if fused is None else fused
         self.d_step.fill_(self.i)
         torch.cuda.synchronize()
         i0 = self.i

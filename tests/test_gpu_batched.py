@@ -123,7 +123,10 @@ def test_tile_sizes_same_raster_and_autotune_restores(kind):
     b = BatchedCOBA(specs, V_init=ref.V0)
     r1 = b.run(20, raster=True, fused=False).cpu().numpy()
     times = b.autotune(steps=5, repeats=1)
-    assert set(times) == {64, 128, 256, 512, 1024} and b.tile == min(times, key=times.get)
+    tiles = {k: v for k, v in times.items() if k != "fused"}
+    assert set(tiles) == {64, 128, 256, 512, 1024} and b.tile == min(tiles, key=tiles.get)
+    assert ("fused" in times) == b.fused
+    assert b.use_fused == (b.fused and times["fused"] < tiles[b.tile])
     r2 = b.run(30, raster=True, fused=False).cpu().numpy()
     np.testing.assert_array_equal(np.concatenate([r1, r2]), r_ref)
     np.testing.assert_array_equal(b.V.cpu().numpy(), ref.V.cpu().numpy())
