@@ -37,6 +37,24 @@ HEADLINE = ("heter_f32", 0.10)
 TILES = 32   # output tiles per copy on MI355X (CUs per XCD); set from the plan at run time
 
 
+_PROBE = None
+
+
+def probe_lib():
+    """libbpx_probe.so: measurement probes (a plain copy for counter
+    calibration, the COO/gather access-pattern ceilings), built beside the
+    product library but not part of it."""
+    global _PROBE
+    if _PROBE is None:
+        import ctypes
+        P = ctypes.CDLL(os.path.join(ROOT, "brainpy_amd", "libbpx_probe.so"))
+        vp, i64 = ctypes.c_void_p, ctypes.c_int64
+        P.bpx_probe_copy.argtypes = [vp, vp, vp, i64]
+        P.bpx_probe_coo.argtypes = [vp, vp, vp, vp, vp, i64, i64, ctypes.c_int, vp, i64]
+        _PROBE = P
+    return _PROBE
+
+
 def algorithmic_bytes(active_rows, kind):
     """Algorithmic bytes of one event csrmv transpose call, as the operator
     receives its inputs (whatever the kernel's own layout): events (1 B
@@ -207,10 +225,10 @@ def bench_coo(_lib, torch, steps=50, warmup=5):
     ceil = {}
     for mode, name in ((0, "read_row_col_value"), (1, "read_col_gather"), (2, "read_all_gather")):
         def pcall():
-            rc = L.bpx_coo_probe(sp, _lib.ptr(row), _lib.ptr(idx), _lib.ptr(w), _lib.ptr(v), n, nse, mode,
+            rc = probe_lib().bpx_probe_coo(sp, _lib.ptr(row), _lib.ptr(idx), _lib.ptr(w), _lib.ptr(v), n, nse, mode,
                                  _lib.ptr(probe_out), probe_out.numel())
             if rc != 0:
-                raise RuntimeError(f"bpx_coo_probe failed ({rc})")
+                raise RuntimeError(f"bpx_probe_coo failed ({rc})")
         for _ in range(3):
             pcall()
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -259,7 +277,7 @@ def bench_forms(_lib, torch, steps=30, warmup=3):
     bm.sparse.csrmv with a dense f32 vector (gather, and transpose = scatter
     with atomics) and bm.event.csrmv without transpose (gather of 10 %
     Bernoulli bool events).  The gathers read a random 4-byte (or 1-byte)
-    element per synapse, so bpx_coo_probe's read-col-and-gather time on the
+    element per synapse, so the probe's read-col-and-gather time on the
     same indices is reported as their ceiling."""
     from brainpy_amd.connect import fixed_prob_csr_device
     dev = _lib.device()
@@ -324,17 +342,17 @@ def bench_forms(_lib, torch, steps=30, warmup=3):
     for k in range(13):
         if k == 3:
             ev0.record()
-        rc = L.bpx_coo_probe(sp, None, _lib.ptr(idx), None, _lib.ptr(x), n, nnz, 1, _lib.ptr(probe_out),
-                             probe_out.numel())
+        rc = probe_lib().bpx_probe_coo(sp, None, _lib.ptr(idx), None, _lib.ptr(x), n, nnz, 1, _lib.ptr(probe_out),
+                                       probe_out.numel())
         if rc != 0:
-            raise RuntimeError(f"bpx_coo_probe failed ({rc})")
+            raise RuntimeError(f"bpx_probe_coo failed ({rc})")
     ev1.record()
     ev1.synchronize()
     ceiling = ev0.elapsed_time(ev1) * 1e3 / 10
     for r in res:
         if not r["transpose"]:
             r["read_col_gather_ceiling_us"] = ceiling
-    print(f"# gather ceiling (read col + gather, bpx_coo_probe mode 1): {ceiling:.1f} us", file=sys.stderr,
+    print(f"# gather ceiling (read col + gather, probe mode 1): {ceiling:.1f} us", file=sys.stderr,
           flush=True)
     del idx, ptr, w
     # bm.event.info: count, scan and write kernels; bytes = one read of the
@@ -828,17 +846,90 @@ def bench_served(_lib, torch, n_requests=80, rate=50.0, seed=7):
     return r
 
 
-def measured_traffic(name="event_scatter_100k_p0.01_rate10_heter_f32.md"):
+HEADLINE_PROFILE = "event_scatter_100k_p0.01_rate10_heter_f32.md"
+
+
+def measured_traffic(name=HEADLINE_PROFILE):
     """HBM bytes per launch of the headline kernel from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated) and the
+    summary's own date line, or (None, None)."""
     path = os.path.join(ROOT, "profiles", name)
+    traffic, date = None, None
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1])
+                traffic = int(line.split(":")[1])
+            elif line.startswith("measured:"):
+                date = line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return None
+    return traffic, date
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(indices, indptr, w, ev, events_per_call, seconds, gpu_out):
+    """The headline call on the host: the oracle's scatter (the reference CPU
+    path's loop, event/_csr_matvec.py:321-340: serial over rows, an f32
+    `+=` per synapse) built here with -O3 -march=native (oracle/Makefile
+    `native`; the prebuilt -O2 library if the compiler is unavailable), timed
+    call by call over about `seconds` at 1 thread (median), then a threaded
+    variant (private outputs, summed) at the machine's thread share."""
+    import ctypes
+    import subprocess
+    import tempfile
+    flags = "gcc -O3 -march=native -ffp-contract=off -fopenmp"
+    path = os.path.join(tempfile.mkdtemp(prefix="bpx_cpu_"), "liborc_native.so")
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"OUT={path}"], check=True,
+                       timeout=120, capture_output=True)
+    except Exception:
+        path = os.path.join(ROOT, "oracle", "liborc.so")
+        flags = "gcc -O2 -ffp-contract=off -fopenmp (prebuilt)"
+    L = ctypes.CDLL(path)
+    vp, i64 = ctypes.c_void_p, ctypes.c_int64
+    L.orc_event_csrmv_scatter_f32.argtypes = [vp, i64, vp, vp, vp, i64, i64, vp]
+    L.orc_event_csrmv_scatter_f32_threads.argtypes = [vp, i64, vp, vp, vp, i64, i64, vp, ctypes.c_int, vp]
+    h_idx = np.ascontiguousarray(indices.cpu().numpy())
+    h_ptr = np.ascontiguousarray(indptr.cpu().numpy())
+    h_w = np.ascontiguousarray(w.cpu().numpy())
+    h_ev = np.ascontiguousarray(ev.cpu().numpy().view(np.uint8))
+    m, n = h_ptr.size - 1, N
+    out = np.empty(n, np.float32)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+
+    def run(fn, *extra):
+        fn(P(h_w), h_w.size, P(h_idx), P(h_ptr), P(h_ev), m, n, P(out), *extra)
+
+    def timed(fn, budget, *extra):
+        run(fn, *extra)   # warm-up
+        times = []
+        t_end = time.perf_counter() + budget
+        while len(times) < 20 or (time.perf_counter() < t_end and len(times) < 5000):
+            t0 = time.perf_counter()
+            run(fn, *extra)
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times)), len(times)
+
+    med1, calls1 = timed(L.orc_event_csrmv_scatter_f32, seconds)
+    rel = float(np.max(np.abs(gpu_out - out) / np.maximum(np.abs(out), 1.0)))
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    scratch = np.empty(threads * n, np.float32)
+    medt, callst = timed(L.orc_event_csrmv_scatter_f32_threads, seconds / 3, threads, P(scratch))
+    return {"value": events_per_call / med1, "unit": "synaptic events/s", "cores": 1, "kind": "port",
+            "sample": f"median of {calls1} calls of the headline call (10% spikes, heter f32, 1e7 events/call), "
+                      "serial scatter of oracle/bpx_oracle.c",
+            "cpu": _cpu_model(), "flags": flags,
+            "all_threads": {"value": events_per_call / medt, "threads": threads, "calls": callst},
+            "gpu_vs_cpu_max_rel_diff": rel}
 
 
 def main():
@@ -848,7 +939,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-sweep", action="store_true", help="time the headline configuration only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the jitconn and network legs")
     ap.add_argument("--only", choices=["jitconn", "batched", "coba", "smallbatch", "served", "coo", "forms", "conn", "stdp",
                                        "bsr"],
@@ -856,6 +947,8 @@ def main():
                     help="run only this leg (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run 3 plain device copies of this many MiB (HBM counter calibration)")
+    ap.add_argument("--detail", default=os.path.join(ROOT, "gpurun_out", "bench_detail.json"),
+                    help="file that receives the sweep and the other legs (the last stdout line is the headline)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the library runs on a single MI355X and "
@@ -865,6 +958,7 @@ def main():
     import torch
     from brainpy_amd import _lib
     from brainpy_amd.connect import fixed_prob_csr_device
+    import brainpy_amd.math as bm
 
     dev = _lib.device()
     L = _lib.lib()
@@ -878,7 +972,6 @@ def main():
               "bsr": lambda: bench_bsr(_lib, torch)}[args.only]
         print(json.dumps({"only": args.only, "result": fn()}), flush=True)
         return
-    stream = torch.cuda.current_stream()
     sp = _lib.stream_ptr()
     print(f"# building {M}x{N} CSR, {K_ROW}/row, nnz={M * K_ROW:.3g}", file=sys.stderr, flush=True)
     indices, indptr = fixed_prob_csr_device(M, N, PROB, seed=2024, device=dev)
@@ -897,22 +990,14 @@ def main():
         g.manual_seed(int(r * 1e6) + 11)
         spikes[r] = torch.rand(M, generator=g, device=dev) < r
     out = torch.empty(N, dtype=torch.float32, device=dev)
-    from brainpy_amd.math.event import TiledCSR
-    print("# building tiled plans", file=sys.stderr, flush=True)
-    plans = {
-        "homo": TiledCSR(weights["homo"], indices, indptr, shape=(M, N)),
-        "heter_f32": TiledCSR(w_f32, indices, indptr, shape=(M, N)),
-        "heter_bf16": TiledCSR(w_f32, indices, indptr, shape=(M, N), weight_dtype=torch.bfloat16),
-    }
     torch.cuda.synchronize()
     import ctypes
 
-    def call_tiled(kind, r):
-        plan = plans[kind]
-        rc = L.bpx_event_csrmv_tiled(sp, ctypes.byref(plan.plan), _lib.ptr(spikes[r]), _lib.BPX_BOOL,
-                                     _lib.RULE_POSITIVE, 0, _lib.ptr(out))
-        if rc != 0:
-            raise RuntimeError(f"bpx_event_csrmv_tiled failed ({rc})")
+    def call_public(kind, r):
+        # bm.event.csrmv(data, indices, indptr, events, shape, transpose=True):
+        # the reference's operator signature; repeated calls on one matrix
+        # run on the cached tiled plan
+        return bm.event.csrmv(weights[kind], indices, indptr, spikes[r], shape=(M, N), transpose=True)
 
     def call_csr(kind, r):
         w = weights[kind]
@@ -922,9 +1007,25 @@ def main():
         if rc != 0:
             raise RuntimeError(f"bpx_event_csrmv failed ({rc})")
 
-    def measure(kind, r, steps, warmup, path="tiled"):
-        call = call_tiled if path == "tiled" else call_csr
-        kid = _lib.KID["tiled"] if path == "tiled" else _lib.KID["event_scatter"]
+    # the first public call on a matrix is stateless, the second builds the
+    # plan (its one-time cost, reported beside the per-call time)
+    build_ms = {}
+    for kind in ("homo", "heter_f32", "heter_bf16"):
+        call_public(kind, HEADLINE[1])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        call_public(kind, HEADLINE[1])
+        torch.cuda.synchronize()
+        build_ms[kind] = (time.perf_counter() - t0) * 1e3
+    info = bm.event.plan_cache_info()
+    if info["plans"] != 3:
+        raise RuntimeError(f"the public path did not build its plans: {info}")
+    global TILES
+    TILES = next(p for p, _ in bm.event._PLANS._plans.values() if p is not None).plan.tiles
+
+    def measure(kind, r, steps, warmup, path="public"):
+        call = call_public if path == "public" else call_csr
+        kid = _lib.KID["tiled"] if path == "public" else _lib.KID["event_scatter"]
         active = int(spikes[r].sum().item())
         events = active * K_ROW
         for _ in range(warmup):
@@ -945,7 +1046,7 @@ def main():
         ms, launches = _lib.timer_read(kid)
         kern_s = ms / 1e3 / max(launches, 1)
         abytes = algorithmic_bytes(active, kind)
-        lbytes = layout_bytes(active, kind, path)
+        lbytes = layout_bytes(active, kind, "tiled" if path == "public" else "csr")
         return {
             "path": path, "kind": kind, "rate": r, "active_rows": active, "events_per_call": events,
             "ms_per_step": wall / steps * 1e3, "events_per_s": events * steps / wall,
@@ -953,26 +1054,19 @@ def main():
             "kernel_GBs": abytes / kern_s / 1e9, "layout_bytes": lbytes, "layout_GBs": lbytes / kern_s / 1e9,
         }
 
-    global TILES
-    TILES = plans["heter_f32"].plan.tiles
     if args.copy_mib:
         src = torch.empty(args.copy_mib << 20, dtype=torch.uint8, device=dev)
         dst = torch.empty_like(src)
         for _ in range(3):
-            L.bpx_copy(sp, _lib.ptr(src), _lib.ptr(dst), src.numel())
+            probe_lib().bpx_probe_copy(sp, _lib.ptr(src), _lib.ptr(dst), src.numel())
         torch.cuda.synchronize()
         del src, dst
-    # the tiled path must agree with the plain CSR path
-    for kind in ("homo", "heter_f32"):
-        call_csr(kind, HEADLINE[1])
-        ref = out.clone()
-        call_tiled(kind, HEADLINE[1])
-        torch.cuda.synchronize()
-        err = float(((out - ref).abs() / ref.abs().clamp_min(1.0)).max())
-        if err > 1e-5:
-            raise RuntimeError(f"tiled path disagrees with the CSR path ({kind}: {err})")
-    # FP32 and BF16-stored heterogeneous weights: relative error of the
-    # scattered current against an FP64 sum of the same (FP32) weights
+    # parity of the public path: integer counts exact against the stateless
+    # kernel; heterogeneous weights against an FP64 sum of the same weights
+    call_csr("homo", HEADLINE[1])
+    ref = out.clone()
+    if not torch.equal(call_public("homo", HEADLINE[1]), ref):
+        raise RuntimeError("public path (plan) disagrees with the stateless kernel on integer counts")
     ev = spikes[HEADLINE[1]]
     rows = torch.nonzero(ev).flatten()
     starts = indptr[rows].long()
@@ -981,53 +1075,55 @@ def main():
     ent = torch.repeat_interleave(starts - first, lens) + torch.arange(int(lens.sum()), device=dev)
     ref64 = torch.zeros(N, dtype=torch.float64, device=dev)
     ref64.index_add_(0, indices[ent].long(), w_f32[ent].double())
+    mag64 = torch.zeros(N, dtype=torch.float64, device=dev)
+    mag64.index_add_(0, indices[ent].long(), w_f32[ent].double().abs())
     del ent
-    big = ref64.abs() > 1.0
     f64_err = {}
     for kind in ("heter_f32", "heter_bf16"):
-        call_tiled(kind, HEADLINE[1])
+        y = call_public(kind, HEADLINE[1])
         torch.cuda.synchronize()
-        e = (out.double() - ref64).abs()[big] / ref64.abs()[big]
+        e = (y.double() - ref64).abs() / mag64.clamp_min(1e-30)
         f64_err[kind] = {"max": float(e.max()), "mean": float(e.mean())}
+    if f64_err["heter_f32"]["max"] > 1e-5:
+        raise RuntimeError(f"public path error against FP64 above 1e-5: {f64_err}")
+    detail = {"plan_build_ms": build_ms, "plan_cache": bm.event.plan_cache_info(),
+              "current_rel_err_vs_f64": dict(f64_err, against="FP64 sum of the FP32 weights, relative to the "
+                                                              "sum of |w| of each output")}
     sweep = []
     if not args.no_sweep:
-        for path in ("csr", "tiled"):
+        for path in ("csr", "public"):
             for kind in ("homo", "heter_f32", "heter_bf16"):
                 for r in RATES:
                     res = measure(kind, r, max(20, args.steps // 4), max(5, args.warmup // 4), path)
                     sweep.append(res)
-                    print(f"# {path:5s} {kind:10s} rate={r:<6} {res['ms_per_step']*1e3:8.1f} us/call "
+                    print(f"# {path:6s} {kind:10s} rate={r:<6} {res['ms_per_step']*1e3:8.1f} us/call "
                           f"kernel {res['kernel_us']:8.1f} us {res['kernel_GBs']:8.1f} GB/s "
                           f"{res['events_per_s']:.3e} ev/s", file=sys.stderr, flush=True)
-    # graph-amortised: 100 calls (zero-fill + scatter) captured in one HIP
-    # graph; the output of two replays is checked against the eager result
+    detail["sweep"] = [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()} for s in sweep]
+    # graph-amortised: 100 public calls captured in one HIP graph (on the
+    # cached plan); two replays are checked against the eager result
     graph = []
     if not args.no_sweep:
         for r in RATES:
-            call_tiled(HEADLINE[0], r)
+            ref_out = call_public(HEADLINE[0], r).clone()
             torch.cuda.synchronize()
-            ref_out = out.clone()
             g = torch.cuda.CUDAGraph()
             cs = torch.cuda.Stream()
             cs.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(cs):
                 with torch.cuda.graph(g, stream=cs):
                     for _ in range(100):
-                        rc = L.bpx_event_csrmv_tiled(_lib.stream_ptr(), ctypes.byref(plans[HEADLINE[0]].plan),
-                                                     _lib.ptr(spikes[r]), _lib.BPX_BOOL, _lib.RULE_POSITIVE, 0,
-                                                     _lib.ptr(out))
-                        if rc != 0:
-                            raise RuntimeError(f"bpx_event_csrmv_tiled failed in capture ({rc})")
+                        gout = call_public(HEADLINE[0], r)
             torch.cuda.current_stream().wait_stream(cs)
             torch.cuda.synchronize()
             reps = 5
             for _ in range(2):
-                out.fill_(-1.0)
+                gout.fill_(-1.0)
                 g.replay()
                 torch.cuda.synchronize()
-                # (the 16 output copies are flushed with FP32 atomics, so the
+                # (the output copies are flushed with FP32 atomics, so the
                 # last bits depend on their order)
-                if not torch.allclose(out, ref_out, rtol=1e-5, atol=1e-5):
+                if not torch.allclose(gout, ref_out, rtol=1e-5, atol=1e-5):
                     raise RuntimeError(f"graph replay differs from the eager result (rate {r})")
             t0 = time.perf_counter()
             for _ in range(reps):
@@ -1037,30 +1133,33 @@ def main():
             active = int(spikes[r].sum().item())
             graph.append({"kind": HEADLINE[0], "rate": r, "us_per_call": per_call * 1e6,
                           "events_per_s": active * K_ROW / per_call})
-            print(f"# graph tiled {HEADLINE[0]} rate={r:<6} {per_call*1e6:8.1f} us/call "
+            print(f"# graph public {HEADLINE[0]} rate={r:<6} {per_call*1e6:8.1f} us/call "
                   f"{active * K_ROW / per_call:.3e} ev/s", file=sys.stderr, flush=True)
-            del g
-    head = measure(HEADLINE[0], HEADLINE[1], args.steps, args.warmup, "tiled")
+            del g, gout
+    detail["graph_amortised"] = graph
+    head = measure(HEADLINE[0], HEADLINE[1], args.steps, args.warmup, "public")
     plain = measure(HEADLINE[0], HEADLINE[1], args.steps, args.warmup, "csr")
     print(f"# headline {head}", file=sys.stderr, flush=True)
-    # the same kernel when most rows are active: the streaming efficiency
-    # without the fixed per-call costs that dominate at 10 %
+    print(f"# stateless scatter (first call on a matrix) {plain['ms_per_step']*1e3:.1f} us/call, "
+          f"kernel {plain['kernel_us']:.1f} us", file=sys.stderr, flush=True)
+    detail["headline"] = head
+    detail["stateless_path"] = plain
     high = []
     if not args.no_sweep:
         for r in (0.5, 1.0):
             gr = torch.Generator(device=dev)
             gr.manual_seed(int(r * 1e6) + 11)
             spikes[r] = torch.rand(M, generator=gr, device=dev) < r
-            res_r = measure(HEADLINE[0], r, 50, 5, "tiled")
+            res_r = measure(HEADLINE[0], r, 50, 5, "public")
             high.append(res_r)
-            print(f"# tiled {HEADLINE[0]} rate={r:<6} {res_r['ms_per_step']*1e3:8.1f} us/call kernel "
+            print(f"# public {HEADLINE[0]} rate={r:<6} {res_r['ms_per_step']*1e3:8.1f} us/call kernel "
                   f"{res_r['kernel_us']:8.1f} us {res_r['kernel_GBs']:7.0f} GB/s "
                   f"({100 * res_r['kernel_GBs'] / HBM_PEAK_GBS:.1f}%) {res_r['events_per_s']:.3e} ev/s",
                   file=sys.stderr, flush=True)
             del spikes[r]
-    print(f"# current error vs f64 (10% heter): f32 max {f64_err['heter_f32']['max']:.2e} mean "
-          f"{f64_err['heter_f32']['mean']:.2e}, bf16 max {f64_err['heter_bf16']['max']:.2e} mean "
-          f"{f64_err['heter_bf16']['mean']:.2e}", file=sys.stderr, flush=True)
+    detail["high_rate"] = high
+    print(f"# current error vs f64 (10% heter, per output / sum|w|): f32 max {f64_err['heter_f32']['max']:.2e}, "
+          f"bf16 max {f64_err['heter_bf16']['max']:.2e}", file=sys.stderr, flush=True)
 
     extra = {}
     if not args.no_extra:
@@ -1074,40 +1173,24 @@ def main():
         extra["conn"] = bench_conn(_lib, torch)
         extra["stdp"] = bench_stdp(_lib, torch)
         extra["bsr"] = bench_bsr(_lib, torch)
+    detail["extra"] = extra
 
-    # CPU baseline: the oracle's serial scatter (the reference CPU loop
-    # order, event/_csr_matvec.py:321-340), one thread, on a bounded number
-    # of calls of the headline workload
     cpu = None
     if not args.no_cpu:
-        from oracle import orc
-        h_idx = indices.cpu().numpy()
-        h_ptr = indptr.cpu().numpy()
-        h_w = weights[HEADLINE[0]].cpu().numpy()
-        h_ev = spikes[HEADLINE[1]].cpu().numpy().view(np.uint8)
-        h_out = np.empty(N, np.float32)
-        orc.event_csrmv_scatter_f32(h_w, h_idx, h_ptr, h_ev, (M, N), out=h_out)   # warm-up
-        calls = 0
-        t0 = time.perf_counter()
-        while True:
-            orc.event_csrmv_scatter_f32(h_w, h_idx, h_ptr, h_ev, (M, N), out=h_out)
-            calls += 1
-            el = time.perf_counter() - t0
-            if el >= args.cpu_seconds or calls >= 2000:
-                break
-        # the GPU result of the same call agrees with the CPU one
-        call_tiled(*HEADLINE)
-        torch.cuda.synchronize()
-        rel = float(np.max(np.abs(out.cpu().numpy() - h_out) / np.maximum(np.abs(h_out), 1.0)))
-        cpu = {"value": head["events_per_call"] * calls / el, "unit": "synaptic events/s", "cores": 1,
-               "kind": "port",
-               "sample": f"{calls} calls of the headline workload ({HEADLINE[1]*100:.0f}% spikes, heter f32) "
-                         f"in {el:.1f} s: serial scatter of oracle/bpx_oracle.c, gcc -O2, 1 thread",
-               "gpu_vs_cpu_max_rel_diff": rel}
+        y = call_public(HEADLINE[0], HEADLINE[1]).cpu().numpy()
+        cpu = cpu_baseline(indices, indptr, w_f32, spikes[HEADLINE[1]], head["events_per_call"], args.cpu_seconds, y)
         print(f"# cpu {cpu}", file=sys.stderr, flush=True)
+        detail["cpu_baseline"] = cpu
+    try:
+        os.makedirs(os.path.dirname(args.detail), exist_ok=True)
+        with open(args.detail, "w") as f:
+            json.dump(detail, f, indent=1)
+    except OSError as e:
+        print(f"# could not write {args.detail}: {e}", file=sys.stderr)
 
+    traffic, traffic_date = measured_traffic()
     line = {
-        "metric": "synaptic events/s (event csrmv transpose, 100k x 100k, p=0.01)",
+        "metric": "synaptic events/s",
         "value": head["events_per_s"],
         "unit": "synaptic events/s",
         "n_gpus": 1,
@@ -1121,23 +1204,17 @@ def main():
         "data": "synthetic (seeded FixedProb CSR and Bernoulli spikes, generated on the GPU)",
         "config": {"workload": "event_csrmv_transpose_100kx100k_p0.01_rate0.10_heter_f32", "m": M, "n": N,
                    "nnz": nnz, "spike_rate": HEADLINE[1], "weights": HEADLINE[0], "index_dtype": "int32",
-                   "path": "tiled CSR plan (built once before timing)",
-                   "active_rows": head["active_rows"], "events_per_step": head["events_per_call"]},
-        "roofline": {"bound": "hbm", "achieved": head["kernel_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": head["kernel_GBs"] / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "k_tiled_scatter (heter f32, bool events)", "kernel_us": head["kernel_us"],
-                     "algorithmic_bytes_per_launch": head["algorithmic_bytes"],
-                     "layout_bytes_per_launch": head["layout_bytes"],
-                     "layout_frac": head["layout_GBs"] / HBM_PEAK_GBS},
-        "current_rel_err_vs_f64": dict(f64_err, against="FP64 sum of the FP32 weights, outputs with |y| > 1"),
-        "graph_amortised": graph,
-        "high_rate": high,
-        "plain_csr_path": {"events_per_s": plain["events_per_s"], "ms_per_step": plain["ms_per_step"],
-                           "kernel_us": plain["kernel_us"], "kernel_GBs": plain["kernel_GBs"],
-                           "kernel": "k_csr_scatter (stateless bm.event.csrmv)"},
-        "cpu_baseline": cpu,
-        "sweep": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()} for s in sweep],
-        "extra": extra,
+                   "path": "bm.event.csrmv(transpose=True) on its cached tiled plan",
+                   "plan_build_ms": round(build_ms[HEADLINE[0]], 3), "events_per_step": head["events_per_call"]},
+        "roofline": {"bound": "hbm", "achieved": round(head["kernel_GBs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(head["kernel_GBs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": f"profiles/{HEADLINE_PROFILE} ({traffic_date})",
+                     "kernel": "k_tiled_scatter", "kernel_us": round(head["kernel_us"], 3),
+                     "algorithmic_bytes_per_launch": head["algorithmic_bytes"]},
+        "cpu_baseline": None if cpu is None else {
+            "value": cpu["value"], "unit": cpu["unit"], "cores": cpu["cores"], "kind": cpu["kind"],
+            "sample": cpu["sample"], "cpu": cpu["cpu"], "flags": cpu["flags"],
+            "all_threads": {"value": cpu["all_threads"]["value"], "threads": cpu["all_threads"]["threads"]}},
     }
     print(json.dumps(line), flush=True)
 

@@ -56,7 +56,6 @@ SIGNATURES = {
     "bpx_jitconn_workspace": ([_i64, _i64], _sz),
     "bpx_lif_coba_step": ([_vp, _i64, _i32, _i32, _vp, _vp, _vp, ctypes.POINTER(_vp), _vp, _vp, _vp, _f32, _vp,
                            _f32, _f32, _f32, _f32, _f32, _f32, _f32, _f32], _i32),
-    "bpx_copy": ([_vp, _vp, _vp, _i64], _i32),
     "bpx_csr_on_pre_update": ([_vp, _vp, _i32, _vp, _vp, _i32, _i64, _vp, _i32, _i64, _vp, _i64,
                                ctypes.c_double, ctypes.c_double, _vp], _i32),
     "bpx_csc_on_post_update": ([_vp, _vp, _i32, _vp, _vp, _vp, _i32, _i64, _vp, _i32, _i64, _vp, _i64,
@@ -67,7 +66,6 @@ SIGNATURES = {
     "bpx_coo_matmat": ([_vp, _vp, _i32, _i64, _vp, _vp, _i32, _i64, _vp, _i32, _i64, _i64, _i64, _i32, _vp], _i32),
     "bpx_coo_sddmm": ([_vp, _vp, _vp, _i32, _i64, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _i32, _vp],
                       _i32),
-    "bpx_coo_probe": ([_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i32, _vp, _i64], _i32),
     "bpx_bsrmv": ([_vp, _vp, _i32, _vp, _vp, _i32, _i64, _i64, _vp, _i64, _i64, _i32, _vp], _i32),
     "bpx_bsr_sddmm": ([_vp, _vp, _vp, _i32, _i64, _i64, _vp, _vp, _i32, _i64, _i64, _vp], _i32),
     "bpx_bcsrmm": ([_vp, _vp, _i32, _vp, _vp, _vp, _i32, _i64, _i64, _i64, _i64, _vp], _i32),
@@ -158,6 +156,13 @@ def to_device(x, dtype=None):
     return t.contiguous()
 
 
+def written(t):
+    """Record that a kernel wrote into `t` through its raw pointer: bumps
+    the tensor's version counter as an in-place torch op would, so caches
+    keyed on it (the event csrmv plan cache) see the new contents."""
+    torch.autograd.graph.increment_version(t)
+
+
 def timer_enable(on=True):
     check(lib().bpx_timer_enable(int(on)), "bpx_timer_enable")
 
@@ -174,4 +179,4 @@ def timer_read(kernel_id):
 
 
 KID = dict(event_scatter=0, event_gather=1, csr_scatter=2, csr_gather=3, jit_outdim=4, jit_scatter=5, info=6,
-           lif=7, fixed_order=8, copy=9, net=10, tiled=11, coo=12, bsr=13, net_run=14)
+           lif=7, fixed_order=8, net=10, tiled=11, coo=12, bsr=13, net_run=14)

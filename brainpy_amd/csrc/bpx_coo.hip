@@ -256,34 +256,6 @@ __global__ __launch_bounds__(256) void k_coo_sddmm(const I* __restrict__ row, co
   }
 }
 
-// Ceilings of the COO access pattern (measurement helper): per lane, 4
-// consecutive entries per dwordx4 load, grid-stride, one f32 sum per thread
-// written to out (so nothing is optimised away).  mode 0 reads row, col and
-// value; mode 1 reads col and gathers src[col]; mode 2 does both.
-__global__ __launch_bounds__(256) void k_coo_probe(const int32_t* __restrict__ row, const int32_t* __restrict__ col,
-                                                   const float* __restrict__ w, const float* __restrict__ src,
-                                                   int64_t n4, int mode, float* __restrict__ out) {
-  typedef int32_t i4 __attribute__((ext_vector_type(4)));
-  typedef float f4 __attribute__((ext_vector_type(4)));
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  float acc = 0.f;
-  for (int64_t i = tid; i < n4; i += stride) {
-    const i4 c = ((const i4*)col)[i];
-    if (mode != 1) {
-      const i4 r = ((const i4*)row)[i];
-      const f4 x = ((const f4*)w)[i];
-      if (mode == 0)
-        acc += (float)(r.x ^ c.y) + x.x + x.y + x.z + x.w + (float)(r.w + c.w);
-      else
-        acc += x.x * src[c.x] + x.y * src[c.y] + x.z * src[c.z] + x.w * src[c.w] + (float)(r.x & 1);
-    } else {
-      acc += src[c.x] + src[c.y] + src[c.z] + src[c.w];
-    }
-  }
-  out[tid] = acc;
-}
-
 unsigned grid_for(int64_t work_items, int per_block) {
   int64_t blocks = (work_items + per_block - 1) / per_block;
   if (blocks < 1) blocks = 1;
@@ -439,20 +411,6 @@ int bpx_coo_sddmm(void* stream, const void* row, const void* col, int index_dtyp
     BPX_CHECK_LAUNCH();
     return BPX_OK;
   }));
-}
-
-int bpx_coo_probe(void* stream, const int32_t* row, const int32_t* col, const float* data, const float* src,
-                  int64_t src_len, int64_t nse, int mode, float* out, int64_t out_len) {
-  if (nse < 0 || (nse & 3) || out_len < 256 || (out_len & 255) || mode < 0 || mode > 2) return BPX_ERR_SHAPE;
-  if (!col || !out || (mode != 1 && (!row || !data)) || (mode != 0 && (!src || src_len <= 0))) return BPX_ERR_ARG;
-  for (const void* p : {(const void*)row, (const void*)col, (const void*)data})
-    if (p && !aligned16(p)) return BPX_ERR_ARG;
-  int64_t blocks = out_len / 256;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_coo_probe, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, row, col, data, src,
-                     nse / 4, mode, out);
-  BPX_CHECK_LAUNCH();
-  return BPX_OK;
 }
 
 }  // extern "C"

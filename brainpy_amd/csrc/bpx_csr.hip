@@ -296,16 +296,14 @@ __global__ __launch_bounds__(256) void k_fixed_order(const V* __restrict__ data,
 // ---------------------------------------------------------------------------
 // host dispatch
 // ---------------------------------------------------------------------------
-// blocks of the bitmask gather: 8 per CU (each packs the event vector once)
+// blocks of the bitmask gather: 2 per CU (each packs the event vector once)
 static int64_t gather_bits_grid() {
   static int64_t g = [] {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
       cus = 256;
-    const char* e = getenv("BPX_GATHER_BLOCKS_PER_CU");
-    const int per = e ? atoi(e) : 2;
-    return (int64_t)cus * (per > 0 ? per : 2);
+    return (int64_t)cus * 2;
   }();
   return g;
 }

@@ -423,16 +423,6 @@ static int jit_cus() {
   return c;
 }
 
-// one-thread-per-row kernel for bool events (BPX_JIT_ROWS=0 selects the
-// thread-per-(row, lane) bitmask kernel, for comparison)
-static bool jit_rows_kernel() {
-  static int v = [] {
-    const char* e = getenv("BPX_JIT_ROWS");
-    return e ? atoi(e) : 1;
-  }();
-  return v != 0;
-}
-
 static bool bits_fit(int64_t nwords) {
   static int shm = [] {
     int dev = 0, v = 0;
@@ -450,7 +440,7 @@ static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t c
   const uint32_t step = (uint32_t)(((num_row + 1) >> 5) > 1 ? ((num_row + 1) >> 5) : 1);
   if (outdim) {
     const int64_t nwords = (num_col + 31) / 32;
-    if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && jit_rows_kernel() && bits_fit(nwords) && ws &&
+    if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && bits_fit(nwords) && ws &&
         ws_bytes >= (size_t)nwords * 4 && num_col < 0xFFFFFFFFll) {
       uint32_t* bits = (uint32_t*)ws;
       hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((num_col + 255) / 256)), dim3(256), 0, s, (const uint8_t*)vec,

@@ -127,14 +127,8 @@ static void copy_weights(hipStream_t s, const void* w, void* out, int64_t n) {
 
 // rows per wave chunk: the largest power of two (at most 64) that still gives
 // about 16384 waves, so that enough rows' dependent index -> weight chains
-// are in flight (BPX_PLAST_ROWS_PER_WAVE overrides, 1..64)
+// are in flight
 static int plast_rows_per_wave(int64_t nrows) {
-  static const int forced = [] {
-    const char* e = getenv("BPX_PLAST_ROWS_PER_WAVE");
-    const int v = e ? atoi(e) : 0;
-    return v >= 1 && v <= 64 ? v : 0;
-  }();
-  if (forced) return forced;
   int rw = 64;
   while (rw > 1 && nrows / rw < 16384) rw >>= 1;
   return rw;

@@ -1,4 +1,4 @@
-// bpx_runtime.hip -- status strings, the kernel timer and the copy kernel.
+// bpx_runtime.hip -- status strings and the kernel timer.
 #include <mutex>
 #include <vector>
 
@@ -61,18 +61,6 @@ void timer_acquire(int kid, hipEvent_t* start, hipEvent_t* stop) {
   *stop = e;
 }
 
-// A plain streaming copy: 16-byte vector loads and stores, grid-stride.
-__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n16; i += stride) dst[i] = src[i];
-}
-
-__global__ void k_copy1(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[i];
-}
-
 }  // namespace bpx
 
 extern "C" {
@@ -113,25 +101,6 @@ int bpx_timer_reset(void) {
     bpx::g_total_ms[i] = 0.0;
     bpx::g_count[i] = 0;
   }
-  return BPX_OK;
-}
-
-int bpx_copy(void* stream, const void* src, void* dst, int64_t bytes) {
-  if (bytes < 0) return BPX_ERR_ARG;
-  if (bytes == 0) return BPX_OK;
-  hipStream_t s = (hipStream_t)stream;
-  bool aligned = (((uintptr_t)src | (uintptr_t)dst | (uintptr_t)bytes) & 15) == 0;
-  if (aligned) {
-    int64_t n16 = bytes / 16;
-    int64_t blocks = (n16 + 255) / 256;
-    if (blocks > 256 * 32) blocks = 256 * 32;
-    BPX_LAUNCH(bpx::KID_COPY, bpx::k_copy16, dim3((unsigned)blocks), dim3(256), 0, s,
-               (const uint4*)src, (uint4*)dst, n16);
-  } else {
-    int64_t blocks = (bytes + 255) / 256;
-    hipLaunchKernelGGL(bpx::k_copy1, dim3((unsigned)blocks), dim3(256), 0, s, (const uint8_t*)src, (uint8_t*)dst, bytes);
-  }
-  BPX_CHECK_LAUNCH();
   return BPX_OK;
 }
 

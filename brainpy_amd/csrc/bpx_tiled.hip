@@ -248,7 +248,10 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
     finite_x = mx <= 3.402823466e38f;       // false for inf and NaN
     int ex = 0;
     if (finite_x && mx > 0.f) frexpf(mx, &ex);   // mx < 2^ex
-    if (ex > 0) {
+    // products |w * x| < max|w| * 2^ex: scaling by 2^(shift - ex) keeps every
+    // term below 2^50 and every sum below 2^61 for any ex, and for |x| < 1
+    // (ex <= 0) it raises the resolution with the products' magnitude
+    if (ex != 0) {
       fx_scale = ldexp(fx_scale, -ex);
       fx_inv = ldexp(fx_inv, ex);
     }
@@ -434,26 +437,12 @@ __global__ __launch_bounds__(256) void k_fill_zero_t(O* __restrict__ out, int64_
   for (; i < n; i += stride) out[i] = O(0);
 }
 
-// output copies per XCD (2 by default: two 1024-thread blocks per CU hide
-// the memory latency of the row walk; BPX_TILED_COPIES overrides for tuning)
-static int tiled_copies_per_xcd() {
-  static int c = [] {
-    const char* e = getenv("BPX_TILED_COPIES");
-    int v = e ? atoi(e) : 2;
-    return v >= 1 && v <= 4 ? v : 2;
-  }();
-  return c;
-}
+// output copies per XCD: two 1024-thread blocks per CU hide the memory
+// latency of the row walk
+static int tiled_copies_per_xcd() { return 2; }
 
-// rows a lane group streams per round (BPX_TILED_ROWS overrides for tuning)
-static int tiled_rows_per_group() {
-  static int r = [] {
-    const char* e = getenv("BPX_TILED_ROWS");
-    int v = e ? atoi(e) : 1;
-    return (v == 1 || v == 2) ? v : 1;
-  }();
-  return r;
-}
+// rows a lane group streams per round
+static int tiled_rows_per_group() { return 1; }
 
 struct TiledShape {
   int P, T;

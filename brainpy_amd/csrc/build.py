@@ -19,7 +19,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libbpx.so")
-SOURCES = ["bpx_runtime.hip", "bpx_csr.hip", "bpx_info.hip", "bpx_jitconn.hip", "bpx_neuron.hip", "bpx_tiled.hip", "bpx_net.hip", "bpx_plast.hip", "bpx_coo.hip", "bpx_conn.hip", "bpx_bsr.hip"]
+PROBE_LIB = os.path.join(PKG, "libbpx_probe.so")   # measurement probes, loaded by the benchmark only
+PROBE_SOURCES = ["bpx_probe.hip"]
+SOURCES = ["bpx_runtime.hip", "bpx_csr.hip", "bpx_info.hip", "bpx_jitconn.hip", "bpx_neuron.hip", "bpx_tiled.hip",
+           "bpx_net.hip", "bpx_plast.hip", "bpx_coo.hip", "bpx_conn.hip", "bpx_bsr.hip", "bpx_xla.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -29,11 +32,18 @@ FLAGS = [
 ]
 
 
+def _link(target, objs):
+    tmp = target + ".tmp"
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    subprocess.check_call(cmd)
+    os.replace(tmp, target)
+
+
 def build(verbose=False, jobs=8):
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
     procs, objs = [], []
-    for src in SOURCES:
+    for src in SOURCES + PROBE_SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
         cmd = [HIPCC] + FLAGS + ["-c", os.path.join(HERE, src), "-o", obj]
@@ -53,10 +63,8 @@ def build(verbose=False, jobs=8):
             sys.stderr.write(text)
     if failed:
         raise RuntimeError("hipcc failed for: " + ", ".join(failed))
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
-    subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
+    _link(LIB, objs[:len(SOURCES)])
+    _link(PROBE_LIB, objs[len(SOURCES):])
     return LIB
 
 

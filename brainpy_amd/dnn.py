@@ -92,6 +92,7 @@ def csr_on_pre_update(w, indices, indptr, spike, trace, w_min=None, w_max=None, 
                                           _lib.bpx_dtype(sp), sp.numel(), _lib.ptr(tr), tr.numel(), lo, hi,
                                           _lib.ptr(out))
     _lib.check(rc, "bpx_csr_on_pre_update")
+    _lib.written(out)
     return out
 
 
@@ -116,6 +117,7 @@ def csc_on_post_update(w, post_ids, indptr, w_ids, spike, trace, w_min=None, w_m
                                            _lib.ptr(sp), _lib.bpx_dtype(sp), sp.numel(), _lib.ptr(tr), tr.numel(),
                                            lo, hi, _lib.ptr(out))
     _lib.check(rc, "bpx_csc_on_post_update")
+    _lib.written(out)
     return out
 
 

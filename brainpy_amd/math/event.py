@@ -7,7 +7,9 @@ Signatures, validation and error messages follow
 ``event/_info_collection.py:24-43``.  Inputs may be numpy arrays, python
 scalars or torch tensors; results are torch tensors on the GPU.
 """
+import collections
 import ctypes
+import threading
 from typing import Tuple
 
 import numpy as np
@@ -16,7 +18,8 @@ import torch
 from .. import _lib
 from ._common import acc_dtype, batch_rows, data_tensor, index_tensor, ndim, needs_ad
 
-__all__ = ["csrmv", "csrmv_taichi", "info", "csrmv_fixed_order", "csr_to_csc", "csrmv_add", "TiledCSR"]
+__all__ = ["csrmv", "csrmv_taichi", "info", "csrmv_fixed_order", "csr_to_csc", "csrmv_add", "TiledCSR",
+           "plan_cache_info", "clear_plan_cache", "set_plan_cache"]
 
 
 def _check(data, indices, indptr, events, shape, transpose, allowed_index, index_msg):
@@ -73,18 +76,132 @@ def _events_tensor(events):
     return ev.to(torch.float32)
 
 
-def _launch(d, idx, ptr, ev, shape, transpose, rule):
+def _launch(d, idx, ptr, ev, shape, transpose, rule, host_w=None):
     # raw pointers go to the kernel: views are made dense first
     d, idx, ptr, ev = d.contiguous(), idx.contiguous(), ptr.contiguous(), ev.contiguous()
     m, n = int(shape[0]), int(shape[1])
     if ptr.shape[0] != m + 1:
         raise ValueError(f'indptr should have shape ({m + 1},), but we got {tuple(ptr.shape)}.')
+    if transpose:
+        plan = _PLANS.lookup(d, idx, ptr, m, n, ev, host_w)
+        if plan is not None:
+            return plan._exec(ev, rule, None)
     out = torch.empty(n if transpose else m, dtype=acc_dtype(d.dtype), device=ev.device)
     rc = _lib.lib().bpx_event_csrmv(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), d.shape[0], _lib.ptr(idx),
                                     _lib.ptr(ptr), _lib.bpx_dtype(idx), idx.shape[0], _lib.ptr(ev),
                                     _lib.bpx_dtype(ev), rule, m, n, int(bool(transpose)), _lib.ptr(out))
     _lib.check(rc, "bpx_event_csrmv")
     return out
+
+
+class _PlanCache:
+    """Tiled plans (:class:`TiledCSR`) of the matrices that ``csrmv(...,
+    transpose=True)`` is called with repeatedly.
+
+    The reference's layers call the operator statelessly every step with
+    the same arrays (dnn/linear.py:630-634); the plan replaces one global
+    atomic per synaptic event by LDS accumulation, so the second call on the
+    same (data, indices, indptr) builds a plan and later calls run on it.
+    A key is the storage address, dtype, length and version counter of each
+    array (a homogeneous weight given on the host is keyed by its value):
+    writing into an array in place bumps its version, so the next call
+    builds a new plan from the new contents.  An entry holds references to
+    its arrays, so their addresses cannot be reused by other tensors while
+    it lives.  Least recently used entries are dropped beyond ``capacity``
+    plans or ``max_bytes`` of plan memory.  Eligible: f32 or bf16 weights,
+    bool or f32 events, shapes the plan layout accepts; everything else runs
+    the stateless scatter.  Results equal the stateless kernel's within f32
+    summation order (each output copy is summed exactly in fixed point)."""
+
+    def __init__(self, capacity=8, max_bytes=16 << 30):
+        self._lock = threading.Lock()
+        self._plans = collections.OrderedDict()   # key -> (TiledCSR or None, refs)
+        self._seen = collections.OrderedDict()    # keys seen once
+        self.capacity = capacity
+        self.max_bytes = max_bytes
+        self.enabled = True
+        self.builds = 0
+        self.hits = 0
+
+    @staticmethod
+    def _key(d, idx, ptr, m, n, host_w):
+        dk = (("value", float(host_w)) if host_w is not None
+              else (d.data_ptr(), d._version))
+        return (dk, d.dtype, d.shape[0], idx.data_ptr(), idx._version, idx.dtype, idx.shape[0],
+                ptr.data_ptr(), ptr._version, m, n, d.device.index)
+
+    def lookup(self, d, idx, ptr, m, n, ev, host_w):
+        if not self.enabled or d.dtype not in (torch.float32, torch.bfloat16) or \
+                ev.dtype not in (torch.bool, torch.float32) or idx.shape[0] == 0:
+            return None
+        key = self._key(d, idx, ptr, m, n, host_w)
+        with self._lock:
+            ent = self._plans.get(key)
+            if ent is not None:
+                self._plans.move_to_end(key)
+                self.hits += 1
+                return ent[0]
+            if key not in self._seen:
+                self._seen[key] = True
+                while len(self._seen) > 4 * self.capacity:
+                    self._seen.popitem(last=False)
+                return None
+            del self._seen[key]
+        try:
+            plan = TiledCSR(d, idx, ptr, shape=(m, n), weight_dtype=d.dtype if d.shape[0] > 1 else None)
+        except _lib.BPXError as e:
+            if "inconsistent shapes" not in str(e):
+                raise
+            plan = None     # outside the plan's limits: stateless from now on
+        with self._lock:
+            self.builds += 1
+            self._plans[key] = (plan, (d, idx, ptr))
+            self._evict()
+        return plan
+
+    def _evict(self):
+        def total():
+            return sum(p.mem.numel() for p, _ in self._plans.values() if p is not None)
+        while len(self._plans) > self.capacity or (len(self._plans) > 1 and total() > self.max_bytes):
+            self._plans.popitem(last=False)
+
+    def clear(self):
+        with self._lock:
+            self._plans.clear()
+            self._seen.clear()
+
+    def info(self):
+        with self._lock:
+            return {"plans": sum(p is not None for p, _ in self._plans.values()),
+                    "ineligible": sum(p is None for p, _ in self._plans.values()),
+                    "bytes": sum(p.mem.numel() for p, _ in self._plans.values() if p is not None),
+                    "builds": self.builds, "hits": self.hits, "capacity": self.capacity,
+                    "max_bytes": self.max_bytes, "enabled": self.enabled}
+
+
+_PLANS = _PlanCache()
+
+
+def plan_cache_info():
+    """Counters of the plan cache behind ``csrmv(..., transpose=True)``."""
+    return _PLANS.info()
+
+
+def clear_plan_cache():
+    """Drop every cached plan (and the references it holds)."""
+    _PLANS.clear()
+
+
+def set_plan_cache(enabled=None, capacity=None, max_bytes=None):
+    """Switch the plan cache on or off, or change its limits."""
+    with _PLANS._lock:
+        if enabled is not None:
+            _PLANS.enabled = bool(enabled)
+        if capacity is not None:
+            _PLANS.capacity = int(capacity)
+        if max_bytes is not None:
+            _PLANS.max_bytes = int(max_bytes)
+        _PLANS._evict()
 
 
 class _EventCSRMV(torch.autograd.Function):
@@ -129,6 +246,7 @@ class _EventCSRMV(torch.autograd.Function):
 
 
 def _run(data, indices, indptr, events, shape, transpose, rule):
+    host_w = float(data[0]) if isinstance(data, np.ndarray) and data.shape == (1,) else None
     d = data_tensor(data)
     idx = index_tensor(indices)
     ptr = index_tensor(indptr)
@@ -139,7 +257,7 @@ def _run(data, indices, indptr, events, shape, transpose, rule):
         d = d.to(torch.float32)
     if needs_ad(d, ev):
         return _EventCSRMV.apply(d, ev, idx, ptr, (int(shape[0]), int(shape[1])), bool(transpose), rule)
-    return _launch(d, idx, ptr, ev, shape, transpose, rule)
+    return _launch(d, idx, ptr, ev, shape, transpose, rule, host_w)
 
 
 def csrmv(data, indices, indptr, events, *, shape: Tuple[int, int], transpose: bool = False) -> torch.Tensor:
@@ -309,6 +427,7 @@ class TiledCSR:
                                          _lib.ptr(self.mem), self.mem.numel()), "bpx_tiled_csr_build")
         self.shape = (m, n)
         self.weight_dtype = wcode
+        self._plan_ref = ctypes.byref(self.plan)
 
     def csrmv(self, events, *, out: torch.Tensor = None, rule: int = _lib.RULE_POSITIVE) -> torch.Tensor:
         """``events @ A`` (a new vector), or ``out += events @ A`` when
@@ -319,14 +438,19 @@ class TiledCSR:
         m, n = self.shape
         if ev.ndim != 1 or ev.shape[0] != m:
             raise ValueError(f'Shape mismatch, vec ({ev.shape[0]},) @ mat {self.shape}.')
+        if out is not None and (out.dtype != torch.float32 or out.shape != (n,) or not out.is_contiguous()):
+            raise ValueError(f'out must be a contiguous float32 vector of shape ({n},).')
+        return self._exec(ev.contiguous(), rule, out)
+
+    def _exec(self, ev, rule, out):
+        """Launch on a contiguous bool/f32 event vector of length m."""
         accumulate = out is not None
         if out is None:
-            out = torch.empty(n, dtype=torch.float32, device=ev.device)
-        elif out.dtype != torch.float32 or out.shape != (n,) or not out.is_contiguous():
-            raise ValueError(f'out must be a contiguous float32 vector of shape ({n},).')
-        rc = _lib.lib().bpx_event_csrmv_tiled(_lib.stream_ptr(), ctypes.byref(self.plan), _lib.ptr(ev),
+            out = torch.empty(self.shape[1], dtype=torch.float32, device=ev.device)
+        rc = _lib.lib().bpx_event_csrmv_tiled(_lib.stream_ptr(), self._plan_ref, _lib.ptr(ev),
                                               _lib.bpx_dtype(ev), rule, int(accumulate), _lib.ptr(out))
-        _lib.check(rc, "bpx_event_csrmv_tiled")
+        if rc != 0:
+            _lib.check(rc, "bpx_event_csrmv_tiled")
         return out
 
     def matvec(self, x, *, out: torch.Tensor = None) -> torch.Tensor:

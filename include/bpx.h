@@ -424,20 +424,61 @@ int bpx_fixed_prob_csr(void* stream, int64_t m, int64_t n, int64_t k, uint64_t s
                        int allow_multi_conn, int32_t* indices, int32_t* indptr, void* workspace,
                        size_t workspace_bytes);
 
-/*
- * Measurement helper, no counterpart in the reference: the ceilings of the
- * COO access pattern on a given (row, col, data) set -- mode 0 streams row,
- * col and data (12 B per entry), mode 1 streams col and gathers src[col],
- * mode 2 does both -- with one f32 sum per thread written to out (out_len a
- * multiple of 256; one 256-thread block per 256 entries of out, at most
- * 65536 blocks).  nse must be a multiple of 4, the arrays 16-byte aligned,
- * and col values inside [0, src_len).
- */
-int bpx_coo_probe(void* stream, const int32_t* row, const int32_t* col, const float* data, const float* src,
-                  int64_t src_len, int64_t nse, int mode, float* out, int64_t out_len);
+/* -------------------------------------------------------------------------
+ * XLA custom-call adapters.
+ *
+ * The reference registers its GPU operators as legacy XLA custom calls,
+ * `xla_client.register_custom_call_target(name, capsule, platform)`
+ * (dependency_check.py:75-79), with the GPU convention
+ * `void fn(stream, void** buffers, const char* opaque, size_t opaque_len)`:
+ * buffers are the operands in the order of the reference's lowering,
+ * followed by the outputs.  These adapters have that signature and wrap the
+ * entry points above (same kernels).  Their opaque is a bpx_xla_desc (built
+ * by the host binding in place of brainpylib's build_double_size_descriptor /
+ * build_nonzero_descriptor, which carry only the shape): it adds the dtypes
+ * and lengths the kernels need.  The legacy convention has no error
+ * channel: the status of the calling thread's last adapter call is
+ * bpx_xla_last_status().
+ *
+ *  bpx_xla_event_csrmv  replaces event_csrmv_{homo,heter}_* targets
+ *    (event/_csr_matvec.py:453-500): buffers (data, indices, indptr, events, out).
+ *  bpx_xla_csrmv        replaces csrmv_{homo,heter}_vector_* targets
+ *    (sparse/_csr_mv.py:328-361): buffers (data, indices, indptr, vector, out).
+ *  bpx_xla_event_info   replaces nonzero_* targets (event/_info_collection.py:
+ *    78-121): buffers (events, ids, num, workspace) -- the workspace is a third
+ *    output of bpx_event_info_workspace(batch, n) bytes in ws_bytes.
+ *  bpx_xla_jitconn_mv   replaces gpu_jit_{event_,}csrmv_{atomic_,}prob_* targets
+ *    (jitconn/_event_matvec.py:198-229, 384-414, 578-608; jitconn/_matvec.py:
+ *    341-378, 535-574, 717-760): buffers (vec, weight, clen, seed, out) for homo, (vec, w_low|w_mu,
+ *    w_high|w_sigma, clen, seed, out) otherwise, then an optional bitmask
+ *    workspace of ws_bytes (bpx_jitconn_workspace).  weight/clen/seed are
+ *    one-element device arrays there; the adapter reads them with one
+ *    synchronous copy per call.
+ * ------------------------------------------------------------------------- */
+#define BPX_XLA_MAGIC 0x31585042u   /* "BPX1" */
+typedef struct {
+  uint32_t magic;           /* BPX_XLA_MAGIC */
+  int32_t data_dtype;       /* csrmv weights; jitconn weight operands (F32 or F64) */
+  int32_t index_dtype;      /* csrmv indices / indptr */
+  int32_t vec_dtype;        /* events or vector */
+  int32_t rule;             /* BPX_RULE_* for float events */
+  int32_t transpose;        /* csrmv */
+  int32_t dist;             /* jitconn BPX_DIST_* */
+  int32_t outdim_parallel;  /* jitconn */
+  int32_t layout;           /* jitconn BPX_LAYOUT_* */
+  int32_t event;            /* jitconn: 1 event operator, 0 dense-vector operator */
+  int64_t m, n;             /* csrmv shape; jitconn (num_row, num_col) after the transpose swap; info n */
+  int64_t nnz;              /* csrmv len(indices) */
+  int64_t data_len;         /* csrmv 1 (homogeneous) or nnz */
+  int64_t batch;            /* info rows */
+  int64_t ws_bytes;         /* bytes of the trailing workspace buffer (0: none) */
+} bpx_xla_desc;
 
-/* Plain device copy (bandwidth calibration of the HBM counters). */
-int bpx_copy(void* stream, const void* src, void* dst, int64_t bytes);
+void bpx_xla_event_csrmv(void* stream, void** buffers, const char* opaque, size_t opaque_len);
+void bpx_xla_csrmv(void* stream, void** buffers, const char* opaque, size_t opaque_len);
+void bpx_xla_event_info(void* stream, void** buffers, const char* opaque, size_t opaque_len);
+void bpx_xla_jitconn_mv(void* stream, void** buffers, const char* opaque, size_t opaque_len);
+int bpx_xla_last_status(void);
 
 /*
  * Kernel timer.  While enabled, the dominant kernel of each entry point is
@@ -445,7 +486,7 @@ int bpx_copy(void* stream, const void* src, void* dst, int64_t bytes);
  * synchronises those events and returns the summed device milliseconds and
  * launch count for one kernel id (0 event scatter, 1 event gather,
  * 2 csr scatter, 3 csr gather, 4 jitconn outdim, 5 jitconn scatter,
- * 6 event info, 7 lif step, 8 fixed order, 9 copy, 10 network step,
+ * 6 event info, 7 lif step, 8 fixed order, 10 network step,
  * 11 tiled event scatter, 12 COO products, 13 BSR products).
  */
 int bpx_timer_enable(int on);

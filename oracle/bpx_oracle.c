@@ -213,6 +213,52 @@ int orc_event_csrmv_scatter_f32(const float* data, int64_t data_len, const int32
   return 0;
 }
 
+/* The same scatter spread over threads (the CPU baseline's all-thread
+ * figure; the reference's own CPU transpose scatter is serial): thread t
+ * scatters a contiguous block of rows into a private output, then the
+ * private outputs are summed column by column.  Sums of a column are taken
+ * in another order than the serial loop's, so results agree within f32
+ * rounding, not bit for bit. */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+int orc_event_csrmv_scatter_f32_threads(const float* data, int64_t data_len, const int32_t* indices,
+                                        const int32_t* indptr, const uint8_t* ev, int64_t m, int64_t n,
+                                        float* out, int threads, float* scratch) {
+  /* scratch: threads * n floats */
+#ifdef _OPENMP
+  if (threads < 1) threads = 1;
+#pragma omp parallel num_threads(threads)
+  {
+    const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+    float* acc = scratch + (int64_t)t * n;
+    for (int64_t i = 0; i < n; ++i) acc[i] = 0.0f;
+    const int64_t r0 = m * t / nt, r1 = m * (t + 1) / nt;
+    for (int64_t r = r0; r < r1; ++r) {
+      if (!ev[r]) continue;
+      if (data_len == 1) {
+        const float w = data[0];
+        for (int32_t j = indptr[r]; j < indptr[r + 1]; ++j) acc[indices[j]] += w;
+      } else {
+        for (int32_t j = indptr[r]; j < indptr[r + 1]; ++j) acc[indices[j]] += data[j];
+      }
+    }
+#pragma omp barrier
+#pragma omp for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      float v = 0.0f;
+      for (int k = 0; k < nt; ++k) v += scratch[(int64_t)k * n + i];
+      out[i] = v;
+    }
+  }
+  return 0;
+#else
+  (void)threads;
+  (void)scratch;
+  return orc_event_csrmv_scatter_f32(data, data_len, indices, indptr, ev, m, n, out);
+#endif
+}
+
 /* ------------------------------------------------------------------------ */
 /* sparse csrmv (sparse/_csr_mv.py:126-165)                                  */
 /*  transpose=True : serial scatter `out[col] += w * v[row]` (f32 accum,     */

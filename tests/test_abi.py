@@ -32,6 +32,26 @@ def test_library_exports_every_declared_symbol():
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libbpx.so not built")
+def test_measurement_probes_stay_out_of_the_product_library():
+    lib = ctypes.CDLL(LIB)
+    for name in ("bpx_coo_probe", "bpx_copy", "bpx_probe_coo", "bpx_probe_copy"):
+        assert not hasattr(lib, name), name
+    probe = os.path.join(ROOT, "brainpy_amd", "libbpx_probe.so")
+    if os.path.exists(probe):
+        P = ctypes.CDLL(probe)
+        f = P.bpx_probe_coo
+        f.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+                                              ctypes.c_int64]
+        assert f(None, None, None, None, None, 1, 10, 1, None, 256) == -2       # nse % 4
+        assert f(None, None, None, None, None, 1, 8, 1, None, 100) == -2        # out_len % 256
+        assert f(None, None, None, None, None, 1, 8, 3, None, 256) == -2        # mode
+        assert f(None, None, None, None, None, 1, 8, 1, None, 256) == -5        # null arrays
+        g = P.bpx_probe_copy
+        g.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+        assert g(None, None, None, -1) == -5 and g(None, None, None, 0) == 0
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libbpx.so not built")
 def test_host_only_entry_points():
     lib = ctypes.CDLL(LIB)
     lib.bpx_status_string.restype = ctypes.c_char_p
@@ -156,9 +176,6 @@ def test_argument_validation_of_coo_conn_and_plan_entry_points():
     assert L.bpx_coo_matmat(None, None, F32, 1, None, None, I32, 10, None, F32, 4, 5, -1, 1, None) == SHAPE
     assert L.bpx_coo_matmat(None, None, F32, 1, None, None, I32, 10, None, I32, 4, 5, 2, 1, None) == DTYPE
     assert L.bpx_coo_sddmm(None, None, None, I32, -1, None, 1, 1, 0, None, 1, 1, 0, 1, F32, None) == SHAPE
-    assert L.bpx_coo_probe(None, None, None, None, None, 1, 10, 1, None, 256) == SHAPE         # nse % 4
-    assert L.bpx_coo_probe(None, None, None, None, None, 1, 8, 1, None, 100) == SHAPE          # out_len % 256
-    assert L.bpx_coo_probe(None, None, None, None, None, 1, 8, 3, None, 256) == SHAPE          # mode
     assert L.bpx_fixed_prob_csr(None, 10, 5, 6, 1, 1, 0, None, None, None, 0) == SHAPE         # k > n
     assert L.bpx_fixed_prob_csr(None, 10, 20, 5, 1, 0, 0, None, None, None, 0) == SHAPE        # no self, m != n
     assert L.bpx_fixed_prob_csr(None, 10, 2_000_000, 5, 1, 1, 0, None, None, None, 0) == ARG   # null outputs

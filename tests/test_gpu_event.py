@@ -264,7 +264,7 @@ def test_tiled_plan_vs_oracle(shape, wkind, sort):
 
 @pytest.mark.parametrize("shape", [(100, 200), (3000, 50), (5000, 100000)])
 @pytest.mark.parametrize("wkind", ["homo", "f32", "bf16"])
-@pytest.mark.parametrize("xscale", [1.0, 3e5, 1e-4])
+@pytest.mark.parametrize("xscale", [1.0, 3e5, 1e-4, 1e-9, 1e-20])
 def test_tiled_plan_dense_matvec(shape, wkind, xscale):
     # x @ A for a dense vector with zeros, on the plan, against the FP64
     # dense product within 1e-5 of the sum of term magnitudes (the products

@@ -1,0 +1,140 @@
+"""The plan cache behind bm.event.csrmv(..., transpose=True): the second call
+on the same (data, indices, indptr) builds a tiled plan and later calls run
+on it; in-place writes (torch ops, the STDP kernels) invalidate it; results
+stay equal to the oracle's on every call."""
+import numpy as np
+import pytest
+import torch
+
+import brainpy_amd.math as bm
+from brainpy_amd import dnn
+from oracle import orc
+from tests.helpers import bernoulli, dense_of, fixed_prob_csr
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+@pytest.fixture(autouse=True)
+def fresh_cache():
+    bm.event.clear_plan_cache()
+    bm.event.set_plan_cache(enabled=True, capacity=8)
+    yield
+    bm.event.clear_plan_cache()
+    bm.event.set_plan_cache(enabled=True, capacity=8)
+
+
+def _inputs(m=2000, n=3000, p=0.02, seed=3):
+    idx, ptr = fixed_prob_csr(m, n, p, seed)
+    w = np.random.default_rng(seed).normal(0.5, 0.1, idx.size).astype(np.float32)
+    return idx, ptr, w
+
+
+def _check(y, w, idx, ptr, ev, shape):
+    D = dense_of(w, idx, ptr, shape)
+    on = np.asarray(ev, np.float64)
+    ref = on @ D
+    assert np.all(np.abs(y.cpu().numpy() - ref) <= 1e-5 * (on @ np.abs(D)) + 1e-30)
+
+
+def test_second_call_builds_third_hits():
+    idx, ptr, w = _inputs()
+    t = [torch.from_numpy(a).to(DEV) for a in (w, idx, ptr)]
+    ev = torch.from_numpy(bernoulli(2000, 0.1, 5)).to(DEV)
+    for k in range(4):
+        y = bm.event.csrmv(*t, ev, shape=(2000, 3000), transpose=True)
+        _check(y, w, idx, ptr, ev.cpu().numpy(), (2000, 3000))
+        info = bm.event.plan_cache_info()
+        assert info["builds"] == (0 if k == 0 else 1)
+        assert info["hits"] == max(0, k - 1)
+    assert bm.event.plan_cache_info()["plans"] == 1
+
+
+def test_homogeneous_integer_counts_exact_on_plan():
+    idx, ptr, _ = _inputs(seed=4)
+    ti, tp = torch.from_numpy(idx).to(DEV), torch.from_numpy(ptr).to(DEV)
+    ev = bernoulli(2000, 0.3, 6)
+    ref = orc.event_csrmv(np.float32(1.0), idx, ptr, ev, (2000, 3000), True)
+    for _ in range(3):
+        y = bm.event.csrmv(1.0, ti, tp, torch.from_numpy(ev).to(DEV), shape=(2000, 3000), transpose=True)
+        np.testing.assert_array_equal(y.cpu().numpy(), ref)
+    # a host scalar weight is keyed by value: another value gets its own plan
+    for _ in range(3):
+        y = bm.event.csrmv(2.5, ti, tp, torch.from_numpy(ev).to(DEV), shape=(2000, 3000), transpose=True)
+        np.testing.assert_array_equal(y.cpu().numpy(), ref * np.float32(2.5))
+    assert bm.event.plan_cache_info()["plans"] == 2
+
+
+def test_in_place_writes_invalidate():
+    idx, ptr, w = _inputs(seed=5)
+    tw, ti, tp = (torch.from_numpy(a).to(DEV) for a in (w, idx, ptr))
+    ev = torch.from_numpy(bernoulli(2000, 0.2, 7)).to(DEV)
+    for _ in range(3):
+        bm.event.csrmv(tw, ti, tp, ev, shape=(2000, 3000), transpose=True)
+    tw.mul_(2.0)      # torch in-place op: version bump
+    w2 = w * np.float32(2.0)
+    for _ in range(3):
+        _check(bm.event.csrmv(tw, ti, tp, ev, shape=(2000, 3000), transpose=True), w2, idx, ptr,
+               ev.cpu().numpy(), (2000, 3000))
+    # the STDP kernel writes the weights in place through its raw pointer
+    spike = bernoulli(2000, 0.5, 8)
+    trace = np.random.default_rng(9).normal(0.0, 0.05, 3000).astype(np.float32)
+    dnn.csr_on_pre_update(tw, ti, tp, spike, trace, 0.0, 2.0, out=tw)
+    w3 = orc.csr_on_pre_update(w2, idx, ptr, spike, trace, 0.0, 2.0)
+    np.testing.assert_array_equal(tw.cpu().numpy(), w3)
+    for _ in range(3):
+        _check(bm.event.csrmv(tw, ti, tp, ev, shape=(2000, 3000), transpose=True), w3, idx, ptr,
+               ev.cpu().numpy(), (2000, 3000))
+
+
+def test_layer_steps_run_on_one_plan():
+    from brainpy_amd.connect import FixedProb
+    conn = FixedProb(0.02, 4000, 5000, seed=2)
+    w = np.random.default_rng(1).normal(0.5, 0.1, conn.indices.size).astype(np.float32)
+    lay = dnn.EventCSRLinear(conn, w)
+    for k in range(6):
+        ev = bernoulli(4000, 0.05, 100 + k)
+        _check(lay(ev), w, conn.indices, conn.indptr, ev, (4000, 5000))
+    info = bm.event.plan_cache_info()
+    assert info["builds"] == 1 and info["hits"] == 4
+
+
+def test_ineligible_and_disabled_run_stateless():
+    # outputs beyond the plan's limit, f64 weights, f64 events and a disabled
+    # cache all take the stateless scatter with correct results
+    m, n = 50, 300000
+    idx, ptr = fixed_prob_csr(m, n, 0.001, 11)
+    w = np.random.default_rng(2).normal(0.5, 0.1, idx.size).astype(np.float32)
+    tw, ti, tp = (torch.from_numpy(a).to(DEV) for a in (w, idx, ptr))
+    ev = torch.from_numpy(bernoulli(m, 0.5, 3)).to(DEV)
+    for _ in range(3):
+        _check(bm.event.csrmv(tw, ti, tp, ev, shape=(m, n), transpose=True), w, idx, ptr, ev.cpu().numpy(), (m, n))
+    info = bm.event.plan_cache_info()
+    assert info["plans"] == 0 and info["ineligible"] == 1
+    idx, ptr, w = _inputs(seed=6)
+    t64 = torch.from_numpy(w.astype(np.float64)).to(DEV)
+    ti, tp = torch.from_numpy(idx).to(DEV), torch.from_numpy(ptr).to(DEV)
+    ev = torch.from_numpy(bernoulli(2000, 0.2, 4)).to(DEV)
+    for _ in range(3):
+        y = bm.event.csrmv(t64, ti, tp, ev, shape=(2000, 3000), transpose=True)
+        assert y.dtype == torch.float64
+    bm.event.set_plan_cache(enabled=False)
+    tw = torch.from_numpy(w).to(DEV)
+    for _ in range(3):
+        _check(bm.event.csrmv(tw, ti, tp, ev, shape=(2000, 3000), transpose=True), w, idx, ptr, ev.cpu().numpy(),
+               (2000, 3000))
+    assert bm.event.plan_cache_info()["builds"] == 1
+
+
+def test_capacity_evicts_least_recent():
+    bm.event.set_plan_cache(capacity=2)
+    mats = [_inputs(m=500, n=700, seed=20 + k) for k in range(3)]
+    ts = [[torch.from_numpy(a).to(DEV) for a in (w, idx, ptr)] for idx, ptr, w in mats]
+    ev = torch.from_numpy(bernoulli(500, 0.2, 1)).to(DEV)
+    for t in ts:
+        for _ in range(2):
+            bm.event.csrmv(*t, ev, shape=(500, 700), transpose=True)
+    assert bm.event.plan_cache_info()["plans"] == 2
+    for (idx, ptr, w), t in zip(mats, ts):
+        _check(bm.event.csrmv(*t, ev, shape=(500, 700), transpose=True), w, idx, ptr, ev.cpu().numpy(), (500, 700))
