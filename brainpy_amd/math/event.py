@@ -169,6 +169,8 @@ class _PlanCache:
         with self._lock:
             self._plans.clear()
             self._seen.clear()
+            self.builds = 0
+            self.hits = 0
 
     def info(self):
         with self._lock:
@@ -188,7 +190,8 @@ def plan_cache_info():
 
 
 def clear_plan_cache():
-    """Drop every cached plan (and the references it holds)."""
+    """Drop every cached plan (and the references it holds) and reset the
+    counters."""
     _PLANS.clear()
 
 

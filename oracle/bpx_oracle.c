@@ -734,13 +734,14 @@ int orc_lif_coba_step_f64(int64_t n, int ltc, int n_g,
 /* form (row p lists network-local targets; targets of E neurons feed g_exc, */
 /* of I neurons g_inh).  Homogeneous weights: the scatter output of a target */
 /* receiving c spikes is ((0 + w) + w) + ... (c terms), computed as such.    */
-/* raster (nullable) receives steps x N spike flags; returns total spikes.   */
+/* raster (nullable) receives steps x N spike flags, V_out (nullable) the    */
+/* final potentials; returns total spikes.                                   */
 /* ------------------------------------------------------------------------ */
 int64_t orc_coba_net_run(int32_t n_exc, int32_t n_inh, const int32_t* indptr, const int32_t* indices,
                          float w_exc, float w_inh, float I_ext, const float* V0, int64_t steps, int ltc,
                          float V_rest, float V_th, float V_reset, float tau, float tau_ref, float R,
                          float tau_e, float tau_i, float E_e, float E_i, float dt, uint8_t* raster,
-                         const int32_t* delay4) {
+                         const int32_t* delay4, float* V_out) {
   /* delay4: steps of delay of the projections E2E, E2I, I2E, I2I (NULL: all
    * 1).  The projection with delay d reads the spikes of step i - d at step
    * i (delay.py:305-349, rotation update with the delay data written after
@@ -789,6 +790,7 @@ int64_t orc_coba_net_run(int32_t n_exc, int32_t n_inh, const int32_t* indptr, co
     for (int32_t j = 0; j < N; ++j) total += sp[j];
     if (raster) memcpy(raster + i * N, sp, N);
   }
+  if (V_out) memcpy(V_out, V, sizeof(float) * N);
   free(V); free(tl); free(ge); free(gi); free(oute); free(outi); free(sp); free(hist);
   return total;
 }

@@ -60,7 +60,7 @@ def lib():
         L.orc_lif_coba_step.argtypes = [i64, i32, i32, vp, vp, vp, fp, vp, vp, vp, f32, vp,
                                         f32, f32, f32, f32, f32, f32, f32, f32]
         L.orc_coba_net_run.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, vp, f32, f32, f32, vp, i64, i32,
-                                       f32, f32, f32, f32, f32, f32, f32, f32, f32, f32, f32, vp, vp]
+                                       f32, f32, f32, f32, f32, f32, f32, f32, f32, f32, f32, vp, vp, vp]
         L.orc_coba_net_run.restype = ctypes.c_int64
         d = ctypes.c_double
         L.orc_lif_coba_step_f64.argtypes = [i64, i32, i32, vp, vp, vp, fp, vp, vp, d, vp,
@@ -222,20 +222,24 @@ def lif_coba_step_f64(V, t_last, spike, gs, tau_g, E_g, I_ext, params, t, dt, lt
 
 
 def coba_net_run(n_exc, n_inh, indptr, indices, w_exc, w_inh, I_ext, V0, steps, params, tau_e=5.0, tau_i=10.0,
-                 E_e=0.0, E_i=-80.0, dt=0.1, ltc=True, raster=True, delays=None):
+                 E_e=0.0, E_i=-80.0, dt=0.1, ltc=True, raster=True, delays=None, return_v=False):
     """Serial CPU run of a whole COBA network from its combined CSR; returns
-    (total spikes, raster [steps, N] uint8 or None).  `delays`: steps of
-    delay of the E2E, E2I, I2E, I2I projections (None: all 1)."""
+    (total spikes, raster [steps, N] uint8 or None), plus the final
+    potentials with `return_v`.  `delays`: steps of delay of the E2E, E2I,
+    I2E, I2I projections (None: all 1)."""
     d4 = None if delays is None else np.ascontiguousarray(delays, np.int32)
     N = n_exc + n_inh
     indptr = np.ascontiguousarray(indptr, np.int32)
     indices = np.ascontiguousarray(indices, np.int32)
     V0 = np.ascontiguousarray(V0, np.float32)
     r = np.zeros((steps, N), np.uint8) if raster else None
+    v_out = np.empty(N, np.float32) if return_v else None
     tot = lib().orc_coba_net_run(n_exc, n_inh, _p(indptr), _p(indices), np.float32(w_exc), np.float32(w_inh),
                                  np.float32(I_ext), _p(V0), steps, int(ltc), *[np.float32(x) for x in params],
                                  np.float32(tau_e), np.float32(tau_i), np.float32(E_e), np.float32(E_i),
-                                 np.float32(dt), _p(r), _p(d4))
+                                 np.float32(dt), _p(r), _p(d4), _p(v_out))
+    if return_v:
+        return tot, r, v_out
     return tot, r
 
 

@@ -64,3 +64,43 @@ def test_dense_forms():
     ref = W.copy()
     ref[:, sp_post] = np.clip(W[:, sp_post] + tr_pre[:, None], np.float32(-0.5), np.float32(0.8))
     np.testing.assert_array_equal(orc.dense_on_spike(0, W, sp_post, tr_pre, -0.5, 0.8), ref)
+
+
+def _dense(w, ind, ptr, shape):
+    W = np.full(shape, np.nan, np.float32)
+    rows = np.repeat(np.arange(shape[0]), np.diff(ptr))
+    W[rows, ind] = w
+    return W
+
+
+@pytest.mark.parametrize("bounds", [(-np.inf, np.inf), (0.0, 1.0), (0.6, 0.2)])
+def test_stdp_against_dense_matrix_definition(bounds):
+    # The definition of dnn/linear.py:650-700 on the dense weight matrix (no
+    # CSR arithmetic): a spiking pre neuron i moves every existing synapse
+    # (i, j) by trace[j]; a spiking post neuron j moves every existing
+    # synapse (i, j) by trace[i]; both then take
+    # np.minimum(np.maximum(x, w_min), w_max) (with w_min > w_max that is
+    # w_max).  FixedProb rows hold distinct targets, so (i, j) is one synapse.
+    lo, hi = bounds
+    m, n = 9, 7
+    ind, ptr = fixed_prob_csr(m, n, 0.45, seed=6)
+    rng = np.random.default_rng(7)
+    w = rng.normal(0.4, 0.3, ind.size).astype(np.float32)
+    exists = ~np.isnan(_dense(w, ind, ptr, (m, n)))
+    sp_pre = rng.random(m) < 0.5
+    tr_post = rng.normal(0.0, 0.4, n).astype(np.float32)
+    W = _dense(w, ind, ptr, (m, n))
+    upd = exists & sp_pre[:, None]
+    W[upd] = np.minimum(np.maximum((W + tr_post[None, :])[upd], np.float32(lo)), np.float32(hi))
+    got = _dense(orc.csr_on_pre_update(w, ind, ptr, sp_pre, tr_post, lo, hi), ind, ptr, (m, n))
+    np.testing.assert_array_equal(got[exists], W[exists])
+    sp_post = rng.random(n) < 0.5
+    tr_pre = rng.normal(0.0, 0.4, m).astype(np.float32)
+    W = _dense(w, ind, ptr, (m, n))
+    upd = exists & sp_post[None, :]
+    W[upd] = np.minimum(np.maximum((W + tr_pre[:, None])[upd], np.float32(lo)), np.float32(hi))
+    pre_ids, colptr, w_ids = csr_to_csc(ind, ptr, (m, n), data=np.arange(ind.size))
+    got = _dense(orc.csc_on_post_update(w, pre_ids, colptr, w_ids, sp_post, tr_pre, lo, hi), ind, ptr, (m, n))
+    np.testing.assert_array_equal(got[exists], W[exists])
+    if lo > hi:
+        assert np.all(got[upd] == np.float32(hi))

@@ -67,3 +67,20 @@ def test_normal_moments():
         k, v = np_ref.normal(k, 0.5, 0.1)
         out.append(v[0])
     np.testing.assert_allclose(z[:64], np.array(out, np.float64), rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("seed", [5, 42, 2024])
+def test_normal_is_box_muller_sine_of_consecutive_uniforms(seed):
+    # lfsr88_normal (tifunc.py:89-133): u1 then u2 from the (pinned) uniform
+    # stream, u1 floored at 1e-10, z = sqrt(-2 ln u1) * sin(2 pi u2),
+    # mu + sigma * z.  Recomputed here in FP64 from the uniform stream; a
+    # sine/cosine or u1/u2 swap keeps the moments but fails this.
+    mu, sigma, count = 0.5, 0.1, 4000
+    u = orc.lfsr88_uniform_stream(seed, 0.0, 1.0, 2 * count).astype(np.float64)
+    u1, u2 = np.maximum(u[0::2], 1e-10), u[1::2]
+    ref = mu + sigma * np.sqrt(-2.0 * np.log(u1)) * np.sin(2.0 * np.pi * u2)
+    z = orc.lfsr88_normal_stream(seed, mu, sigma, count).astype(np.float64)
+    assert np.max(np.abs(z - ref)) <= 2e-6 * (abs(mu) + 5 * sigma)
+    cos = mu + sigma * np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+    swap = mu + sigma * np.sqrt(-2.0 * np.log(np.maximum(u2, 1e-10))) * np.sin(2.0 * np.pi * u[0::2])
+    assert np.max(np.abs(z - cos)) > 0.1 and np.max(np.abs(z - swap)) > 0.1
