@@ -13,6 +13,8 @@
 // tile's ids into LDS (block scan of the lane counts) and stores them as one
 // coalesced run.
 // The result equals the serial loop of event/_info_collection.py:143-154.
+#include <mutex>
+
 #include "bpx_common.h"
 
 namespace bpx {
@@ -298,6 +300,36 @@ __global__ void k_zero_i32(int32_t* p, int64_t n) {
   if (i < n) p[i] = 0;
 }
 
+// Scratch of bpx_event_info_noalloc: the tile counts and tile offsets that
+// bpx_event_info keeps in its workspace, in a static device array (one copy
+// per GPU, held until the process ends).  Host code never passes `scratch`
+// itself (on the host it only names the symbol): its device address is
+// looked up once per device with hipGetSymbolAddress and kept in
+// g_scratch_ptr.  Every call writes all the counts and offsets it reads, so
+// nothing has to start at zero; calls that overlap on one GPU race.
+constexpr size_t kInfoScratchBytes = BPX_EVENT_INFO_SCRATCH_BYTES;
+static __device__ __align__(256) int32_t scratch[kInfoScratchBytes / sizeof(int32_t)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static int32_t* g_scratch_ptr[kMaxDevices];
+
+// device address of `scratch` on the current device (looked up on the
+// first call on that device, with the device current)
+static int info_scratch(int32_t** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return BPX_ERR_LAUNCH;
+  if (dev < 0 || dev >= kMaxDevices) return BPX_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return BPX_ERR_LAUNCH;
+    g_scratch_ptr[dev] = (int32_t*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return BPX_OK;
+}
+
 }  // namespace bpx
 
 using namespace bpx;
@@ -331,6 +363,24 @@ int bpx_event_info(void* stream, const void* events, int ev_dtype, int64_t batch
     case BPX_I32: return run_info<int32_t>(s, events, batch, n, ids, num, (int32_t*)workspace);
     default: return BPX_ERR_DTYPE;
   }
+}
+
+int bpx_event_info_noalloc(void* stream, const void* events, int ev_dtype, int64_t batch, int64_t n, int32_t* ids,
+                           int32_t* num) {
+  if (batch < 0 || n < 0) return BPX_ERR_SHAPE;
+  if (batch > 65535) return BPX_ERR_SHAPE;
+  if (ev_dtype != BPX_BOOL && ev_dtype != BPX_F32 && ev_dtype != BPX_F64 && ev_dtype != BPX_I32) return BPX_ERR_DTYPE;
+  if (batch == 0) return BPX_OK;
+  if (!num || (n > 0 && (!events || !ids))) return BPX_ERR_ARG;
+  if (n > INT32_MAX) return BPX_ERR_SHAPE;
+  const size_t need = bpx_event_info_workspace(batch, n);
+  if (need > kInfoScratchBytes) return BPX_ERR_WORKSPACE;
+  int32_t* ws = nullptr;
+  if (n > 0) {
+    const int rc = info_scratch(&ws);
+    if (rc != BPX_OK) return rc;
+  }
+  return bpx_event_info(stream, events, ev_dtype, batch, n, ids, num, ws, need);
 }
 
 }  // extern "C"

@@ -71,6 +71,21 @@ int bpx_event_info(void* stream, const void* events, int ev_dtype, int64_t batch
                    int32_t* ids, int32_t* num, void* workspace, size_t workspace_bytes);
 
 /*
+ * bpx_event_info without a workspace and without any allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The workspace lives in a static device array of
+ * BPX_EVENT_INFO_SCRATCH_BYTES per GPU, held until the process ends;
+ * sizes whose bpx_event_info_workspace(batch, n) exceeds it return
+ * BPX_ERR_WORKSPACE (e.g. a batch of 65535 rows of 100,000 events).  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (on
+ * different streams) race.  Results equal bpx_event_info's.
+ */
+#define BPX_EVENT_INFO_SCRATCH_BYTES (4u << 20)
+int bpx_event_info_noalloc(void* stream, const void* events, int ev_dtype, int64_t batch, int64_t n,
+                           int32_t* ids, int32_t* num);
+
+/*
  * Event-driven CSR matvec.  Replaces the brainpylib targets
  * `event_csrmv{_homo,_heter}{_float,_double}{_int,_long}{_bool,}{_transpose,}`
  * (event/_csr_matvec.py:453-500) and the Taichi kernels of csrmv_taichi
