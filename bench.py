@@ -695,94 +695,121 @@ def _current_error_f64(torch, specs, bf16, steps, stored_weights=False):
     return {"max": float(w.max()), "mean": float(w.mean()), "steps": steps}
 
 
-def bench_batched(_lib, torch, steps=2000, n_nets=256, plain_sample=8, plain_steps=100, err_steps=400):
+def _plain_service(torch, spec, steps, window=100):
+    """Device-bound time of one network on the plain FP32 path: the
+    reference's per-network step (four event jitconn projections, g += out,
+    two population updates; ~14 launches) captured in a HIP graph of
+    `window` steps with the step index on the device -- the counterpart of
+    the reference's one compiled scan per simulation -- replayed until
+    `steps` steps are done (a shorter tail window is captured for the rest).
+    Graph capture is outside the timed region, as compilation is."""
+    from brainpy_amd.dyn.batched import PlainCOBA
+    p = PlainCOBA(spec)
+    k = min(window, steps)
+    p.capture(k)
+    rest = steps % k
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps // k):
+        p.replay()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    if rest:
+        p.capture(rest)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p.replay()
+        torch.cuda.synchronize()
+        t += time.perf_counter() - t0
+    return t, int(p.i)
+
+
+def bench_batched(_lib, torch, steps=2000, n_nets=256, err_steps=400, plain=True):
     """C4: 256 independent E/I networks (4:1), N log-uniform in [1k, 64k],
     per-network seed, we ~ U(0.3, 0.9), wi ~ U(4, 9), input ~ U(15, 25) mV,
     jitconn in-degree 80, BF16 weights, FP32 state, 2000 steps, one launch
     per step for all networks, captured in HIP graphs of 100 steps.
-    Baseline: the plain per-network FP32 op-by-op path, timed on a sample of
-    networks (spread over the size range) and scaled to all 256 x 2000 steps."""
-    from brainpy_amd.dyn.batched import BatchedCOBA, PlainCOBA, bf16_round
-    specs = _c4_specs(n_nets)
-    b = BatchedCOBA(specs, bf16=True)
-    tune = b.autotune(steps=300)   # neurons per block, timed on this batch's first 300 steps (state restored)
-    print("# batched C4 autotune (us/step by tile): " + ", ".join(f"{k}: {v:.1f}" for k, v in tune.items())
-          + f" -> {b.tile}", file=sys.stderr, flush=True)
-    b.capture(100)
-    b.replay()
-    torch.cuda.synchronize()
-    b.reset()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps // 100):
+    Baseline: the plain per-network FP32 path over the whole workload (every
+    network, every step), each network one HIP graph of 100 steps replayed
+    20 times, networks one after the other."""
+    from brainpy_amd.dyn.batched import BatchedCOBA, bf16_round
+
+    def timed_batch(bf16):
+        b = BatchedCOBA(specs, bf16=bf16)
+        tune = b.autotune(steps=300)   # neurons per block, timed on the first 300 steps (state restored)
+        b.capture(100)
         b.replay()
-    torch.cuda.synchronize()
-    t_b = time.perf_counter() - t0
-    events = int(b.events.sum())
-    spikes = int(b.spike_count.sum())
-    total_neurons = b.total
-    # the same batch with FP32 homogeneous weights (scalar kernel arguments:
-    # same bytes, same kernel)
-    b32 = BatchedCOBA(specs, bf16=False)
-    b32.capture(100)
-    b32.replay()
-    torch.cuda.synchronize()
-    b32.reset()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps // 100):
-        b32.replay()
-    torch.cuda.synchronize()
-    t_b32 = time.perf_counter() - t0
-    events32 = int(b32.events.sum())
-    del b32
-    # plain FP32 baseline on a size-stratified sample
-    order = np.argsort([s["n_exc"] + s["n_inh"] for s in specs])
-    pick = order[np.linspace(0, len(order) - 1, plain_sample).astype(int)]
-    per_step = []
-    for k in pick:
-        p = PlainCOBA(specs[k])
-        p.run(5)
+        torch.cuda.synchronize()
+        b.reset()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        p.run(plain_steps)
+        for _ in range(steps // 100):
+            b.replay()
         torch.cuda.synchronize()
-        per_step.append((time.perf_counter() - t0) / plain_steps)
-    t_plain_est = float(np.mean(per_step)) * steps * n_nets
-    # per-step synaptic current against an FP64 twin (same spikes, unrounded
-    # weights): BF16-rounded homogeneous weights and FP32 weights
+        t = time.perf_counter() - t0
+        res = (t, int(b.events.sum()), int(b.spike_count.sum()), b.total, b.tile, tune)
+        del b
+        return res
+
+    specs = _c4_specs(n_nets)
+    t_b, events, spikes, total_neurons, tile, tune = timed_batch(True)
+    print("# batched C4 autotune (us/step by tile): " + ", ".join(f"{k}: {v:.1f}" for k, v in tune.items())
+          + f" -> {tile}", file=sys.stderr, flush=True)
+    # FP32 homogeneous weights: scalar kernel arguments, same bytes and kernel
+    t_b32, events32, spikes32, _, _, _ = timed_batch(False)
+    # per-step synaptic current against an FP64 twin (same spikes): BF16-rounded
+    # and FP32 homogeneous weights
     cur_err = {"bf16_weights": _current_error_f64(torch, specs, True, err_steps),
                "bf16_weights_vs_stored": _current_error_f64(torch, specs, True, err_steps, stored_weights=True),
                "f32_weights": _current_error_f64(torch, specs, False, err_steps)}
     rel = []
-    for s in specs:
+    for sp in specs:
         for key in ("w_exc", "w_inh"):
-            w = np.float64(np.float32(s[key]))
-            rel.append(abs(bf16_round(s[key]) - w) / w)
-    r = {"workload": "batched_coba_256nets_1k-64k_jit80_bf16_2000steps", "networks": n_nets,
-         "total_neurons": total_neurons, "steps": steps, "batched_s": t_b, "sims_per_s": n_nets / t_b,
-         "synaptic_events": events, "spikes": spikes, "events_per_s": events / t_b, "us_per_step": t_b / steps * 1e6,
-         "plain_fp32_est_s": t_plain_est, "plain_sims_per_s": n_nets / t_plain_est,
-         "plain_events_per_s_est": events / t_plain_est, "speedup_vs_plain": t_plain_est / t_b,
-         "plain_sample_networks": int(plain_sample), "plain_sample_steps": plain_steps,
-         "bf16_weight_rel_err_max": float(max(rel)), "bf16_weight_rel_err_mean": float(np.mean(rel)),
-         "current_rel_err_vs_f64": cur_err, "f32_weights_s": t_b32, "f32_weights_events_per_s": events32 / t_b32,
-         "tile": b.tile, "autotune_us_per_step": {str(k): v for k, v in tune.items()}}
-    print(f"# batched C4 {t_b:.3f} s ({t_b/steps*1e6:.1f} us/step) {events/t_b:.3e} ev/s, plain est "
-          f"{t_plain_est:.1f} s -> x{t_plain_est/t_b:.1f}; current err vs f64: bf16 max "
-          f"{cur_err['bf16_weights']['max']:.2e} mean {cur_err['bf16_weights']['mean']:.2e} (vs stored bf16 "
-          f"{cur_err['bf16_weights_vs_stored']['max']:.2e}), f32 max {cur_err['f32_weights']['max']:.2e}",
-          file=sys.stderr, flush=True)
+            w = np.float64(np.float32(sp[key]))
+            rel.append(abs(bf16_round(sp[key]) - w) / w)
+    r = {"workload": "batched_coba_256nets_1k-64k_jit80_2000steps", "networks": n_nets,
+         "total_neurons": total_neurons, "steps": steps,
+         "bf16_weights": {"s": t_b, "sims_per_s": n_nets / t_b, "synaptic_events": events, "spikes": spikes,
+                          "events_per_s": events / t_b, "us_per_step": t_b / steps * 1e6,
+                          "current_rel_err_vs_f64": cur_err["bf16_weights"]},
+         "f32_weights": {"s": t_b32, "sims_per_s": n_nets / t_b32, "synaptic_events": events32, "spikes": spikes32,
+                         "events_per_s": events32 / t_b32, "us_per_step": t_b32 / steps * 1e6,
+                         "current_rel_err_vs_f64": cur_err["f32_weights"]},
+         "bf16_arithmetic_err_vs_f64_same_weights": cur_err["bf16_weights_vs_stored"],
+         "bf16_weight_rel_err_max": float(max(rel)), "tile": tile,
+         "autotune_us_per_step": {str(k): v for k, v in tune.items()}}
+    msg = (f"# batched C4 bf16 {t_b:.3f} s ({t_b/steps*1e6:.1f} us/step) {events/t_b:.3e} ev/s, err vs f64 max "
+           f"{cur_err['bf16_weights']['max']:.2e} | f32 {t_b32:.3f} s {events32/t_b32:.3e} ev/s, err max "
+           f"{cur_err['f32_weights']['max']:.2e}")
+    if plain:
+        # plain per-network FP32 path, whole workload (its dynamics equal the
+        # FP32 batch bit for bit, so its events are events32)
+        t_plain, done = 0.0, 0
+        for sp in specs:
+            t, n = _plain_service(torch, sp, steps)
+            t_plain += t
+            done += n
+        assert done == steps * n_nets
+        r["plain_fp32"] = {"s": t_plain, "sims_per_s": n_nets / t_plain, "events_per_s": events32 / t_plain,
+                           "how": "every network, 2000 steps each, HIP graphs of 100 steps, one network at a time"}
+        r["speedup_bf16_vs_plain_fp32"] = (events / t_b) / (events32 / t_plain)
+        r["speedup_f32_vs_plain_fp32"] = t_plain / t_b32
+        msg += (f" | plain fp32 (graphs, whole workload) {t_plain:.2f} s {events32/t_plain:.3e} ev/s -> "
+                f"x{r['speedup_bf16_vs_plain_fp32']:.1f} (bf16 batch), x{r['speedup_f32_vs_plain_fp32']:.1f} (f32 batch)")
+    print(msg, file=sys.stderr, flush=True)
     return r
 
 
-def bench_served(_lib, torch, n_requests=80, rate=50.0, seed=7):
+def bench_served(_lib, torch, n_requests=80, rate=50.0, seed=7, plain=True, err_steps=400, err_sample=16):
     """C5: Poisson arrivals at 50 req/s of COBA requests, N log-uniform in
-    [4k, 1M], durations U[100, 5000] steps, jitconn in-degree 80, BF16
-    weights, FP32 state, admitted against a 200 GB HBM budget; continuous
-    batching versus a FIFO queue running one request at a time (same
-    kernel, same arrival trace)."""
-    from brainpy_amd.dyn.serve import Request, SimServer
+    [4k, 1M], durations U[100, 5000] steps, jitconn in-degree 80, FP32
+    state, admitted against a 200 GB HBM budget (the pools grow with the
+    admitted state up to it).  Continuous batching with BF16 and with FP32
+    weights, a FIFO queue of one request at a time on the same kernel, and
+    the plain per-network FP32 path serving the whole trace one request at a
+    time (each request one HIP-graph run, see _plain_service; latencies from
+    the arrival times and the measured service times)."""
+    from brainpy_amd.dyn.serve import DONE, Request, SimServer
     rng = np.random.Generator(np.random.PCG64(seed))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
     sizes = np.exp(rng.uniform(np.log(4000), np.log(1_000_000), n_requests)).astype(np.int64)
@@ -790,59 +817,70 @@ def bench_served(_lib, torch, n_requests=80, rate=50.0, seed=7):
     params = [(int(sizes[k]), 50_000 + k, float(rng.uniform(0.3, 0.9)), float(rng.uniform(4, 9)),
                float(rng.uniform(15, 25)), int(durs[k]), float(arrivals[k])) for k in range(n_requests)]
 
-    def run(max_active):
-        srv = SimServer(capacity=1 << 25, hbm_budget=200 * (1 << 30), bf16=True, max_active=max_active)
+    def run(max_active, bf16=True):
+        srv = SimServer(hbm_budget=200 * (1 << 30), bf16=bf16, max_active=max_active)
         reqs = [Request(k, n, s, we, wi, I, st, arrival=a) for k, (n, s, we, wi, I, st, a) in enumerate(params)]
         done = srv.serve(reqs)
+        assert all(r.status == DONE for r in done) and len(done) == n_requests
         lat = [r.completed - r.arrival for r in done]
         ev = sum(r.events for r in done)
-        return {"completed": len(done), "wall_s": srv.wall, "p50_latency_s": _percentile(lat, 50),
-                "p99_latency_s": _percentile(lat, 99), "events": ev, "events_per_s": ev / srv.wall,
-                "sims_per_s": len(done) / srv.wall, "steps_launched": srv.step_idx}
+        out = {"completed": len(done), "wall_s": srv.wall, "p50_latency_s": _percentile(lat, 50),
+               "p99_latency_s": _percentile(lat, 99), "events": ev, "events_per_s": ev / srv.wall,
+               "sims_per_s": len(done) / srv.wall, "steps_launched": srv.step_idx,
+               "pool_neurons": srv.capacity}
+        del srv
+        return out, {r.rid: r.events for r in done}
 
-    served = run(None)
-    fifo = run(1)
-    # the plain per-network FP32 op-by-op path (PlainCOBA, ~14 launches per
-    # step): per-step time measured on 4 requests spread over the size range;
-    # each request's service time = its steps x the per-step time of the
-    # nearest sampled size, served one at a time in arrival order (a FIFO
-    # queue on the same trace) for its throughput and latencies
-    from brainpy_amd.dyn.batched import PlainCOBA
+    served, _ = run(None)
+    served32, ev32 = run(None, bf16=False)
+    fifo, _ = run(1)
+    # synaptic current against FP64 on a size-stratified sample of the
+    # requests' networks (same kernel, same per-network step)
     order = np.argsort(sizes)
-    pick = order[np.linspace(0, n_requests - 1, 4).astype(int)]
-    sampled = []
+    pick = order[np.linspace(0, n_requests - 1, err_sample).astype(int)]
+    sample = []
     for k in pick:
         n_k, seed_k, we, wi, I, _, _ = params[k]
         ne = int(round(n_k * 0.8))
-        pc = PlainCOBA(dict(n_exc=ne, n_inh=n_k - ne, seed=seed_k, w_exc=we, w_inh=wi, I_ext=I, in_degree=80))
-        pc.run(3)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        pc.run(20)
-        torch.cuda.synchronize()
-        sampled.append((n_k, (time.perf_counter() - t0) / 20))
-        del pc
-    ss = np.array([x[0] for x in sampled], np.float64)
-    ts = np.array([x[1] for x in sampled], np.float64)
-    t_free, lat, busy = 0.0, [], 0.0
-    for k in np.argsort(arrivals):
-        svc = float(durs[k]) * float(ts[np.argmin(np.abs(np.log(ss) - np.log(float(sizes[k]))))])
-        start = max(float(arrivals[k]), t_free)
-        t_free = start + svc
-        busy += svc
-        lat.append(t_free - float(arrivals[k]))
-    plain = {"busy_s": busy, "events_per_s_est": served["events"] / busy, "p50_latency_s_est": _percentile(lat, 50),
-             "p99_latency_s_est": _percentile(lat, 99),
-             "sampled_per_step_s": {str(int(a)): float(b) for a, b in sampled}}
-    r = {"workload": f"served_coba_poisson{rate:g}_{n_requests}req_4k-1M_100-5000steps_jit80_bf16",
-         "requests": n_requests, "continuous_batching": served, "fifo_queue": fifo, "plain_fp32_fifo_est": plain,
-         "throughput_ratio": served["events_per_s"] / fifo["events_per_s"] if fifo["events_per_s"] else None,
-         "throughput_vs_plain": served["events_per_s"] / plain["events_per_s_est"]}
-    print(f"# served C5 batching {served['events_per_s']:.3e} ev/s p50 {served['p50_latency_s']:.3f} s "
-          f"p99 {served['p99_latency_s']:.3f} s | fifo {fifo['events_per_s']:.3e} ev/s p50 "
-          f"{fifo['p50_latency_s']:.3f} s p99 {fifo['p99_latency_s']:.3f} s | plain FP32 (est.) "
-          f"{plain['events_per_s_est']:.3e} ev/s p50 {plain['p50_latency_s_est']:.1f} s p99 "
-          f"{plain['p99_latency_s_est']:.1f} s -> x{r['throughput_vs_plain']:.0f}", file=sys.stderr, flush=True)
+        sample.append(dict(n_exc=ne, n_inh=n_k - ne, seed=seed_k, w_exc=we, w_inh=wi, I_ext=I, in_degree=80))
+    served["current_rel_err_vs_f64"] = _current_error_f64(torch, sample, True, err_steps)
+    served32["current_rel_err_vs_f64"] = _current_error_f64(torch, sample, False, err_steps)
+    r = {"workload": f"served_coba_poisson{rate:g}_{n_requests}req_4k-1M_100-5000steps_jit80",
+         "requests": n_requests, "continuous_batching_bf16": served, "continuous_batching_f32": served32,
+         "fifo_queue_bf16": fifo,
+         "throughput_ratio_vs_fifo": served["events_per_s"] / fifo["events_per_s"] if fifo["events_per_s"] else None,
+         "current_error_sample": f"{err_sample} requests spread over the size range, {err_steps} steps"}
+    msg = (f"# served C5 batching bf16 {served['events_per_s']:.3e} ev/s p50 {served['p50_latency_s']:.3f} s "
+           f"p99 {served['p99_latency_s']:.3f} s err {served['current_rel_err_vs_f64']['max']:.2e} | f32 "
+           f"{served32['events_per_s']:.3e} ev/s p99 {served32['p99_latency_s']:.3f} s err "
+           f"{served32['current_rel_err_vs_f64']['max']:.2e} | fifo {fifo['events_per_s']:.3e} ev/s p50 "
+           f"{fifo['p50_latency_s']:.3f} s p99 {fifo['p99_latency_s']:.3f} s")
+    if plain:
+        # the plain FP32 path serving the trace in arrival order, one request
+        # at a time (its per-request dynamics, hence events, equal the FP32
+        # served run's)
+        t_free, lat, busy, ev = 0.0, [], 0.0, 0
+        for k in np.argsort(arrivals):
+            n_k, seed_k, we, wi, I, st, a = params[k]
+            ne = int(round(n_k * 0.8))
+            svc, _ = _plain_service(torch, dict(n_exc=ne, n_inh=n_k - ne, seed=seed_k, w_exc=we, w_inh=wi,
+                                                I_ext=I, in_degree=80), st)
+            start = max(a, t_free)
+            t_free = start + svc
+            busy += svc
+            ev += ev32[k]
+            lat.append(t_free - a)
+        r["plain_fp32_fifo"] = {"busy_s": busy, "events_per_s": ev / max(t_free - float(arrivals.min()), busy),
+                                "events_per_busy_s": ev / busy, "p50_latency_s": _percentile(lat, 50),
+                                "p99_latency_s": _percentile(lat, 99),
+                                "how": "every request of the trace run on the plain path (HIP graphs of 100 "
+                                       "steps), one at a time in arrival order; latency = measured service "
+                                       "times queued on the arrival times"}
+        r["throughput_vs_plain"] = served["events_per_s"] / r["plain_fp32_fifo"]["events_per_busy_s"]
+        msg += (f" | plain fp32 fifo {r['plain_fp32_fifo']['events_per_busy_s']:.3e} ev/busy-s p50 "
+                f"{r['plain_fp32_fifo']['p50_latency_s']:.2f} s p99 {r['plain_fp32_fifo']['p99_latency_s']:.2f} s"
+                f" -> x{r['throughput_vs_plain']:.1f}")
+    print(msg, file=sys.stderr, flush=True)
     return r
 
 
@@ -964,7 +1002,7 @@ def main():
     L = _lib.lib()
     if args.only:
         fn = {"jitconn": lambda: bench_jitconn(L, _lib, torch, max(10, args.steps // 10), 3),
-              "batched": lambda: bench_batched(_lib, torch, plain_sample=2, plain_steps=20),
+              "batched": lambda: bench_batched(_lib, torch),
               "coba": lambda: bench_coba(_lib, torch), "smallbatch": lambda: bench_small_batch(torch),
               "served": lambda: bench_served(_lib, torch),
               "coo": lambda: bench_coo(_lib, torch), "forms": lambda: bench_forms(_lib, torch),

@@ -76,6 +76,8 @@ SIGNATURES = {
     "bpx_event_csrmv_tiled": ([_vp, ctypes.POINTER(TiledCSRPlan), _vp, _i32, _i32, _i32, _vp], _i32),
     "bpx_csrmv_tiled": ([_vp, ctypes.POINTER(TiledCSRPlan), _vp, _i32, _i32, _vp, _vp, _sz], _i32),
     "bpx_expon_coef": ([_f32, _f32], _f32),
+    "bpx_lif_coba_step_dev": ([_vp, _i64, _i32, _i32, _vp, _vp, _vp, ctypes.POINTER(_vp), _vp, _vp, _vp, _f32, _vp,
+                               _f32, _f32, _f32, _f32, _f32, _f32, _vp, _i64, _f32], _i32),
     "bpx_timer_enable": ([_i32], _i32),
     "bpx_timer_read": ([_i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)], _i32),
     "bpx_timer_reset": ([], _i32),
@@ -107,15 +109,39 @@ def check(rc, op):
         raise BPXError(f"{op} failed: {lib().bpx_status_string(rc).decode()} ({rc})")
 
 
+_gpu_ok = False
+
+
 def device():
-    if not torch.cuda.is_available():
-        raise BPXError("brainpy_amd needs a ROCm GPU (torch.cuda.is_available() is False); "
-                       "there is no CPU fallback")
+    global _gpu_ok
+    if not _gpu_ok:
+        if not torch.cuda.is_available():
+            raise BPXError("brainpy_amd needs a ROCm GPU (torch.cuda.is_available() is False); "
+                           "there is no CPU fallback")
+        _gpu_ok = True
     return torch.device("cuda", torch.cuda.current_device())
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def stream_ptr(dev=None):
+    """The current HIP stream of `dev` (default: the current device) as a
+    pointer, the stream every launch of this package goes to."""
+    if _raw_stream is not None:
+        idx = torch.cuda.current_device() if dev is None else torch.device(dev).index
+        if idx is None:
+            idx = torch.cuda.current_device()
+        return ctypes.c_void_p(_raw_stream(idx))
     return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def raw_stream(index):
+    """The current stream of device `index` as an int (the fast form of
+    stream_ptr for launches on a known device)."""
+    if _raw_stream is not None:
+        return _raw_stream(index)
+    return torch.cuda.current_stream(index).cuda_stream
 
 
 def ptr(t):

@@ -38,6 +38,8 @@ struct LifArgs {
   float I_ext;
   const float* I_vec;
   float V_rest, V_th, V_reset, tau, tau_ref, R, t, dt, cV;
+  const int64_t* d_step;   // non-null: t = f32(*d_step + step) * dt, read on the device
+  int64_t step;
 };
 
 __global__ __launch_bounds__(256) void k_lif_coba(LifArgs a) {
@@ -69,11 +71,12 @@ __global__ __launch_bounds__(256) void k_lif_coba(LifArgs a) {
     c = a.cV;
   }
   float v1 = v + c * f;
+  const float t = a.d_step ? (float)(*a.d_step + a.step) * a.dt : a.t;
   const float tl = a.t_last[i];
-  if ((a.t - tl) <= a.tau_ref) v1 = v;
+  if ((t - tl) <= a.tau_ref) v1 = v;
   const bool sp = v1 >= a.V_th;
   a.V[i] = sp ? a.V_reset : v1;
-  if (sp) a.t_last[i] = a.t;
+  if (sp) a.t_last[i] = t;
   a.spike[i] = sp ? 1 : 0;
 }
 
@@ -81,14 +84,15 @@ __global__ __launch_bounds__(256) void k_lif_coba(LifArgs a) {
 
 using namespace bpx;
 
-extern "C" int bpx_lif_coba_step(void* stream, int64_t n, int ltc, int n_g, float* V, float* t_last, uint8_t* spike,
-                                 float* const* g, const float* tau_g, const float* E_g, const float* c_g, float I_ext,
-                                 const float* I_vec, float V_rest, float V_th, float V_reset, float tau, float tau_ref,
-                                 float R, float t, float dt) {
+static int lif_step(void* stream, int64_t n, int ltc, int n_g, float* V, float* t_last, uint8_t* spike,
+                    float* const* g, const float* tau_g, const float* E_g, const float* c_g, float I_ext,
+                    const float* I_vec, float V_rest, float V_th, float V_reset, float tau, float tau_ref, float R,
+                    float t, const int64_t* d_step, int64_t step, float dt) {
   if (n < 0) return BPX_ERR_SHAPE;
   if (n_g < 0 || n_g > kMaxG) return BPX_ERR_ARG;
   if (n_g > 0 && (!g || !tau_g || !E_g || !c_g)) return BPX_ERR_ARG;
   if (n == 0) return BPX_OK;
+  if (!V || !t_last || !spike) return BPX_ERR_ARG;
   LifArgs a = {};
   a.n = n;
   a.ltc = ltc;
@@ -97,6 +101,7 @@ extern "C" int bpx_lif_coba_step(void* stream, int64_t n, int ltc, int n_g, floa
   a.t_last = t_last;
   a.spike = spike;
   for (int k = 0; k < n_g; ++k) {
+    if (!g[k]) return BPX_ERR_ARG;
     a.g[k] = g[k];
     a.tau_g[k] = tau_g[k];
     a.E_g[k] = E_g[k];
@@ -112,11 +117,31 @@ extern "C" int bpx_lif_coba_step(void* stream, int64_t n, int ltc, int n_g, floa
   a.R = R;
   a.t = t;
   a.dt = dt;
+  a.d_step = d_step;
+  a.step = step;
   a.cV = dt * exprel_det(dt * (-(1.0f / tau)));
   hipStream_t s = (hipStream_t)stream;
   BPX_LAUNCH(KID_LIF, k_lif_coba, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
   BPX_CHECK_LAUNCH();
   return BPX_OK;
+}
+
+extern "C" int bpx_lif_coba_step(void* stream, int64_t n, int ltc, int n_g, float* V, float* t_last, uint8_t* spike,
+                                 float* const* g, const float* tau_g, const float* E_g, const float* c_g, float I_ext,
+                                 const float* I_vec, float V_rest, float V_th, float V_reset, float tau, float tau_ref,
+                                 float R, float t, float dt) {
+  return lif_step(stream, n, ltc, n_g, V, t_last, spike, g, tau_g, E_g, c_g, I_ext, I_vec, V_rest, V_th, V_reset,
+                  tau, tau_ref, R, t, nullptr, 0, dt);
+}
+
+extern "C" int bpx_lif_coba_step_dev(void* stream, int64_t n, int ltc, int n_g, float* V, float* t_last,
+                                     uint8_t* spike, float* const* g, const float* tau_g, const float* E_g,
+                                     const float* c_g, float I_ext, const float* I_vec, float V_rest, float V_th,
+                                     float V_reset, float tau, float tau_ref, float R, const int64_t* d_step,
+                                     int64_t step, float dt) {
+  if (!d_step) return BPX_ERR_ARG;
+  return lif_step(stream, n, ltc, n_g, V, t_last, spike, g, tau_g, E_g, c_g, I_ext, I_vec, V_rest, V_th, V_reset,
+                  tau, tau_ref, R, 0.f, d_step, step, dt);
 }
 
 extern "C" float bpx_expon_coef(float tau, float dt) {

@@ -469,7 +469,10 @@ class PlainCOBA:
         self.ltc = int(ltc)
         self.i = 0
 
-    def step(self):
+    def _launch_step(self, local=None):
+        """The launches of one step: `local` None takes the time from the
+        host (t = i dt), otherwise from the device step counter plus
+        `local` (for captured graphs)."""
         L = _lib.lib()
         s = _lib.stream_ptr()
         for q, pre, post, kind, w in self.projs:
@@ -479,16 +482,43 @@ class PlainCOBA:
                                   _lib.ptr(out), None, 0)
             _lib.check(rc, "bpx_jitconn_mv")
             self.g[post][kind].add_(out)
-        t = np.float32(self.i) * self.dt
         for p in ("E", "I"):
             gp = (ctypes.c_void_p * 2)(self.g[p][0].data_ptr(), self.g[p][1].data_ptr())
-            rc = L.bpx_lif_coba_step(s, self.size[p], self.ltc, 2, _lib.ptr(self.V[p]), _lib.ptr(self.t_last[p]),
-                                     _lib.ptr(self.spike[p]), ctypes.cast(gp, ctypes.POINTER(ctypes.c_void_p)),
-                                     ctypes.cast(self.tau_g, ctypes.c_void_p), ctypes.cast(self.E_g, ctypes.c_void_p),
-                                     ctypes.cast(self.c_g, ctypes.c_void_p), self.I_ext, None, *self.params, t,
-                                     self.dt)
+            args = (s, self.size[p], self.ltc, 2, _lib.ptr(self.V[p]), _lib.ptr(self.t_last[p]),
+                    _lib.ptr(self.spike[p]), ctypes.cast(gp, ctypes.POINTER(ctypes.c_void_p)),
+                    ctypes.cast(self.tau_g, ctypes.c_void_p), ctypes.cast(self.E_g, ctypes.c_void_p),
+                    ctypes.cast(self.c_g, ctypes.c_void_p), self.I_ext, None, *self.params)
+            if local is None:
+                rc = L.bpx_lif_coba_step(*args, np.float32(self.i) * self.dt, self.dt)
+            else:
+                rc = L.bpx_lif_coba_step_dev(*args, _lib.ptr(self.d_step), local, self.dt)
             _lib.check(rc, "bpx_lif_coba_step")
+
+    def step(self):
+        self._launch_step()
         self.i += 1
+
+    def capture(self, steps):
+        """Record `steps` steps (about 14 launches each) in one HIP graph --
+        the counterpart of the reference's compiled scan over steps, with
+        the step index on the device; replay() advances `steps` steps."""
+        _register()
+        self.d_step = torch.full((1,), self.i, dtype=torch.int64, device=self.dev)
+        g = torch.cuda.CUDAGraph()
+        cs = torch.cuda.Stream()
+        cs.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(cs):
+            with torch.cuda.graph(g, stream=cs):
+                for k in range(steps):
+                    self._launch_step(local=k)
+                _lib.check(_lib.lib().bpx_net_advance(_lib.stream_ptr(), _lib.ptr(self.d_step), steps),
+                           "bpx_net_advance")
+        torch.cuda.current_stream().wait_stream(cs)
+        self.graph, self.graph_steps = g, steps
+
+    def replay(self):
+        self.graph.replay()
+        self.i += self.graph_steps
 
     def run(self, steps, raster=False):
         out = torch.empty((steps, self.ne + self.ni), dtype=torch.uint8, device=self.dev) if raster else None

@@ -76,13 +76,13 @@ def _events_tensor(events):
     return ev.to(torch.float32)
 
 
-def _launch(d, idx, ptr, ev, shape, transpose, rule, host_w=None):
+def _launch(d, idx, ptr, ev, shape, transpose, rule, host_w=None, use_plan=True):
     # raw pointers go to the kernel: views are made dense first
     d, idx, ptr, ev = d.contiguous(), idx.contiguous(), ptr.contiguous(), ev.contiguous()
     m, n = int(shape[0]), int(shape[1])
     if ptr.shape[0] != m + 1:
         raise ValueError(f'indptr should have shape ({m + 1},), but we got {tuple(ptr.shape)}.')
-    if transpose:
+    if transpose and use_plan:
         plan = _PLANS.lookup(d, idx, ptr, m, n, ev, host_w)
         if plan is not None:
             return plan._exec(ev, rule, None)
@@ -263,6 +263,48 @@ def _run(data, indices, indptr, events, shape, transpose, rule):
     return _launch(d, idx, ptr, ev, shape, transpose, rule, host_w)
 
 
+_EV_CODE = {torch.bool: _lib.BPX_BOOL, torch.float32: _lib.BPX_F32}
+_FAST_W = (torch.float32, torch.bfloat16)
+_FAST_I = (torch.int32, torch.int64)
+_wrapped = torch._C._functorch.is_functorch_wrapped_tensor
+
+
+def _fast_transpose(data, indices, indptr, events, shape):
+    """``csrmv(..., transpose=True)`` for device tensors that need neither
+    conversion nor differentiation -- the per-step call of a projection
+    layer.  Returns None whenever anything is unusual (other types, dtypes,
+    devices, views, gradients, inconsistent sizes); the general path then
+    checks the arguments and raises the reference's errors."""
+    T = torch.Tensor
+    if type(data) is not T or type(indices) is not T or type(indptr) is not T or type(events) is not T:
+        return None
+    if not (data.is_cuda and events.is_cuda) or len(shape) != 2:
+        return None
+    if data.dtype not in _FAST_W or events.dtype not in _EV_CODE or indices.dtype not in _FAST_I or \
+            indptr.dtype is not indices.dtype:
+        return None
+    if data.dim() != 1 or indices.dim() != 1 or indptr.dim() != 1 or events.dim() != 1:
+        return None
+    m, n = int(shape[0]), int(shape[1])
+    nd = data.shape[0]
+    if (nd != 1 and nd != indices.shape[0]) or events.shape[0] != m or indptr.shape[0] != m + 1:
+        return None
+    if not (data.is_contiguous() and indices.is_contiguous() and indptr.is_contiguous() and events.is_contiguous()):
+        return None
+    if torch.is_grad_enabled() and (data.requires_grad or events.requires_grad):
+        return None
+    if _wrapped(data) or _wrapped(events):
+        return None
+    dev = events.get_device()
+    if data.get_device() != dev or indices.get_device() != dev or indptr.get_device() != dev or \
+            dev != torch.cuda.current_device():
+        return None
+    plan = _PLANS.lookup(data, indices, indptr, m, n, events, None)
+    if plan is None:
+        return _launch(data, indices, indptr, events, shape, True, _lib.RULE_POSITIVE, use_plan=False)
+    return plan._exec(events, _lib.RULE_POSITIVE, None)
+
+
 def csrmv(data, indices, indptr, events, *, shape: Tuple[int, int], transpose: bool = False) -> torch.Tensor:
     """Product of a sparse CSR matrix and a dense event vector.
 
@@ -273,6 +315,10 @@ def csrmv(data, indices, indptr, events, *, shape: Tuple[int, int], transpose: b
     f32).  Returns ``(shape[1] if transpose else shape[0],)``.
     Reference: brainpy/_src/math/event/_csr_matvec.py:38-110.
     """
+    if transpose is True:
+        out = _fast_transpose(data, indices, indptr, events, shape)
+        if out is not None:
+            return out
     data, indices, indptr, events = (_as_array(x) for x in (data, indices, indptr, events))
     if ndim(data) == 0:
         data = data.reshape(1)
@@ -431,6 +477,7 @@ class TiledCSR:
         self.shape = (m, n)
         self.weight_dtype = wcode
         self._plan_ref = ctypes.byref(self.plan)
+        self._fn = L.bpx_event_csrmv_tiled
 
     def csrmv(self, events, *, out: torch.Tensor = None, rule: int = _lib.RULE_POSITIVE) -> torch.Tensor:
         """``events @ A`` (a new vector), or ``out += events @ A`` when
@@ -446,12 +493,13 @@ class TiledCSR:
         return self._exec(ev.contiguous(), rule, out)
 
     def _exec(self, ev, rule, out):
-        """Launch on a contiguous bool/f32 event vector of length m."""
+        """Launch on a contiguous bool/f32 event vector of length m, on the
+        current stream of its device."""
         accumulate = out is not None
         if out is None:
             out = torch.empty(self.shape[1], dtype=torch.float32, device=ev.device)
-        rc = _lib.lib().bpx_event_csrmv_tiled(_lib.stream_ptr(), self._plan_ref, _lib.ptr(ev),
-                                              _lib.bpx_dtype(ev), rule, int(accumulate), _lib.ptr(out))
+        rc = self._fn(_lib.raw_stream(ev.get_device()), self._plan_ref, ev.data_ptr(),
+                      _EV_CODE[ev.dtype], rule, accumulate, out.data_ptr())
         if rc != 0:
             _lib.check(rc, "bpx_event_csrmv_tiled")
         return out

@@ -235,6 +235,20 @@ int bpx_lif_coba_step(void* stream, int64_t n, int ltc, int n_g,
                       float t, float dt);
 
 /*
+ * The same step with the time read on the device: t = f32(*d_step + step) *
+ * dt (the reference's t = i * dt of runners.py:619-643 with the step index
+ * i held in device memory), so a captured HIP graph of K steps (step =
+ * 0..K-1, then bpx_net_advance(d_step, K)) continues in time on every
+ * replay, as the reference's compiled scan carries i on the device.
+ */
+int bpx_lif_coba_step_dev(void* stream, int64_t n, int ltc, int n_g,
+                          float* V, float* t_last, uint8_t* spike, float* const* g,
+                          const float* tau_g, const float* E_g, const float* c_g,
+                          float I_ext, const float* I_vec,
+                          float V_rest, float V_th, float V_reset, float tau, float tau_ref, float R,
+                          const int64_t* d_step, int64_t step, float dt);
+
+/*
  * Host helper: the Expon exp_euler factor c = dt * exprel(dt * (-(1/tau)))
  * in f32 with the library's deterministic exp, i.e. the `dt*phi` of
  * integrators/ode/exponential.py:357-362 for dg/dt = -g/tau

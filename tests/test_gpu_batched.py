@@ -183,28 +183,129 @@ def test_plain_path_equals_batched():
         np.testing.assert_array_equal(rb[:, b.net_slice(k)], rp)
 
 
-def test_served_alone_equals_in_mix():
-    from brainpy_amd.dyn.serve import Request, SimServer
-    rng = np.random.default_rng(5)
+def test_plain_path_graph_equals_eager():
+    # the plain op-by-op path captured in HIP graphs (device step counter,
+    # bpx_lif_coba_step_dev) continues in time on every replay: after 3 x 40
+    # replayed steps the state equals 120 eager steps bit for bit
+    from brainpy_amd.dyn.batched import PlainCOBA
+    for sp in _jit_specs([1500, 4000], seed0=21):
+        a = PlainCOBA(sp)
+        a.run(120)
+        b = PlainCOBA(sp)
+        b.capture(40)
+        for _ in range(3):
+            b.replay()
+        torch.cuda.synchronize()
+        assert b.i == a.i == 120
+        for p in ("E", "I"):
+            assert torch.equal(a.V[p], b.V[p]) and torch.equal(a.t_last[p], b.t_last[p])
+            assert torch.equal(a.g[p][0], b.g[p][0]) and torch.equal(a.g[p][1], b.g[p][1])
+        assert int(a.t_last["E"].max()) > 0          # spikes happened after the first window
 
-    def make(rid, arrival):
-        n = int(np.exp(rng.uniform(np.log(4000), np.log(40000))))
-        return Request(rid, n, seed=1000 + rid, w_exc=rng.uniform(0.3, 0.9), w_inh=rng.uniform(4, 9),
-                       I_ext=rng.uniform(15, 25), steps=int(rng.integers(50, 300)), arrival=arrival,
-                       keep_state=True)
 
-    reqs = [make(k, 0.002 * k) for k in range(12)]
-    specs = [(r.n, r.seed, r.w_exc, r.w_inh, r.I_ext, r.steps) for r in reqs]
-    srv = SimServer(capacity=1 << 17)   # smaller than the sum of the requests: some queue
+def _oracle_served(req, srv, steps):
+    """The oracle's serial run of one served request (its own seeds, BF16
+    weights, V0 from PCG64(seed + 7)): spike count, events delivered and
+    final potentials."""
+    from brainpy_amd.dyn.batched import bf16_round
+    prm = srv.params
+    ne, ni = req.n_exc, req.n_inh
+    rng = np.random.Generator(np.random.PCG64(req.seed + 7))
+    V0 = rng.uniform(prm.V_rest, prm.V_th, req.n).astype(np.float32)
+    Vs = {"E": V0[:ne].copy(), "I": V0[ne:].copy()}
+    size = {"E": ne, "I": ni}
+    tl = {p: np.full(size[p], -1e7, np.float32) for p in "EI"}
+    s = {p: np.zeros(size[p], np.uint8) for p in "EI"}
+    g = {p: [np.zeros(size[p], np.float32), np.zeros(size[p], np.float32)] for p in "EI"}
+    we, wi = np.float32(bf16_round(req.w_exc)), np.float32(bf16_round(req.w_inh))
+    clen = conn_len(min(80.0 / req.n, 1.0))
+    projs = [(0, "E", "E", 0, we), (1, "E", "I", 0, we), (2, "I", "E", 1, wi), (3, "I", "I", 1, wi)]
+    c_g = [np.float32(prm.c_e), np.float32(prm.c_i)]
+    pending = None        # conductance delivered by the previous step's spikes
+    spikes = events = 0
+    for i in range(steps):
+        t = np.float32(i) * np.float32(prm.dt)
+        if pending is not None:
+            for post, kind, out in pending:
+                g[post][kind] += out
+        for p in "EI":
+            orc.lif_coba_step(Vs[p], tl[p], s[p], g[p], [prm.tau_e, prm.tau_i], [prm.E_e, prm.E_i], c_g, req.I_ext,
+                              PARAMS, t, prm.dt, ltc=bool(prm.ltc))
+        spikes += int(s["E"].sum()) + int(s["I"].sum())
+        # this step's spikes are delivered (and counted as events) now and
+        # reach the conductances at the next step
+        pending = []
+        for q, pre, post, kind, w in projs:
+            seed = (req.seed * 4 + q) & 0xFFFFFFFF
+            ev = s[pre].view(bool)
+            pending.append((post, kind, orc.jitconn_mv(ev, "homo", 1, False, orc.LAYOUT_LANE32, size[post],
+                                                       size[pre], w, 0.0, clen, seed)))
+            events += int(round(float(orc.jitconn_mv(ev, "homo", 1, False, orc.LAYOUT_LANE32, size[post], size[pre],
+                                                     1.0, 0.0, clen, seed).sum())))
+    return spikes, events, np.concatenate([Vs["E"], Vs["I"]])
+
+
+def _served_mix(seed=5, count=8, lo=2000, hi=9000, smin=40, smax=160):
+    from brainpy_amd.dyn.serve import Request
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        reqs.append(Request(k, n, seed=1000 + k, w_exc=rng.uniform(0.3, 0.9), w_inh=rng.uniform(4, 9),
+                            I_ext=rng.uniform(15, 25), steps=int(rng.integers(smin, smax)), arrival=0.001 * k,
+                            keep_state=True))
+    return reqs
+
+
+def test_served_mix_vs_oracle():
+    # every request of a served mix (pools smaller than the sum, so some
+    # queue and join later with their own time origin) equals the oracle's
+    # serial run of that request alone: spike count, synaptic events and
+    # final potentials bit for bit
+    from brainpy_amd.dyn.serve import DONE, SimServer
+    reqs = _served_mix()
+    srv = SimServer(capacity=1 << 14, initial_capacity=1 << 12)   # grows 4k -> 16k, some requests wait
     done = {r.rid: r for r in srv.serve(reqs)}
-    assert len(done) == 12 and sum(r.spikes for r in done.values()) > 0
-    # admission respected the budget, so some requests waited
-    for k in (0, 5, 11):
-        n, seed, we, wi, I, steps = specs[k]
-        alone = SimServer(capacity=1 << 16)
-        (r,) = alone.serve([Request(k, n, seed, we, wi, I, steps, keep_state=True)])
-        assert r.spikes == done[k].spikes and r.events == done[k].events
-        np.testing.assert_array_equal(r.V_final, done[k].V_final)
+    assert len(done) == len(reqs) and all(r.status == DONE for r in done.values())
+    assert max(r.admitted for r in done.values()) > min(r.completed for r in done.values())   # queueing happened
+    for r in reqs:
+        spikes, events, V = _oracle_served(r, srv, r.steps)
+        assert (r.spikes, r.events) == (spikes, events), r.rid
+        np.testing.assert_array_equal(r.V_final, V)
+
+
+def test_served_failed_slot_is_isolated():
+    # a NaN written into one running network's potential: the health check
+    # stops that network (status failed) and every other request's results
+    # are bit-identical to a run without the fault
+    from brainpy_amd.dyn.serve import DONE, FAILED, SimServer
+    clean = {r.rid: r for r in SimServer(capacity=1 << 16, health_every=8).serve(_served_mix(seed=9))}
+    srv = SimServer(capacity=1 << 16, health_every=8)
+    reqs = _served_mix(seed=9)
+    victim = reqs[2]
+
+    def poison(s):
+        if s.step_idx == 20 and victim.status == "active":
+            s.ex.inject_fault(victim, where=victim.n // 2)
+    done = {r.rid: r for r in srv.serve(reqs, on_step=poison)}
+    assert done[2].status == FAILED and done[2].steps_done <= 24
+    for k, r in done.items():
+        if k != 2:
+            assert r.status == DONE
+            assert (r.spikes, r.events) == (clean[k].spikes, clean[k].events)
+            np.testing.assert_array_equal(r.V_final, clean[k].V_final)
+
+
+def test_served_cancel_running_keeps_others():
+    from brainpy_amd.dyn.serve import CANCELLED, DONE, SimServer
+    clean = {r.rid: r for r in SimServer(capacity=1 << 16).serve(_served_mix(seed=11))}
+    srv = SimServer(capacity=1 << 16)
+    done = {r.rid: r for r in srv.serve(_served_mix(seed=11), cancel_at={1: 15})}
+    assert done[1].status == CANCELLED and 0 < done[1].steps_done <= 16
+    for k, r in done.items():
+        if k != 1:
+            assert r.status == DONE and r.spikes == clean[k].spikes
+            np.testing.assert_array_equal(r.V_final, clean[k].V_final)
 
 
 def _twin_error(specs, bf16, steps, stored_weights=False):
