@@ -75,6 +75,12 @@ SIGNATURES = {
     "bpx_tiled_csr_build": ([_vp, _vp, _i32, _i64, _vp, _vp, _i32, ctypes.POINTER(TiledCSRPlan), _vp, _sz], _i32),
     "bpx_event_csrmv_tiled": ([_vp, ctypes.POINTER(TiledCSRPlan), _vp, _i32, _i32, _i32, _vp], _i32),
     "bpx_csrmv_tiled": ([_vp, ctypes.POINTER(TiledCSRPlan), _vp, _i32, _i32, _vp, _vp, _sz], _i32),
+    "bpx_event_csrmv_tiled_batch": ([_vp, ctypes.POINTER(TiledCSRPlan), _vp, _i32, _i32, _i64, _i32, _vp], _i32),
+    "bpx_batch_workspace": ([_i64], _sz),
+    "bpx_event_csrmv_batch": ([_vp, _vp, _i32, _i64, _vp, _vp, _i32, _i64, _vp, _i32, _i32, _i64, _i64, _i64, _i32,
+                               _vp, _vp, _sz], _i32),
+    "bpx_jitconn_mv_batch": ([_vp, _vp, _i32, _i32, _f32, _f32, ctypes.c_int32, _u32, _i64, _i64, _i64, _i32, _i32,
+                              _vp, _vp, _sz], _i32),
     "bpx_expon_coef": ([_f32, _f32], _f32),
     "bpx_lif_coba_step_dev": ([_vp, _i64, _i32, _i32, _vp, _vp, _vp, ctypes.POINTER(_vp), _vp, _vp, _vp, _f32, _vp,
                                _f32, _f32, _f32, _f32, _f32, _f32, _vp, _i64, _f32], _i32),

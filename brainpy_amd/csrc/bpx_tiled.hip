@@ -256,6 +256,9 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
       fx_inv = ldexp(fx_inv, ex);
     }
   }
+  // a batch of event vectors: vector blockIdx.y of [B][m] into [B][n]
+  ev += (int64_t)blockIdx.y * m;
+  out += (int64_t)blockIdx.y * n;
   const int b = blockIdx.x;
   const int x = b & 7;                 // XCD of this block (round-robin dispatch)
   const int jb = b >> 3;
@@ -624,14 +627,22 @@ int bpx_tiled_csr_build(void* stream, const void* data, int data_dtype, int64_t 
 
 int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* events, int ev_dtype, int ev_rule,
                           int accumulate, float* out) {
+  return bpx_event_csrmv_tiled_batch(stream, plan, events, ev_dtype, ev_rule, 1, accumulate, out);
+}
+
+int bpx_event_csrmv_tiled_batch(void* stream, const bpx_tiled_csr* plan, const void* events, int ev_dtype,
+                                int ev_rule, int64_t batch, int accumulate, float* out) {
   if (!plan || !plan->mem) return BPX_ERR_ARG;
   if (ev_rule != BPX_RULE_POSITIVE && ev_rule != BPX_RULE_NONZERO) return BPX_ERR_ARG;
+  if (batch < 0 || batch > 65535) return BPX_ERR_SHAPE;
+  if (batch == 0) return BPX_OK;
   hipStream_t s = (hipStream_t)stream;
   const int64_t m = plan->m, n = plan->n;
+  if (!out || (m > 0 && !events)) return BPX_ERR_ARG;
   if (!accumulate) {
-    int64_t zb = (n + 255) / 256;
+    int64_t zb = (batch * n + 255) / 256;
     if (zb > 4096) zb = 4096;
-    hipLaunchKernelGGL(k_fill_zero_t<float>, dim3((unsigned)zb), dim3(256), 0, s, out, n);
+    hipLaunchKernelGGL(k_fill_zero_t<float>, dim3((unsigned)zb), dim3(256), 0, s, out, batch * n);
   }
   if (m == 0) {
     BPX_CHECK_LAUNCH();
@@ -649,7 +660,7 @@ int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* e
   const size_t shm = (size_t)T * sizeof(unsigned long long);
   const double fx_scale = ldexp(1.0, plan->fx_shift), fx_inv = ldexp(1.0, -plan->fx_shift);
 #define TLR(WK, E, R)                                                                                     \
-  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E, R, false>), dim3(blocks), dim3(kTiledThreads), shm, s, rowtile, \
+  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E, R, false>), dim3(blocks, (unsigned)batch), dim3(kTiledThreads), shm, s, rowtile, \
              lidx, w, hw, (const E*)events, ev_rule, m, n, P, T, cpx, fx_scale, fx_inv,                         \
              (const uint32_t*)nullptr, out)
 #define TL(WK, E)                          \

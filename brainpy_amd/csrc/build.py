@@ -22,7 +22,8 @@ LIB = os.path.join(PKG, "libbpx.so")
 PROBE_LIB = os.path.join(PKG, "libbpx_probe.so")   # measurement probes, loaded by the benchmark only
 PROBE_SOURCES = ["bpx_probe.hip"]
 SOURCES = ["bpx_runtime.hip", "bpx_csr.hip", "bpx_info.hip", "bpx_jitconn.hip", "bpx_neuron.hip", "bpx_tiled.hip",
-           "bpx_net.hip", "bpx_plast.hip", "bpx_coo.hip", "bpx_conn.hip", "bpx_bsr.hip", "bpx_xla.hip"]
+           "bpx_net.hip", "bpx_plast.hip", "bpx_coo.hip", "bpx_conn.hip", "bpx_bsr.hip", "bpx_xla.hip",
+           "bpx_batch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -19,7 +19,7 @@ from .. import _lib
 from ._common import acc_dtype, batch_rows, data_tensor, index_tensor, ndim, needs_ad
 
 __all__ = ["csrmv", "csrmv_taichi", "info", "csrmv_fixed_order", "csr_to_csc", "csrmv_add", "TiledCSR",
-           "plan_cache_info", "clear_plan_cache", "set_plan_cache"]
+           "plan_cache_info", "clear_plan_cache", "set_plan_cache", "csrmv_batch"]
 
 
 def _check(data, indices, indptr, events, shape, transpose, allowed_index, index_msg):
@@ -245,7 +245,45 @@ class _EventCSRMV(torch.autograd.Function):
 
     @staticmethod
     def vmap(info, in_dims, d, ev, idx, ptr, shape, transpose, rule):
+        if in_dims[0] is None and in_dims[2] is None and in_dims[3] is None and in_dims[1] is not None:
+            out = _launch_batch(d, ev.movedim(in_dims[1], 0), idx, ptr, shape, transpose, rule)
+            if out is not None:
+                return out, 0
         return batch_rows(info, in_dims, (d, ev, idx, ptr, shape, transpose, rule), _EventCSRMV.apply)
+
+
+def _launch_batch(d, ev, idx, ptr, shape, transpose, rule):
+    """csrmv over a batch of event vectors ev [B, len] in one launch: on the
+    cached tiled plan for transpose=True when the matrix has one (a batch is
+    a repeated use), else bpx_event_csrmv_batch (each row read once for all
+    vectors).  None when the dtypes are outside the batched kernels (f64)."""
+    if d.dtype not in (torch.float32, torch.bfloat16) or ev.dtype not in (torch.bool, torch.float32):
+        return None
+    d, idx, ptr, ev = d.contiguous(), idx.contiguous(), ptr.contiguous(), ev.contiguous()
+    m, n = int(shape[0]), int(shape[1])
+    B = ev.shape[0]
+    if ptr.shape[0] != m + 1:
+        raise ValueError(f'indptr should have shape ({m + 1},), but we got {tuple(ptr.shape)}.')
+    if transpose and B >= 1:
+        plan = _PLANS.lookup(d, idx, ptr, m, n, ev, None)
+        if plan is None:
+            plan = _PLANS.lookup(d, idx, ptr, m, n, ev, None)     # a batch counts as repeated use
+        if plan is not None:
+            out = torch.empty((B, n), dtype=torch.float32, device=ev.device)
+            rc = _lib.lib().bpx_event_csrmv_tiled_batch(_lib.stream_ptr(), plan._plan_ref, _lib.ptr(ev),
+                                                        _EV_CODE[ev.dtype], rule, B, 0, _lib.ptr(out))
+            _lib.check(rc, "bpx_event_csrmv_tiled_batch")
+            return out
+    L = _lib.lib()
+    ev_len = m if transpose else n
+    out = torch.empty((B, n if transpose else m), dtype=torch.float32, device=ev.device)
+    wsb = int(L.bpx_batch_workspace(ev_len))
+    ws = torch.empty(max(wsb, 4), dtype=torch.uint8, device=ev.device)
+    rc = L.bpx_event_csrmv_batch(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), d.shape[0], _lib.ptr(idx),
+                                 _lib.ptr(ptr), _lib.bpx_dtype(idx), idx.shape[0], _lib.ptr(ev), _lib.bpx_dtype(ev),
+                                 rule, B, m, n, int(bool(transpose)), _lib.ptr(out), _lib.ptr(ws), ws.numel())
+    _lib.check(rc, "bpx_event_csrmv_batch")
+    return out
 
 
 def _run(data, indices, indptr, events, shape, transpose, rule):
@@ -325,6 +363,36 @@ def csrmv(data, indices, indptr, events, *, shape: Tuple[int, int], transpose: b
     _check(data, indices, indptr, events, shape, transpose, _PUBLIC_INDEX,
            '{} should be a 1D vector with int32 or int64 type.')
     return _run(data, indices, indptr, events, shape, transpose, _lib.RULE_POSITIVE)
+
+
+def csrmv_batch(data, indices, indptr, events, *, shape: Tuple[int, int], transpose: bool = False,
+                stateless: bool = False) -> torch.Tensor:
+    """``csrmv`` of every row of ``events`` ([B, len]) in one launch -- what
+    ``torch.func.vmap(csrmv)`` over the events runs.  ``stateless`` skips the
+    plan cache (transpose=True then runs bpx_event_csrmv_batch)."""
+    d = data_tensor(_as_array(data))
+    if d.dtype == torch.float16:
+        d = d.to(torch.float32)
+    idx = index_tensor(_as_array(indices))
+    ptr = index_tensor(_as_array(indptr))
+    if ptr.dtype != idx.dtype:
+        ptr = ptr.to(idx.dtype)
+    ev = _events_tensor(_as_array(events))
+    if ev.ndim != 2:
+        raise ValueError('events should be a 2D [batch, length] array.')
+    _check(d, idx, ptr, ev[0], shape, transpose, _PUBLIC_INDEX, '{} should be a 1D vector with int32 or int64 type.')
+    if stateless:
+        with_cache = _PLANS.enabled
+        _PLANS.enabled = False
+    try:
+        out = _launch_batch(d, ev, idx, ptr, shape, transpose, _lib.RULE_POSITIVE)
+    finally:
+        if stateless:
+            _PLANS.enabled = with_cache
+    if out is None:
+        out = torch.stack([_launch(d, idx, ptr, ev[b], shape, transpose, _lib.RULE_POSITIVE)
+                           for b in range(ev.shape[0])])
+    return out
 
 
 def csrmv_taichi(data, indices, indptr, events, *, shape: Tuple[int, int], transpose: bool = False):

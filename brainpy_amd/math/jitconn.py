@@ -135,7 +135,31 @@ class _JitMV(torch.autograd.Function):
 
     @staticmethod
     def vmap(info, in_dims, *args):
+        v, w = args[0], args[1]
+        if in_dims[0] is not None and all(d is None for d in in_dims[1:]) and args[2] and v.dtype == torch.bool:
+            dist, w0, w1, clen, seed, num_row, num_col, outdim_parallel, layout = args[3:]
+            if w is not None:
+                w0 = float(w.detach().reshape(-1)[0])
+            return _launch_batch(v.movedim(in_dims[0], 0), dist, w0, w1, clen, seed, num_row, num_col,
+                                 outdim_parallel, layout), 0
         return batch_rows(info, in_dims, args, _JitMV.apply)
+
+
+def _launch_batch(v, dist, w0, w1, clen, seed, num_row, num_col, outdim_parallel, layout):
+    """Event operator over bool events v [B, num_col] in one launch
+    (bpx_jitconn_mv_batch): the connections are generated once for the
+    whole batch."""
+    v = v.contiguous()
+    B = v.shape[0]
+    out = torch.empty((B, num_row), dtype=torch.float32, device=v.device)
+    L = _lib.lib()
+    wsb = int(L.bpx_batch_workspace(num_col))
+    ws = torch.empty(max(wsb, 4), dtype=torch.uint8, device=v.device)
+    rc = L.bpx_jitconn_mv_batch(_lib.stream_ptr(), _lib.ptr(v), _lib.BPX_BOOL, dist, np.float32(w0), np.float32(w1),
+                                clen, seed, B, num_row, num_col, int(outdim_parallel), _LAYOUTS[layout],
+                                _lib.ptr(out), _lib.ptr(ws), ws.numel())
+    _lib.check(rc, "bpx_jitconn_mv_batch")
+    return out
 
 
 def _weight_arg(x, name, differentiable):

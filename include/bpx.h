@@ -183,6 +183,12 @@ int bpx_tiled_csr_build(void* stream, const void* data, int data_dtype, int64_t 
 /* out[n] = events @ A (accumulate=0) or out += events @ A (accumulate=1) */
 int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* events, int ev_dtype,
                           int ev_rule, int accumulate, float* out);
+/* The same for a batch of event vectors, events [batch][m] -> out
+ * [batch][n], in one launch (batch <= 65535); the batching rule of the
+ * reference's primitive is a sequential scan over the batch
+ * (op_register/utils.py:16-37). */
+int bpx_event_csrmv_tiled_batch(void* stream, const bpx_tiled_csr* plan, const void* events, int ev_dtype,
+                                int ev_rule, int64_t batch, int accumulate, float* out);
 /*
  * The same plan with a dense f32 vector: out (+)= x @ A, the transpose
  * `csrmv` of sparse/_csr_mv.py:29-123 (cuSPARSE transpose SpMV) for a
@@ -194,6 +200,33 @@ int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* e
  */
 int bpx_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* x, int x_dtype, int accumulate, float* out,
                     void* workspace, size_t workspace_bytes);
+
+/*
+ * Batched event operators: `batch` event vectors in one launch, in place of
+ * the reference's batching rule for these primitives, a sequential scan over
+ * the batch (op_register/utils.py:16-37; event/_csr_matvec.py:549-556,
+ * jitconn/_event_matvec.py:294-302).  The vectors are packed into per-index
+ * bit masks (workspace of bpx_batch_workspace(len) bytes, len = the event
+ * length; 32 vectors per pass), then each row's synapses (or each generated
+ * connection) are read (generated) once for all vectors.
+ *
+ * bpx_event_csrmv_batch: events [batch][m] (transpose) or [batch][n], bool
+ *   or f32 (ev_rule), data f32 or bf16 (homogeneous when data_len == 1),
+ *   out f32 [batch][n] or [batch][m].  transpose=1: one FP32 atomic per
+ *   (vector, synapse) of rows active in any vector; transpose=0: gather with
+ *   one shuffle reduction per vector.
+ * bpx_jitconn_mv_batch: bool events [batch][num_col] -> out [batch][num_row];
+ *   row-parallel results are bit-identical to bpx_jitconn_mv on each vector
+ *   (finite weights); the scatter form adds with FP32 atomics.
+ */
+size_t bpx_batch_workspace(int64_t len);
+int bpx_event_csrmv_batch(void* stream, const void* data, int data_dtype, int64_t data_len,
+                          const void* indices, const void* indptr, int index_dtype, int64_t nnz,
+                          const void* events, int ev_dtype, int ev_rule, int64_t batch, int64_t m, int64_t n,
+                          int transpose, float* out, void* workspace, size_t workspace_bytes);
+int bpx_jitconn_mv_batch(void* stream, const void* events, int ev_dtype, int dist, float w0, float w1,
+                         int32_t clen, uint32_t seed, int64_t batch, int64_t num_row, int64_t num_col,
+                         int outdim_parallel, int layout, float* out, void* workspace, size_t workspace_bytes);
 
 /*
  * Just-in-time connectivity matvec y = M @ v, M never stored.  Replaces the
