@@ -884,6 +884,60 @@ def bench_served(_lib, torch, n_requests=80, rate=50.0, seed=7, plain=True, err_
     return r
 
 
+def bench_batch_ops(_lib, torch, batches=(16, 64), reps=5):
+    """Batched event operators (what torch.func.vmap over the events runs)
+    against a loop of single calls: jitconn 1M x 1M p=1e-4 row-parallel
+    homo (2 % spikes), and the 100k x 100k p=0.01 CSR with heterogeneous
+    f32 weights (10 % spikes) in both directions (scatter on the tiled plan
+    and stateless, gather)."""
+    import brainpy_amd.math as bm
+    from brainpy_amd.connect import fixed_prob_csr_device
+    dev = _lib.device()
+    res = []
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    n = 1_000_000
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    for B in batches:
+        E = torch.rand((B, n), generator=g, device=dev) < 0.02
+        one = lambda e: bm.jitconn.event_mv_prob_homo(e, 1.0, 1e-4, 42, shape=(n, n))  # noqa: E731
+        t_b = timed(lambda: torch.func.vmap(one)(E))
+        t_l = timed(lambda: [one(E[b]) for b in range(B)])
+        res.append({"op": "jitconn event_mv_prob_homo 1Mx1M p=1e-4 (row-parallel)", "batch": B,
+                    "batched_ms": t_b * 1e3, "loop_ms": t_l * 1e3, "speedup": t_l / t_b})
+        del E
+    m = 100_000
+    idx, ptr = fixed_prob_csr_device(m, m, 0.01, seed=2024, device=dev)
+    w = torch.empty(idx.numel(), dtype=torch.float32, device=dev).normal_(0.5, 0.1, generator=g)
+    for B in batches:
+        E = torch.rand((B, m), generator=g, device=dev) < 0.1
+        for transpose, stateless, label in ((True, False, "scatter, tiled plan"), (True, True, "scatter, stateless"),
+                                            (False, False, "gather")):
+            if stateless:
+                bm.event.set_plan_cache(enabled=False)
+            one = lambda e: bm.event.csrmv(w, idx, ptr, e, shape=(m, m), transpose=transpose)  # noqa: E731
+            t_b = timed(lambda: bm.event.csrmv_batch(w, idx, ptr, E, shape=(m, m), transpose=transpose,
+                                                     stateless=stateless))
+            t_l = timed(lambda: [one(E[b]) for b in range(B)])
+            bm.event.set_plan_cache(enabled=True)
+            res.append({"op": f"event csrmv 100kx100k p=0.01 heter f32 10% ({label})", "batch": B,
+                        "batched_ms": t_b * 1e3, "loop_ms": t_l * 1e3, "speedup": t_l / t_b})
+        del E
+    for r in res:
+        print(f"# batch {r['op']:60s} B={r['batch']:3d} batched {r['batched_ms']:8.2f} ms loop {r['loop_ms']:8.2f} ms"
+              f" -> x{r['speedup']:.2f}", file=sys.stderr, flush=True)
+    return {"workload": "vmap over event vectors, B = " + "/".join(map(str, batches)), "results": res}
+
+
 HEADLINE_PROFILE = "event_scatter_100k_p0.01_rate10_heter_f32.md"
 
 
@@ -980,7 +1034,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the jitconn and network legs")
     ap.add_argument("--only", choices=["jitconn", "batched", "coba", "smallbatch", "served", "coo", "forms", "conn", "stdp",
-                                       "bsr"],
+                                       "bsr", "batchops"],
                     default=None,
                     help="run only this leg (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -1007,7 +1061,7 @@ def main():
               "served": lambda: bench_served(_lib, torch),
               "coo": lambda: bench_coo(_lib, torch), "forms": lambda: bench_forms(_lib, torch),
               "conn": lambda: bench_conn(_lib, torch), "stdp": lambda: bench_stdp(_lib, torch),
-              "bsr": lambda: bench_bsr(_lib, torch)}[args.only]
+              "bsr": lambda: bench_bsr(_lib, torch), "batchops": lambda: bench_batch_ops(_lib, torch)}[args.only]
         print(json.dumps({"only": args.only, "result": fn()}), flush=True)
         return
     sp = _lib.stream_ptr()
@@ -1206,6 +1260,7 @@ def main():
         extra["small_batch"] = bench_small_batch(torch)
         extra["batched"] = bench_batched(_lib, torch)
         extra["served"] = bench_served(_lib, torch)
+        extra["batch_ops"] = bench_batch_ops(_lib, torch)
         extra["coo"] = bench_coo(_lib, torch)
         extra["forms"] = bench_forms(_lib, torch)
         extra["conn"] = bench_conn(_lib, torch)
