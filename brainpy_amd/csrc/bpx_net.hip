@@ -177,7 +177,8 @@ constexpr int kRunThreads = 1024;
 constexpr int kRunNpt = 8;                 // networks up to 8192 neurons
 
 template <int CONN>
-__global__ __launch_bounds__(kRunThreads) void k_net_run(NetArgs a, uint8_t* raster, int64_t n_steps, int cap) {
+__global__ __launch_bounds__(kRunThreads) void k_net_run(NetArgs a, uint8_t* raster, int64_t n_steps, int cap,
+                                                        int32_t* status) {
   extern __shared__ uint32_t s_mem[];
   __shared__ int s_nspk[2];
   __shared__ unsigned long long s_events;
@@ -187,7 +188,10 @@ __global__ __launch_bounds__(kRunThreads) void k_net_run(NetArgs a, uint8_t* ras
   const int net = blockIdx.x;
   const bpx_net_desc d = a.nets[net];
   const int N = d.n_exc + d.n_inh;
-  if (N > cap) return;                                   // (the host checks every network fits)
+  if (N > cap) {                                         // more neurons than the caller's max_neurons:
+    if (status && threadIdx.x == 0) atomicExch(status, (int32_t)BPX_ERR_SHAPE);   // left unchanged, reported
+    return;
+  }
   const int64_t total = a.total;
   const int tid = threadIdx.x;
   for (int b = 0; b < nb; ++b) {
@@ -427,7 +431,8 @@ int64_t bpx_net_run_max_neurons(int n_buf) { return n_buf < 2 ? 0 : run_max_neur
 int bpx_net_run(void* stream, const bpx_net_desc* nets, int n_nets, int max_neurons, int max_in_degree,
                 int conn_kind, int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts,
                 int n_buf, uint8_t* raster, uint8_t* spikes, unsigned long long* events, unsigned int* spike_count,
-                const bpx_net_params* prm, int64_t step, const int64_t* d_step, int64_t n_steps) {
+                const bpx_net_params* prm, int64_t step, const int64_t* d_step, int64_t n_steps,
+                int32_t* status) {
   if (!nets || !prm) return BPX_ERR_ARG;
   if (n_buf < 2 || (!d_step && step < 0) || n_steps < 0) return BPX_ERR_ARG;
   if (n_nets < 0 || total < 0) return BPX_ERR_SHAPE;
@@ -442,10 +447,10 @@ int bpx_net_run(void* stream, const bpx_net_desc* nets, int n_nets, int max_neur
   hipStream_t s = (hipStream_t)stream;
   if (conn_kind == BPX_NET_JIT)
     BPX_LAUNCH(KID_NET_RUN, (k_net_run<BPX_NET_JIT>), dim3((unsigned)n_nets), dim3(kRunThreads), shm, s, a, raster,
-               n_steps, max_neurons);
+               n_steps, max_neurons, status);
   else
     BPX_LAUNCH(KID_NET_RUN, (k_net_run<BPX_NET_CSR>), dim3((unsigned)n_nets), dim3(kRunThreads), shm, s, a, raster,
-               n_steps, max_neurons);
+               n_steps, max_neurons, status);
   BPX_CHECK_LAUNCH();
   return BPX_OK;
 }

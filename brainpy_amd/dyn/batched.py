@@ -64,7 +64,7 @@ def _register():
                                   ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(NetParams), ctypes.c_int64,
-                                  ctypes.c_void_p, ctypes.c_int64]
+                                  ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
         L.bpx_net_run.restype = ctypes.c_int
         L._bpx_net_registered = True
     return L
@@ -368,7 +368,7 @@ class BatchedCOBA:
                            _lib.ptr(self.g_exc), _lib.ptr(self.g_inh), _lib.ptr(self.counts), self.n_buf,
                            _lib.ptr(raster), _lib.ptr(self.spikes) if (self.record and raster is None) else None, _lib.ptr(self.events),
                            _lib.ptr(self.spike_count), ctypes.byref(self.params),
-                           device_offset if dev else self.i, _lib.ptr(self.d_step) if dev else None, steps)
+                           device_offset if dev else self.i, _lib.ptr(self.d_step) if dev else None, steps, None)
         _lib.check(rc, "bpx_net_run")
         self.i += steps
 

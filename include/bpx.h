@@ -359,14 +359,20 @@ int bpx_net_advance(void* stream, int64_t* d_step, int64_t delta);
  * every network has at most max_neurons <= bpx_net_run_max_neurons(n_buf)
  * neurons (5456 at n_buf = 2), and no neuron receives more than
  * max_in_degree <= 65535 synapses from one population (counts are kept as
- * 16-bit halves in LDS); larger values return BPX_ERR_SHAPE.  raster
- * (nullable) receives [n_steps][total] spike flags, spikes (nullable) the
- * flags of the last step. */
+ * 16-bit halves in LDS); larger values return BPX_ERR_SHAPE.  The
+ * descriptors live on the device, so the sizes of the networks are checked
+ * there: a network with more than max_neurons neurons is left unchanged and
+ * BPX_ERR_SHAPE is written to *status (nullable device int32, left as it is
+ * otherwise; the caller zeroes it).  max_in_degree is a precondition: the
+ * caller's bound on the synapses one neuron receives from one population
+ * (bpx does not recount it).  raster (nullable) receives [n_steps][total]
+ * spike flags, spikes (nullable) the flags of the last step. */
 int64_t bpx_net_run_max_neurons(int n_buf);
 int bpx_net_run(void* stream, const bpx_net_desc* nets, int n_nets, int max_neurons, int max_in_degree,
                 int conn_kind, int64_t total, float* V, float* t_last, float* g_exc, float* g_inh, uint32_t* counts,
                 int n_buf, uint8_t* raster, uint8_t* spikes, unsigned long long* events, unsigned int* spike_count,
-                const bpx_net_params* params, int64_t step, const int64_t* d_step, int64_t n_steps);
+                const bpx_net_params* params, int64_t step, const int64_t* d_step, int64_t n_steps,
+                int32_t* status);
 
 /* ---------------------------------------------------------------------------
  * Event-driven STDP weight updates.

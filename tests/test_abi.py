@@ -239,7 +239,7 @@ def test_argument_validation_of_net_run():
 
     def run(nets=d, n_nets=1, max_n=4000, in_deg=80, n_buf=2, steps=10, p=ctypes.byref(prm), V=d):
         return L.bpx_net_run(None, nets, n_nets, max_n, in_deg, 1, 4000, V, d, d, d, d, n_buf, None, None, None,
-                             None, p, 0, None, steps)
+                             None, p, 0, None, steps, None)
 
     assert run(nets=None) == ARG
     assert run(p=None) == ARG

@@ -483,6 +483,30 @@ def test_fused_run_at_the_lds_limit_and_fallback():
     big.run(5)      # per-step launches
 
 
+def test_fused_run_reports_a_network_above_max_neurons():
+    # bpx_net_run called directly with max_neurons below a network's size:
+    # that network is left unchanged and BPX_ERR_SHAPE is written to the
+    # status word; the others advance as usual
+    import ctypes
+    from brainpy_amd import _lib
+    specs = _jit_specs([3000, 1200], seed0=31)
+    b = BatchedCOBA(specs)
+    ref = BatchedCOBA(specs, V_init=b.V0)
+    V_big = b.V[b.net_slice(0)].clone()
+    status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    L = _lib.lib()
+    rc = L.bpx_net_run(_lib.stream_ptr(), _lib.ptr(b.d_descs), 2, 2000, b.max_in_degree, b.kind, b.total,
+                       _lib.ptr(b.V), _lib.ptr(b.t_last), _lib.ptr(b.g_exc), _lib.ptr(b.g_inh), _lib.ptr(b.counts),
+                       b.n_buf, None, None, _lib.ptr(b.events), _lib.ptr(b.spike_count), ctypes.byref(b.params), 0,
+                       None, 30, _lib.ptr(status))
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert int(status[0]) == -2
+    assert torch.equal(b.V[b.net_slice(0)], V_big)
+    ref.run(30, fused=False)
+    assert torch.equal(b.V[b.net_slice(1)], ref.V[ref.net_slice(1)])
+
+
 def test_coba4000_5000_steps_every_path_vs_oracle():
     # long horizon: the 4000-neuron network for 5000 steps (0.5 s of model
     # time), per-step launches, the multi-step launch and a captured graph of
