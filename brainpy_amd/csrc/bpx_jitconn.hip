@@ -297,7 +297,10 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
   extern __shared__ uint32_t s_bits[];
   copy_bits_to_lds(gbits, nwords, s_bits);
   __syncthreads();
-  constexpr bool HITS = DIST == BPX_DIST_HOMO && FIN;
+  // homogeneous weights add w0 per active connection like the others
+  // (e ? w0 : +0 with finite weights): the partial's sequence of additions
+  // is the reference lane's r += weight0 * e, and a stream switch is one add
+  constexpr bool HITS = false;
   const int64_t nthreads = (int64_t)gridDim.x * kJitThreads;
   const uint32_t ncol = (uint32_t)num_col;       // num_col < 2^32 (checked by the launcher)
   const uint32_t c0_last = step * 32u - 1u;      // step * l - 1 at l = 32: the row is done
@@ -314,7 +317,7 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
   while (__ballot(live) != 0ull) {
     const bool in = c < end;
     const uint32_t word = s_bits[(in ? c : 0u) >> 5];   // in flight during the draws
-    const uint32_t cb = c & 31u;
+    const uint32_t cb = c;                               // its bit: cb % 32 (c moves on below)
     float w = w0;
     if constexpr (DIST != BPX_DIST_HOMO) {
       if (in) w = draw_weight<DIST>(key, w0, w1);
@@ -345,9 +348,10 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
     }
     c = c + mod_clen<MK>(key.next(), dv, ds) + 1u;
     if constexpr (HITS) {
-      hits += in ? ((word >> cb) & 1u) : 0u;
+      hits += in ? __builtin_amdgcn_ubfe(word, cb, 1) : 0u;
     } else if (in) {
-      const bool on = (word >> cb) & 1u;
+      // bit-field extract of one bit: the offset's low 5 bits select it
+      const bool on = __builtin_amdgcn_ubfe(word, cb, 1) != 0u;
       if constexpr (FIN) acc += on ? w : 0.f;
       else acc += w * (on ? 1.f : 0.f);
     }
