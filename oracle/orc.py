@@ -10,7 +10,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liborc.so")
+# BPX_ORC_LIB selects another build of the same source (the sanitized one)
+_LIB_PATH = os.environ.get("BPX_ORC_LIB") or os.path.join(_HERE, "liborc.so")
 _lib = None
 
 # dtype codes (include/bpx.h)
