@@ -296,6 +296,7 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
                                                                 uint32_t step, float* __restrict__ out) {
   extern __shared__ uint32_t s_bits[];
   copy_bits_to_lds(gbits, nwords, s_bits);
+  if (threadIdx.x == 0) s_bits[nwords] = 0u;   // a zero word: the event word of lanes between streams
   __syncthreads();
   // homogeneous weights add w0 per active connection like the others
   // (e ? w0 : +0 with finite weights): the partial's sequence of additions
@@ -316,11 +317,14 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
   c = c + mod_clen<MK>(key.next(), dv, ds) + 1u;
   while (__ballot(live) != 0ull) {
     const bool in = c < end;
-    const uint32_t word = s_bits[(in ? c : 0u) >> 5];   // in flight during the draws
+    // in flight during the draws; lanes past their stream's end read the zero word
+    const uint32_t word = s_bits[in ? (c >> 5) : (uint32_t)nwords];
     const uint32_t cb = c;                               // its bit: cb % 32 (c moves on below)
     float w = w0;
     if constexpr (DIST != BPX_DIST_HOMO) {
-      if (in) w = draw_weight<DIST>(key, w0, w1);
+      // drawn by every lane: a lane past its stream's end advances a key it
+      // discards at the switch below, and its zero event bit drops the weight
+      if (FIN || in) w = draw_weight<DIST>(key, w0, w1);
     }
     if (!in && live) {
       if constexpr (HITS) {
@@ -347,13 +351,17 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
       end = min(c0 + step, ncol);
     }
     c = c + mod_clen<MK>(key.next(), dv, ds) + 1u;
+    // bit-field extract of one bit (the offset's low 5 bits select it)
+    const uint32_t bit = __builtin_amdgcn_ubfe(word, cb, 1);
     if constexpr (HITS) {
-      hits += in ? __builtin_amdgcn_ubfe(word, cb, 1) : 0u;
+      hits += bit;
+    } else if constexpr (FIN) {
+      // acc + w * e with e in {0, 1}: w * e is exact, so one fused
+      // multiply-add rounds exactly as the reference's product-then-add
+      // does; e = 0 (also every lane past its stream) adds a zero
+      acc = __builtin_fmaf(w, (float)bit, acc);
     } else if (in) {
-      // bit-field extract of one bit: the offset's low 5 bits select it
-      const bool on = __builtin_amdgcn_ubfe(word, cb, 1) != 0u;
-      if constexpr (FIN) acc += on ? w : 0.f;
-      else acc += w * (on ? 1.f : 0.f);
+      acc += w * (float)bit;
     }
   }
 }
@@ -444,7 +452,7 @@ static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t c
   const uint32_t step = (uint32_t)(((num_row + 1) >> 5) > 1 ? ((num_row + 1) >> 5) : 1);
   if (outdim) {
     const int64_t nwords = (num_col + 31) / 32;
-    if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && bits_fit(nwords) && ws &&
+    if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && bits_fit(nwords + 1) && ws &&
         ws_bytes >= (size_t)nwords * 4 && num_col < 0xFFFFFFFFll) {
       uint32_t* bits = (uint32_t*)ws;
       hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((num_col + 255) / 256)), dim3(256), 0, s, (const uint8_t*)vec,
@@ -452,12 +460,20 @@ static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t c
       int64_t blocks = (num_row + kJitThreads - 1) / kJitThreads;
       const int64_t maxb = (int64_t)jit_cus();
       if (blocks > maxb) blocks = maxb;
-      const bool fin = isfinite(w0) && (DIST == BPX_DIST_HOMO || isfinite(w1));
+      // finite weights, and every draw finite too (lanes past their stream's
+      // end draw a weight they multiply by a zero event bit): uniform draws
+      // lie within |w0| + |w1 - w0|, normal ones within |w0| + 6.8 |w1|
+      // (Box-Muller with u1 >= 1e-10 bounds |z| by sqrt(-2 ln 1e-10) < 6.8)
+      bool fin = isfinite(w0) && (DIST == BPX_DIST_HOMO || isfinite(w1));
+      if (DIST == BPX_DIST_UNIFORM)
+        fin = fin && isfinite(w1 - w0) && fabs((double)w0) + fabs((double)(w1 - w0)) < 3.0e38;
+      else if (DIST == BPX_DIST_NORMAL)
+        fin = fin && fabs((double)w0) + 8.0 * fabs((double)w1) < 3.0e38;
       DivShort ds = {1u, 0u, 0u};
       const bool shrt = DivShort::make((uint32_t)clen, &ds) && DivShort::mad24_ok((uint32_t)clen);
 #define JRB(MKV, F)                                                                                           \
   BPX_LAUNCH(KID_JIT_OUTDIM, (k_jit_rows_bits<DIST, MKV, F>), dim3((unsigned)blocks), dim3(kJitThreads),        \
-             (size_t)nwords * 4, s, (const uint32_t*)bits, nwords, w0, w1, dv, ds, seed, num_row, num_col, step, \
+             (size_t)(nwords + 1) * 4, s, (const uint32_t*)bits, nwords, w0, w1, dv, ds, seed, num_row, num_col, step, \
              out)
 #define JRB_F(MKV)             \
   do {                         \

@@ -123,6 +123,28 @@ def test_full_size_1m(od):
     np.testing.assert_array_equal(y, y2)
 
 
+@pytest.mark.parametrize("od", [True, False])
+@pytest.mark.parametrize("dist", ["uniform", "normal"])
+def test_full_size_1m_weighted(od, dist):
+    # the jitconn configuration with its weights: 1M x 1M, p = 1e-4, seed
+    # 42, 2% bool spikes, uniform [0, 1] or normal (0.5, 0.1).  Uniform with
+    # outdim_parallel sums each row in the oracle's order: bit-exact; the
+    # scatter form adds with atomics and normal weights use device
+    # log/sqrt/sin, so those are compared to a tolerance
+    n = 1_000_000
+    ev = np.random.default_rng(43).random(n) < 0.02
+    w0, w1 = (0.0, 1.0) if dist == "uniform" else (0.5, 0.1)
+    f = bm.jitconn.event_mv_prob_uniform if dist == "uniform" else bm.jitconn.event_mv_prob_normal
+    y = f(ev, w0, w1, 1e-4, 42, shape=(n, n), outdim_parallel=od).cpu().numpy()
+    ref = orc.jitconn_mv(ev, dist, 1, od, orc.LAYOUT_LANE32, n, n, np.float32(w0), np.float32(w1),
+                         np_ref.conn_len(1e-4), 42)
+    assert np.count_nonzero(ref) > n // 2
+    if dist == "uniform" and od:
+        np.testing.assert_array_equal(y, ref)
+    else:
+        np.testing.assert_allclose(y, ref, rtol=1e-5, atol=2e-5)
+
+
 @pytest.mark.parametrize("p", [1e-4, 2e-4, 5e-4])
 @pytest.mark.parametrize("dist,w0,w1", [("homo", 0.37, 0.0), ("homo", -1.25, 0.0), ("uniform", -0.5, 2.0)])
 def test_rows_kernel_bit_exact(p, dist, w0, w1):
@@ -140,4 +162,22 @@ def test_rows_kernel_bit_exact(p, dist, w0, w1):
     ref = orc.jitconn_mv(ev, dist, 1, True, orc.LAYOUT_LANE32, shape[0], shape[1], np.float32(w0), np.float32(w1),
                          np_ref.conn_len(p), 77)
     assert np.count_nonzero(ref) > 100
+    np.testing.assert_array_equal(y, ref)
+
+
+@pytest.mark.parametrize("dist,w0,w1", [("homo", np.inf, 0.0), ("homo", -np.inf, 0.0), ("uniform", -3e38, 3e38),
+                                        ("uniform", 1e38, 3.3e38), ("uniform", 0.0, np.nan)])
+def test_rows_kernel_nonfinite_and_overflowing_weights(dist, w0, w1):
+    # weights whose draws are not all finite take the multiply path
+    # (r += w * e, so inf * 0 gives NaN as the reference's lanes do); uniform
+    # bounds whose draws or sums overflow keep their infs where the oracle has them
+    shape = (6000, 6000)
+    ev = np.random.default_rng(5).random(shape[1]) < 0.05
+    if dist == "homo":
+        y = bm.jitconn.event_mv_prob_homo(ev, w0, 2e-3, 11, shape=shape, outdim_parallel=True).cpu().numpy()
+    else:
+        y = bm.jitconn.event_mv_prob_uniform(ev, w0, w1, 2e-3, 11, shape=shape, outdim_parallel=True).cpu().numpy()
+    ref = orc.jitconn_mv(ev, dist, 1, True, orc.LAYOUT_LANE32, shape[0], shape[1], np.float32(w0), np.float32(w1),
+                         np_ref.conn_len(2e-3), 11)
+    assert not np.all(np.isfinite(ref))
     np.testing.assert_array_equal(y, ref)
