@@ -530,3 +530,51 @@ def test_coba4000_5000_steps_every_path_vs_oracle():
     torch.cuda.synchronize()
     assert int(g.spike_count.sum()) == total
     np.testing.assert_array_equal(g.spikes.cpu().numpy(), ref[-1])
+
+
+def test_served_full_size_range_vs_oracle():
+    # the served configuration's size range, 4k to 1M neurons (a 1M-neuron
+    # request forced in), with in-degree 80: every request equals the
+    # oracle's serial run alone, bit for bit
+    from brainpy_amd.dyn.serve import DONE, Request, SimServer
+    reqs = _served_mix(seed=13, count=5, lo=4000, hi=1_000_000, smin=30, smax=60)
+    reqs.append(Request(len(reqs), 1_000_000, seed=1999, w_exc=0.6, w_inh=6.7, I_ext=20.0, steps=50,
+                        arrival=0.001 * len(reqs), keep_state=True))
+    srv = SimServer(capacity=1 << 22)
+    done = {r.rid: r for r in srv.serve(reqs)}
+    assert all(r.status == DONE for r in done.values())
+    for r in reqs:
+        spikes, events, V = _oracle_served(r, srv, r.steps)
+        assert spikes > 0
+        assert (r.spikes, r.events) == (spikes, events), r.rid
+        np.testing.assert_array_equal(r.V_final, V)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_c4_scale_batch_vs_oracle(graph):
+    # the batched configuration's shape: 256 networks, sizes log-uniform in
+    # 1k..64k, 4:1 E:I, jitconn in-degree 80 (above the one-launch run's
+    # size limit, so one launch per step, eager or replayed from a captured
+    # graph); a sample of networks (the smallest, the largest and four
+    # others) equals the oracle's serial run
+    rng = np.random.default_rng(256)
+    sizes = [int(np.exp(v)) for v in rng.uniform(np.log(1000), np.log(64000), 256)]
+    sizes[0], sizes[1] = 1000, 64000
+    batch = BatchedCOBA(_jit_specs(sizes, seed0=500))
+    assert not batch.fused
+    steps = 40
+    if graph:
+        batch.capture(10, fused=False)
+        for _ in range(steps // 10):
+            batch.replay()
+        torch.cuda.synchronize()
+        ref_batch = BatchedCOBA(_jit_specs(sizes, seed0=500))
+        raster = ref_batch.run(steps, raster=True, fused=False).cpu().numpy()
+        np.testing.assert_array_equal(batch.V.cpu().numpy(), ref_batch.V.cpu().numpy())
+    else:
+        raster = batch.run(steps, raster=True, fused=False).cpu().numpy()
+    pick = [0, 1] + [int(k) for k in rng.choice(np.arange(2, 256), 4, replace=False)]
+    for k in pick:
+        ref = _oracle_jit(batch, k, steps)
+        assert ref.sum() > 0
+        np.testing.assert_array_equal(raster[:, batch.net_slice(k)], ref)

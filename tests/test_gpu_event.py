@@ -380,6 +380,31 @@ def test_full_size_headline_matrix():
         assert np.all(err <= 1e-5 * mag + 1e-30), float(np.max(err / np.maximum(mag, 1e-30)))
 
 
+def test_full_size_bf16_weights():
+    # the BF16-stored variant of the 100k x 100k, p = 0.01 operator (1e8
+    # synapses): normal(0.5, 0.1) weights rounded to BF16, 10 % spikes,
+    # accumulated in FP32 by the public transpose path and the tiled plan;
+    # within 1e-5 of each output's sum of |w| against the FP64 sum of the
+    # same BF16 values
+    from brainpy_amd.connect import fixed_prob_csr_device
+    m = n = 100_000
+    dev = torch.device("cuda")
+    idx, ptr = fixed_prob_csr_device(m, n, 0.01, seed=77, device=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    ev = torch.rand(m, generator=g, device=dev) < 0.1
+    wb = torch.empty(idx.numel(), dtype=torch.float32, device=dev).normal_(0.5, 0.1, generator=g).to(torch.bfloat16)
+    idx_h, ptr_h, ev_h = idx.cpu().numpy(), ptr.cpu().numpy(), ev.cpu().numpy()
+    w_h = wb.float().cpu().numpy().astype(np.float64)
+    ref64 = orc.event_csrmv(w_h, idx_h, ptr_h, ev_h, (m, n), True)
+    mag = orc.event_csrmv(np.abs(w_h), idx_h, ptr_h, ev_h, (m, n), True)
+    for y in (bm.event.csrmv(wb, idx, ptr, ev, shape=(m, n), transpose=True),
+              bm.event.TiledCSR(wb, idx, ptr, shape=(m, n)).csrmv(ev)):
+        assert y.dtype == torch.float32
+        err = np.abs(y.cpu().numpy().astype(np.float64) - ref64)
+        assert np.all(err <= 1e-5 * mag + 1e-30), float(np.max(err / np.maximum(mag, 1e-30)))
+
+
 def bm_connect_csr(m, n, p, seed, sort):
     from brainpy_amd.connect import fixed_prob_csr as fp
     return fp(m, n, p, seed, sort=sort)

@@ -52,9 +52,11 @@ def test_lif_step_bit_exact(ltc):
         np.testing.assert_array_equal(a, b.cpu().numpy())
 
 
-def _oracle_net(net, steps):
+def _oracle_net(net, steps, snaps=None):
     """Serial CPU run of the same network: csrmv scatter in ascending row
-    order into a zeroed output, g += out, then the neuron step."""
+    order into a zeroed output, g += out, then the neuron step.  `snaps`
+    (a dict keyed by step) receives, for those steps, the state before the
+    projections and the conductances after them."""
     V = {"E": net.V0[:net.n_exc].copy(), "I": net.V0[net.n_exc:].copy()}
     tl = {p: np.full(n, -1e7, np.float32) for p, n in (("E", net.n_exc), ("I", net.n_inh))}
     sp = {p: np.zeros(n, np.uint8) for p, n in (("E", net.n_exc), ("I", net.n_inh))}
@@ -62,10 +64,14 @@ def _oracle_net(net, steps):
     raster = np.zeros((steps, net.n_exc + net.n_inh), np.uint8)
     for i in range(steps):
         t = np.float32(i) * np.float32(net.dt)
+        if snaps is not None and i in snaps:
+            snaps[i] = {"sp": {p: sp[p].copy() for p in sp}, "g": {p: [x.copy() for x in g[p]] for p in g}}
         for name, pre, post, _, k in net.proj_spec:
             w, idx, ptr, shape = net.csr[name]
             out = orc.event_csrmv(np.float32(w), idx, ptr, sp[pre].view(bool), shape, True)
             g[post][k] += out
+        if snaps is not None and i in snaps:
+            snaps[i]["g_after"] = {p: [x.copy() for x in g[p]] for p in g}
         for p in ("E", "I"):
             orc.lif_coba_step(V[p], tl[p], sp[p], g[p], net.tau_g, net.E_g, net.c_g, net.inp, net.params, t,
                               net.dt, ltc=net.ltc)
@@ -91,3 +97,28 @@ def test_coba_atomic_mode_close():
     raster = net.run(50).cpu().numpy()
     ref, _ = _oracle_net(net, 50)
     assert abs(int(raster.sum()) - int(ref.sum())) <= max(5, 0.05 * ref.sum())
+
+
+def test_coba_atomic_mode_per_step_conductance():
+    # the atomic mode's projections (g += scatter of the spikes, FP32
+    # atomics straight into g) from the oracle's own state at several steps:
+    # each conductance within 1e-6 relative of the oracle's serial
+    # g += (scatter into zeros), i.e. float32 rounding of the order only
+    net = EICOBANet(seed=1, mode="atomic")
+    snaps = {5: None, 20: None, 35: None, 49: None}
+    _oracle_net(net, 50, snaps)
+    dev = net.dev
+    delivered = 0
+    for i, st in snaps.items():
+        for p in ("E", "I"):
+            net.spike[p] = torch.from_numpy(st["sp"][p]).to(dev)
+            for k in range(2):
+                net.g[p][k] = torch.from_numpy(st["g"][p][k]).to(dev)
+        net._project()
+        for p in ("E", "I"):
+            for k in range(2):
+                got = net.g[p][k].cpu().numpy()
+                want = st["g_after"][p][k]
+                delivered += int(np.count_nonzero(want != st["g"][p][k]))
+                np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-6 * float(np.abs(want).max() + 1e-30))
+    assert delivered > 0
