@@ -946,16 +946,18 @@ def measured_traffic(name=HEADLINE_PROFILE):
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated) and the
     summary's own date line, or (None, None)."""
     path = os.path.join(ROOT, "profiles", name)
-    traffic, date = None, None
+    traffic, date, kus = None, None, None
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
                 traffic = int(line.split(":")[1])
             elif line.startswith("measured:"):
-                date = line.split(":", 1)[1].strip()
+                date = line.split(":", 1)[1].split()[0]
+            elif line.startswith("kernel_mean_us:"):
+                kus = float(line.split(":")[1])
     except OSError:
         pass
-    return traffic, date
+    return traffic, date, kus
 
 
 def _cpu_model():
@@ -1281,7 +1283,7 @@ def main():
     except OSError as e:
         print(f"# could not write {args.detail}: {e}", file=sys.stderr)
 
-    traffic, traffic_date = measured_traffic()
+    traffic, traffic_date, traffic_kus = measured_traffic()
     line = {
         "metric": "synaptic events/s",
         "value": head["events_per_s"],
@@ -1302,6 +1304,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(head["kernel_GBs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(head["kernel_GBs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": f"profiles/{HEADLINE_PROFILE} ({traffic_date})",
+                     # the profiled kernel's mean duration: a live kernel_us far from it
+                     # means the committed counters describe another build
+                     "traffic_profile_kernel_us": traffic_kus,
                      "kernel": "k_tiled_scatter", "kernel_us": round(head["kernel_us"], 3),
                      "algorithmic_bytes_per_launch": head["algorithmic_bytes"]},
         "cpu_baseline": None if cpu is None else {
