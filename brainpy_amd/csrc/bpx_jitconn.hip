@@ -295,8 +295,10 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
                                                                 uint32_t seed, int64_t num_row, int64_t num_col,
                                                                 uint32_t step, float* __restrict__ out) {
   extern __shared__ uint32_t s_bits[];
-  copy_bits_to_lds(gbits, nwords, s_bits);
-  if (threadIdx.x == 0) s_bits[nwords] = 0u;   // a zero word: the event word of lanes between streams
+  // word 0 is zero (the event word of lanes between streams), the bitmask
+  // follows from word 1
+  copy_bits_to_lds(gbits, nwords, s_bits + 1);
+  if (threadIdx.x == 0) s_bits[0] = 0u;
   __syncthreads();
   // homogeneous weights add w0 per active connection like the others
   // (e ? w0 : +0 with finite weights): the partial's sequence of additions
@@ -309,16 +311,16 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
   bool live = row < num_row;
   uint32_t kseed = seed + (uint32_t)(row * 32);  // stream key seed + row*32 + l
   Lfsr88 key(kseed);
-  uint32_t c0 = 0xFFFFFFFFu;                      // step * l - 1
-  uint32_t c = c0;
-  uint32_t end = min(c0 + step, ncol);
+  uint32_t c = 0xFFFFFFFFu;                       // step * l - 1
+  uint32_t cn = c + step;                         // step * (l + 1) - 1: the next stream's start
+  uint32_t end = min(cn, ncol);
   float acc = 0.f, racc = 0.f;
   uint32_t hits = 0;
   c = c + mod_clen<MK>(key.next(), dv, ds) + 1u;
-  while (__ballot(live) != 0ull) {
+  while (live) {
     const bool in = c < end;
     // in flight during the draws; lanes past their stream's end read the zero word
-    const uint32_t word = s_bits[in ? (c >> 5) : (uint32_t)nwords];
+    const uint32_t word = s_bits[in ? (c >> 5) + 1u : 0u];
     const uint32_t cb = c;                               // its bit: cb % 32 (c moves on below)
     float w = w0;
     if constexpr (DIST != BPX_DIST_HOMO) {
@@ -326,7 +328,7 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
       // discards at the switch below, and its zero event bit drops the weight
       if (FIN || in) w = draw_weight<DIST>(key, w0, w1);
     }
-    if (!in && live) {
+    if (!in) {
       if constexpr (HITS) {
         // a plain loop (no memory access, so no wait on the event word
         // read in flight): 0 or 1 terms for nearly every stream
@@ -337,18 +339,19 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
         acc = 0.f;
       }
       ++kseed;
-      c0 += step;
-      if (c0 == c0_last) {
+      c = cn;
+      cn += step;
+      if (c == c0_last) {
         out[row] = racc;
         racc = 0.f;
         row += nthreads;
         live = row < num_row;
         kseed = seed + (uint32_t)(row * 32);
-        c0 = 0xFFFFFFFFu;
+        c = 0xFFFFFFFFu;
+        cn = c + step;
       }
       key = Lfsr88(kseed);
-      c = c0;
-      end = min(c0 + step, ncol);
+      end = min(cn, ncol);
     }
     c = c + mod_clen<MK>(key.next(), dv, ds) + 1u;
     // bit-field extract of one bit (the offset's low 5 bits select it)
