@@ -315,28 +315,35 @@ def bench_forms(_lib, torch, steps=30, warmup=3):
              "kernel_us": kern * 1e6, "alg_bytes": alg, "gbs": alg / kern / 1e9,
              "frac": alg / kern / 1e9 / HBM_PEAK_GBS, "synapses_per_s": nnz / wall}
         res.append(r)
-        tag = f"{name} ({'bool' if vdt == _lib.BPX_BOOL else 'f32'} vector)"
+        tag = f"{name} ({'bool' if vdt == _lib.BPX_BOOL else 'f32'} vector)" + (" stateless" if transpose else "")
         print(f"# {tag:26s} transpose={transpose} {r['us_per_call']:8.1f} us/call kernel {r['kernel_us']:8.1f} us "
               f"{r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%) {r['synapses_per_s']:.3e} synapses/s",
               file=sys.stderr, flush=True)
-    # the transpose with the plan for a static matrix (TiledCSR.matvec): all
-    # 1e8 synapses streamed once at 2 + 4 bytes, no global atomic per synapse
-    from brainpy_amd.math.event import TiledCSR
-    plan = TiledCSR(w, idx, ptr, shape=(n, n))
+    # the public bm.sparse.csrmv(..., transpose=True) on the same arrays: its
+    # plan cache builds the tiled plan on the second call, then every call
+    # streams the 1e8 synapses once at 2 + 4 bytes with no global atomic per
+    # synapse (algorithmic bytes: the operator's inputs as passed)
+    import brainpy_amd.math as bm
     xs = torch.rand(n, generator=g, device=dev) + 0.5
+    for _ in range(2):
+        bm.sparse.csrmv(w, idx, ptr, xs, shape=(n, n), transpose=True)
+    hits0 = bm.event.plan_cache_info()["hits"]
 
     def pcall():
-        plan.matvec(xs)
+        bm.sparse.csrmv(w, idx, ptr, xs, shape=(n, n), transpose=True)
     wall, kern = _time_calls(_lib, torch, pcall, _lib.KID["tiled"], steps, warmup)
-    alg = n * 4 + n * 8 * plan.plan.tiles + nnz * 6 + n * 4
-    r = {"op": "sparse.csrmv (TiledCSR.matvec)", "transpose": True, "nnz": nnz, "us_per_call": wall * 1e6,
+    if bm.event.plan_cache_info()["hits"] - hits0 < steps:
+        raise RuntimeError("bm.sparse.csrmv(transpose=True) did not run on its cached plan")
+    alg = nnz * 8 + (n + 1) * 4 + n * 4 + n * 4
+    r = {"op": "bm.sparse.csrmv transpose (cached plan)", "transpose": True, "nnz": nnz, "us_per_call": wall * 1e6,
          "kernel_us": kern * 1e6, "alg_bytes": alg, "gbs": alg / kern / 1e9, "frac": alg / kern / 1e9 / HBM_PEAK_GBS,
          "synapses_per_s": nnz / wall}
     res.append(r)
-    print(f"# sparse.csrmv transpose=1 on the plan {r['us_per_call']:8.1f} us/call kernel {r['kernel_us']:8.1f} us "
-          f"{r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%) {r['synapses_per_s']:.3e} synapses/s", file=sys.stderr,
-          flush=True)
-    del plan, xs
+    print(f"# bm.sparse.csrmv transpose=1 (public path, cached plan) {r['us_per_call']:8.1f} us/call kernel "
+          f"{r['kernel_us']:8.1f} us {r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%) {r['synapses_per_s']:.3e} "
+          f"synapses/s", file=sys.stderr, flush=True)
+    bm.event.clear_plan_cache()
+    del xs
     probe_out = torch.empty(32768 * 256, dtype=torch.float32, device=dev)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for k in range(13):

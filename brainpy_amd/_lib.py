@@ -31,6 +31,7 @@ class TiledCSRPlan(ctypes.Structure):
     """Mirror of bpx_tiled_csr (include/bpx.h)."""
     _fields_ = [("m", ctypes.c_int64), ("n", ctypes.c_int64), ("nnz", ctypes.c_int64), ("tiles", ctypes.c_int32),
                 ("tile_len", ctypes.c_int32), ("weight_dtype", ctypes.c_int32), ("fx_shift", ctypes.c_int32),
+                ("coarse", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("capacity", ctypes.c_int64), ("rowtile_offset", ctypes.c_int64), ("aux_offset", ctypes.c_int64),
                 ("lidx_offset", ctypes.c_int64),
                 ("weights_offset", ctypes.c_int64), ("homo_offset", ctypes.c_int64), ("mem", ctypes.c_void_p)]

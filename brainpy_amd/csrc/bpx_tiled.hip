@@ -171,22 +171,48 @@ __global__ __launch_bounds__(kBuildThreads) void k_tiled_fill(const V* __restric
   }
 }
 
-template <typename V>
+// out[0] = max |v| (NaN propagates, or with FINITE_ONLY the max over finite
+// values), out[1] = min |v| over the finite non-zero values (+inf if none);
+// out must hold {0, +inf} before (k_init_minmax)
+template <typename V, bool FINITE_ONLY = false>
 __global__ __launch_bounds__(256) void k_absmax(const V* __restrict__ data, int64_t len, uint32_t* __restrict__ out) {
   int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * 256;
-  float mx = 0.f;
+  float mx = 0.f, mn = __uint_as_float(0x7F800000u);
   for (; i < len; i += stride) {
-    const float v = fabsf((float)to_f32(data[i]));
+    float v = fabsf((float)to_f32(data[i]));
+    const bool fin = v <= 3.402823466e38f;
+    if (fin && v > 0.f) mn = fminf(mn, v);
+    if constexpr (FINITE_ONLY) v = fin ? v : 0.f;
     mx = (v > mx || v != v) ? v : mx;    // NaN propagates
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     const float y = __shfl_xor(mx, o, 64);
     mx = (y > mx || y != y) ? y : mx;
+    mn = fminf(mn, __shfl_xor(mn, o, 64));
   }
   // non-negative floats (and NaN/inf) order like their bit patterns
-  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(mx));
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(out, __float_as_uint(mx));
+    atomicMin(out + 1, __float_as_uint(mn));
+  }
+}
+
+__global__ void k_init_minmax(uint32_t* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    p[0] = 0u;
+    p[1] = 0x7F800000u;
+  }
+}
+
+// The fixed-point grid (fx_inv, one unit in weight or product units) is
+// fine enough when it is at most 2e-6 of the smallest non-zero term
+// |w|min (* |x|min for a dense x): each term's rounding then costs at most
+// 1e-6 of that term, so an output stays within 1e-6 of its sum of |terms|.
+// Wider ranges take the coarse path: direct FP32 atomics into out.
+__device__ __forceinline__ bool grid_too_coarse(double fx_inv, float tmin) {
+  return tmin <= 3.402823466e38f && fx_inv > 2e-6 * (double)tmin;
 }
 
 __global__ void k_store_scalar(const void* src, int src_dtype, float* dst) {
@@ -228,9 +254,17 @@ __device__ __forceinline__ long long fixed_of(float w, double scale) {
 // DENSE: `ev` is a dense f32 vector x (bm.sparse.csrmv with transpose): the
 // rows with x != 0 are listed with x[row] beside their segment, each weight
 // is multiplied by it (one f32 rounding, as the plain product), and the
-// fixed-point shift comes from the plan's less ceil(log2 max|x|) (device
-// scalar xmax: the f32 bits of max|x|), so the sums still stay below 2^61.
-template <int WK, typename E, int R, bool DENSE>
+// fixed-point shift comes from the plan's less ceil(log2 max|x|) over the
+// finite x (device scalar xmax: the f32 bits of that max), so the sums still
+// stay below 2^61.  A product that is inf or NaN (non-finite x, or an f32
+// overflow) is not summed in fixed point: it sets a flag of its output (+inf,
+// -inf, NaN) in LDS, and the flush adds that special value instead, so the
+// output follows IEEE addition (inf + finite = inf, +inf + -inf = NaN).
+// `homo_w` points at the plan's scalars {homogeneous weight, max |w|,
+// min non-zero |w|} and `xmax` at x's {max finite |x|, min non-zero |x|}.
+// COARSE (events; the plan's flag): every term goes to out with an FP32
+// atomic; DENSE decides the same per call on the device.
+template <int WK, typename E, int R, bool DENSE, bool COARSE>
 __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* __restrict__ rowtile,
                                                                  const uint16_t* __restrict__ lidx,
                                                                  const void* __restrict__ wptr,
@@ -242,12 +276,14 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
   extern __shared__ unsigned long long s_acc[];
   __shared__ int2 s_seg[kWavesPerBlock][kListCap];
   __shared__ float s_xv[DENSE ? kWavesPerBlock : 1][DENSE ? kListCap : 1];
-  bool finite_x = true;
+  // DENSE: non-finite flags, 8 bits per output (1 +inf, 2 -inf, 4 NaN)
+  uint32_t* s_flag = reinterpret_cast<uint32_t*>(s_acc + T);
+  bool coarse = COARSE;
   if constexpr (DENSE) {
-    const float mx = __uint_as_float(*xmax);
-    finite_x = mx <= 3.402823466e38f;       // false for inf and NaN
+    const float mx = __uint_as_float(xmax[0]);
+    const float tmin = __uint_as_float(reinterpret_cast<const uint32_t*>(homo_w)[2]) * __uint_as_float(xmax[1]);
     int ex = 0;
-    if (finite_x && mx > 0.f) frexpf(mx, &ex);   // mx < 2^ex
+    if (mx > 0.f && mx <= 3.402823466e38f) frexpf(mx, &ex);   // mx < 2^ex
     // products |w * x| < max|w| * 2^ex: scaling by 2^(shift - ex) keeps every
     // term below 2^50 and every sum below 2^61 for any ex, and for |x| < 1
     // (ex <= 0) it raises the resolution with the products' magnitude
@@ -255,6 +291,7 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
       fx_scale = ldexp(fx_scale, -ex);
       fx_inv = ldexp(fx_inv, ex);
     }
+    coarse = WK != 0 && grid_too_coarse(fx_inv, tmin);
   }
   // a batch of event vectors: vector blockIdx.y of [B][m] into [B][n]
   ev += (int64_t)blockIdx.y * m;
@@ -268,6 +305,8 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
   const int64_t t0 = (int64_t)p * T;
   const int tlen = (int)((n - t0) < (int64_t)T ? (n - t0) : (int64_t)T);
   for (int k = threadIdx.x; k < T; k += kTiledThreads) s_acc[k] = 0ull;
+  if constexpr (DENSE)
+    for (int k = threadIdx.x; k < (T + 3) / 4; k += kTiledThreads) s_flag[k] = 0u;
   __syncthreads();
   float hw = 0.f;
   if constexpr (WK == 0) hw = *homo_w;
@@ -294,7 +333,7 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
 #pragma unroll
     for (int u = 0; u < kChunksPerBatch; ++u) {
       const int64_t r = ((qb + cstep * u) << 6) + lane;
-      act[u] = finite_x && r < m && ev_true(v[u], rule);
+      act[u] = r < m && ev_true(v[u], rule);
       if constexpr (DENSE) xv[u] = (float)v[u];
     }
   };
@@ -399,13 +438,30 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
 #pragma unroll
           for (int u = 0; u < kVec; ++u) {
             const uint32_t c = (l4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
-            if (c < (uint32_t)tlen) {
+            if (c < (uint32_t)tlen && (COARSE || (DENSE && coarse))) {
+              // coarse grid: the term goes straight to its output
+              if constexpr (DENSE) atomic_add(out + t0 + c, wv[q][u] * xr[q]);
+              else if constexpr (WK != 0) atomic_add(out + t0 + c, wv[q][u]);
+            } else if (c < (uint32_t)tlen) {
               long long fx;
-              if constexpr (WK == 0) fx = hq[q];
-              else if constexpr (DENSE) fx = fixed_of(wv[q][u] * xr[q], fx_scale);
-              else fx = fixed_of(wv[q][u], fx_scale);
-              __hip_atomic_fetch_add(s_acc + c, (unsigned long long)fx, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+              bool special = false;
+              if constexpr (DENSE) {
+                const float prod = (WK == 0) ? hw * xr[q] : wv[q][u] * xr[q];
+                special = !(fabsf(prod) <= 3.402823466e38f);
+                if (special) {
+                  const uint32_t bits = prod != prod ? 4u : (prod > 0.f ? 1u : 2u);
+                  __hip_atomic_fetch_or(s_flag + (c >> 2), bits << (8 * (c & 3)), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                fx = (WK == 0) ? hq[q] : fixed_of(prod, fx_scale);
+              } else if constexpr (WK == 0) {
+                fx = hq[q];
+              } else {
+                fx = fixed_of(wv[q][u], fx_scale);
+              }
+              if (!special)
+                __hip_atomic_fetch_add(s_acc + c, (unsigned long long)fx, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
             }
           }
         }
@@ -420,16 +476,18 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
     q0 = qn;
   }
   __syncthreads();
-  if (DENSE && !finite_x) {
-    // inf or NaN in x: the products are not representable in fixed point;
-    // every output of the tile becomes NaN (written by copy 0)
-    if (copy == 0)
-      for (int k = threadIdx.x; k < tlen; k += kTiledThreads) out[t0 + k] = __uint_as_float(0x7FC00000u);
-    return;
-  }
   for (int k = threadIdx.x; k < tlen; k += kTiledThreads) {
     const long long a = (long long)s_acc[k];
-    if (a != 0) atomic_add(out + t0 + k, (float)((double)a * fx_inv));
+    uint32_t f = 0;
+    if constexpr (DENSE) f = (s_flag[k >> 2] >> (8 * (k & 3))) & 7u;
+    if (f != 0) {
+      // +inf and -inf together, or a NaN term: NaN; else the one infinity
+      const float sv = ((f & 4u) || f == 3u) ? __uint_as_float(0x7FC00000u)
+                                               : __uint_as_float((f & 1u) ? 0x7F800000u : 0xFF800000u);
+      atomic_add(out + t0 + k, sv);
+    } else if (a != 0) {
+      atomic_add(out + t0 + k, (float)((double)a * fx_inv));
+    }
   }
 }
 
@@ -444,8 +502,6 @@ __global__ __launch_bounds__(256) void k_fill_zero_t(O* __restrict__ out, int64_
 // latency of the row walk
 static int tiled_copies_per_xcd() { return 2; }
 
-// rows a lane group streams per round
-static int tiled_rows_per_group() { return 1; }
 
 struct TiledShape {
   int P, T;
@@ -501,6 +557,8 @@ int bpx_tiled_csr_layout(int64_t m, int64_t n, int64_t nnz, int weight_dtype, bp
   plan->tiles = ts.P;
   plan->tile_len = ts.T;
   plan->weight_dtype = weight_dtype;
+  plan->coarse = 0;
+  plan->reserved = 0;
   size_t off = 0;
   auto take = [&](size_t b) {
     size_t o = off;
@@ -516,7 +574,7 @@ int bpx_tiled_csr_layout(int64_t m, int64_t n, int64_t nnz, int weight_dtype, bp
   size_t o_li = take((size_t)cap * sizeof(uint16_t));
   size_t wb = weight_dtype == BPX_F32 ? 4 : weight_dtype == BPX_BF16 ? 2 : 0;
   size_t o_w = take((size_t)cap * wb);
-  size_t o_h = take(2 * sizeof(float));
+  size_t o_h = take(4 * sizeof(float));     // homogeneous weight, max |w|, min non-zero |w|
   plan->aux_offset = (int64_t)o_aux;
   plan->rowtile_offset = (int64_t)o_rt;
   plan->lidx_offset = (int64_t)o_li;
@@ -554,7 +612,7 @@ int bpx_tiled_csr_build(void* stream, const void* data, int data_dtype, int64_t 
   // fixed-point scale of the LDS accumulators: |sum| <= max|w| * m < 2^62
   {
     uint32_t* amax = (uint32_t*)(base + plan->homo_offset + sizeof(float));
-    hipLaunchKernelGGL(k_fill_zero_t<uint32_t>, dim3(1), dim3(64), 0, s, amax, (int64_t)1);
+    hipLaunchKernelGGL(k_init_minmax, dim3(1), dim3(64), 0, s, amax);
     const int64_t len = homo ? 1 : nnz;
     int64_t ab = (len + 255) / 256;
     if (ab > 1024) ab = 1024;
@@ -562,11 +620,12 @@ int bpx_tiled_csr_build(void* stream, const void* data, int data_dtype, int64_t 
     if (data_dtype == BPX_F64) hipLaunchKernelGGL(k_absmax<double>, dim3((unsigned)ab), dim3(256), 0, s, (const double*)data, len, amax);
     else if (data_dtype == BPX_BF16) hipLaunchKernelGGL(k_absmax<bf16_t>, dim3((unsigned)ab), dim3(256), 0, s, (const bf16_t*)data, len, amax);
     else hipLaunchKernelGGL(k_absmax<float>, dim3((unsigned)ab), dim3(256), 0, s, (const float*)data, len, amax);
-    uint32_t bits = 0;
-    if (hipMemcpyAsync(&bits, amax, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return BPX_ERR_LAUNCH;
+    uint32_t bits[2] = {0u, 0u};
+    if (hipMemcpyAsync(bits, amax, sizeof(bits), hipMemcpyDeviceToHost, s) != hipSuccess) return BPX_ERR_LAUNCH;
     if (hipStreamSynchronize(s) != hipSuccess) return BPX_ERR_LAUNCH;
-    float mx;
-    memcpy(&mx, &bits, sizeof(float));
+    float mx, mn;
+    memcpy(&mx, &bits[0], sizeof(float));
+    memcpy(&mn, &bits[1], sizeof(float));
     if (!(mx == mx) || mx == INFINITY) return BPX_ERR_ARG;      // non-finite weights
     double bound = (double)mx * (double)(m > 1 ? m : 1);
     int e = bound > 0.0 ? (int)ceil(log2(bound)) : -100;
@@ -577,6 +636,8 @@ int bpx_tiled_csr_build(void* stream, const void* data, int data_dtype, int64_t 
     if (shift > cap) shift = cap;
     if (shift > 200) shift = 200;
     plan->fx_shift = shift;
+    // the grid's unit against the smallest non-zero weight
+    plan->coarse = (mn <= 3.402823466e38f && ldexp(1.0, -shift) > 2e-6 * (double)mn) ? 1 : 0;
   }
   if (m == 0) {
     BPX_CHECK_LAUNCH();
@@ -655,18 +716,17 @@ int bpx_event_csrmv_tiled_batch(void* stream, const bpx_tiled_csr* plan, const v
   const float* hw = (const float*)(base + plan->homo_offset);
   const int P = plan->tiles, T = plan->tile_len;
   const int cpx = tiled_copies_per_xcd();
-  const int rows = tiled_rows_per_group();
   const unsigned blocks = (unsigned)(8 * cpx * P);
   const size_t shm = (size_t)T * sizeof(unsigned long long);
   const double fx_scale = ldexp(1.0, plan->fx_shift), fx_inv = ldexp(1.0, -plan->fx_shift);
-#define TLR(WK, E, R)                                                                                     \
-  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E, R, false>), dim3(blocks, (unsigned)batch), dim3(kTiledThreads), shm, s, rowtile, \
+#define TLR(WK, E, CO)                                                                                    \
+  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E, 1, false, CO>), dim3(blocks, (unsigned)batch), dim3(kTiledThreads), shm, s, rowtile, \
              lidx, w, hw, (const E*)events, ev_rule, m, n, P, T, cpx, fx_scale, fx_inv,                         \
              (const uint32_t*)nullptr, out)
 #define TL(WK, E)                          \
   do {                                     \
-    if (rows == 1) TLR(WK, E, 1);          \
-    else TLR(WK, E, 2);                    \
+    if (plan->coarse) TLR(WK, E, true);    \
+    else TLR(WK, E, false);                \
   } while (0)
 #define TL_E(WK)                                     \
   do {                                               \
@@ -688,7 +748,7 @@ int bpx_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* x, int 
                     void* workspace, size_t workspace_bytes) {
   if (!plan || !plan->mem) return BPX_ERR_ARG;
   if (x_dtype != BPX_F32) return BPX_ERR_DTYPE;
-  if (!workspace || workspace_bytes < sizeof(uint32_t)) return BPX_ERR_WORKSPACE;
+  if (!workspace || workspace_bytes < 2 * sizeof(uint32_t)) return BPX_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
   const int64_t m = plan->m, n = plan->n;
   if (!accumulate) {
@@ -700,11 +760,11 @@ int bpx_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* x, int 
     BPX_CHECK_LAUNCH();
     return BPX_OK;
   }
-  uint32_t* xmax = (uint32_t*)workspace;
-  hipLaunchKernelGGL(k_fill_zero_t<uint32_t>, dim3(1), dim3(64), 0, s, xmax, (int64_t)1);
+  uint32_t* xmax = (uint32_t*)workspace;      // {max finite |x|, min non-zero finite |x|}
+  hipLaunchKernelGGL(k_init_minmax, dim3(1), dim3(64), 0, s, xmax);
   int64_t ab = (m + 255) / 256;
   if (ab > 1024) ab = 1024;
-  hipLaunchKernelGGL(k_absmax<float>, dim3((unsigned)ab), dim3(256), 0, s, (const float*)x, m, xmax);
+  hipLaunchKernelGGL((k_absmax<float, true>), dim3((unsigned)ab), dim3(256), 0, s, (const float*)x, m, xmax);
   const char* base = (const char*)plan->mem;
   const int32_t* rowtile = (const int32_t*)(base + plan->rowtile_offset);
   const uint16_t* lidx = (const uint16_t*)(base + plan->lidx_offset);
@@ -712,18 +772,14 @@ int bpx_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* x, int 
   const float* hw = (const float*)(base + plan->homo_offset);
   const int P = plan->tiles, T = plan->tile_len;
   const int cpx = tiled_copies_per_xcd();
-  const int rows = tiled_rows_per_group();
   const unsigned blocks = (unsigned)(8 * cpx * P);
-  const size_t shm = (size_t)T * sizeof(unsigned long long);
+  // the accumulators, then the outputs' non-finite flags (one byte each)
+  const size_t shm = (size_t)T * sizeof(unsigned long long) + (size_t)((T + 3) / 4) * sizeof(uint32_t);
   const double fx_scale = ldexp(1.0, plan->fx_shift), fx_inv = ldexp(1.0, -plan->fx_shift);
 #define TLD(WK, R)                                                                                            \
-  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, float, R, true>), dim3(blocks), dim3(kTiledThreads), shm, s,      \
+  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, float, R, true, false>), dim3(blocks), dim3(kTiledThreads), shm, s, \
              rowtile, lidx, w, hw, (const float*)x, BPX_RULE_NONZERO, m, n, P, T, cpx, fx_scale, fx_inv, xmax, out)
-#define TLDR(WK)                         \
-  do {                                   \
-    if (rows == 1) TLD(WK, 1);           \
-    else TLD(WK, 2);                     \
-  } while (0)
+#define TLDR(WK) TLD(WK, 1)
   if (plan->weight_dtype == BPX_TILED_HOMO) TLDR(0);
   else if (plan->weight_dtype == BPX_F32) TLDR(1);
   else TLDR(2);

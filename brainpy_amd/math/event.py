@@ -578,8 +578,9 @@ class TiledCSR:
         rows with x != 0 stream their segments and every product ``w *
         x[row]`` (rounded to f32) is summed exactly in fixed point, with no
         global atomic per synapse.  ``out += x @ A`` when ``out`` is given.
-        Non-finite x gives NaN outputs.  Calls on one plan share a 4-byte
-        device workspace: do not overlap them on different streams."""
+        Products that are inf or NaN (non-finite x, f32 overflow) reach
+        their outputs as IEEE addition would: an infinity, or NaN for a NaN
+        term or infinities of both signs."""
         xt = _lib.to_device(_as_array(x))
         if xt.dtype != torch.float32:
             xt = xt.to(torch.float32)
@@ -592,9 +593,8 @@ class TiledCSR:
             out = torch.empty(n, dtype=torch.float32, device=xt.device)
         elif out.dtype != torch.float32 or out.shape != (n,) or not out.is_contiguous():
             raise ValueError(f'out must be a contiguous float32 vector of shape ({n},).')
-        if getattr(self, "_ws", None) is None:
-            self._ws = torch.empty(4, dtype=torch.uint8, device=xt.device)
+        ws = torch.empty(8, dtype=torch.uint8, device=xt.device)     # max and min |x| of this call
         rc = _lib.lib().bpx_csrmv_tiled(_lib.stream_ptr(), ctypes.byref(self.plan), _lib.ptr(xt),
-                                        _lib.BPX_F32, int(accumulate), _lib.ptr(out), _lib.ptr(self._ws), 4)
+                                        _lib.BPX_F32, int(accumulate), _lib.ptr(out), _lib.ptr(ws), 8)
         _lib.check(rc, "bpx_csrmv_tiled")
         return out

@@ -185,6 +185,8 @@ def test_argument_validation_of_coo_conn_and_plan_entry_points():
     plan.mem = 1
     assert L.bpx_csrmv_tiled(None, ctypes.byref(plan), None, F64, 0, None, None, 0) == DTYPE
     assert L.bpx_csrmv_tiled(None, ctypes.byref(plan), None, F32, 0, None, None, 0) == WS
+    # the workspace holds max and min |x|: 8 bytes (checked before any launch)
+    assert L.bpx_csrmv_tiled(None, ctypes.byref(plan), None, F32, 0, None, ctypes.c_void_p(256), 4) == WS
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libbpx.so not built")

@@ -313,11 +313,35 @@ def test_tiled_plan_dense_matvec_accumulate_and_nonfinite():
     plan.matvec(x, out=base)
     D = dense_of(w, indices, indptr, (m, n))
     np.testing.assert_allclose(base.cpu().numpy(), 2.0 + x.astype(np.float64) @ D, rtol=1e-5, atol=1e-5)
-    x[7] = np.inf
-    assert torch.isnan(plan.matvec(x)).all()
-    x[7] = np.nan
-    assert torch.isnan(plan.matvec(x)).all()
-    assert not torch.isnan(plan.matvec(np.zeros(m, np.float32))).any()
+    # non-finite products reach their outputs as IEEE addition does: row 7
+    # at +inf gives +inf (or -inf through negative weights) where it has
+    # synapses, rows 7 (+inf) and 8 (-inf) sharing a column give NaN, a NaN
+    # row gives NaN on its columns; every other output stays finite and
+    # equal to the product without those rows
+    D64 = D.astype(np.float64)
+
+    def ieee_ref(xv):
+        with np.errstate(invalid="ignore", over="ignore"):
+            prods = xv.astype(np.float32)[:, None] * D.astype(np.float32)      # the f32 products
+            prods[D == 0] = 0.0
+            return prods.astype(np.float64).sum(axis=0)
+
+    for special in ([(7, np.inf)], [(7, -np.inf)], [(7, np.nan)], [(7, np.inf), (8, -np.inf)],
+                    [(7, np.float32(3e38))]):
+        xs = x.copy()
+        for r, v in special:
+            xs[r] = v
+        y = plan.matvec(xs).cpu().numpy().astype(np.float64)
+        ref = ieee_ref(xs)
+        fin = np.isfinite(ref)
+        assert (~fin).any() == any(not np.isfinite(v) for _, v in special)
+        np.testing.assert_array_equal(np.isnan(y), np.isnan(ref))
+        np.testing.assert_array_equal(np.isposinf(y), np.isposinf(ref))
+        np.testing.assert_array_equal(np.isneginf(y), np.isneginf(ref))
+        xf = np.where(np.isfinite(xs), xs, 0).astype(np.float64)
+        mag = np.abs(xf) @ np.abs(D64)
+        assert np.all(np.abs(y[fin] - ref[fin]) <= 1e-5 * mag[fin] + 1e-30)
+    assert np.isfinite(plan.matvec(np.zeros(m, np.float32)).cpu().numpy()).all()
 
 
 def test_tiled_plan_accumulate_and_float_events():
@@ -408,3 +432,36 @@ def test_full_size_bf16_weights():
 def bm_connect_csr(m, n, p, seed, sort):
     from brainpy_amd.connect import fixed_prob_csr as fp
     return fp(m, n, p, seed, sort=sort)
+
+
+def test_tiled_plan_wide_value_ranges_stay_accurate():
+    # weights or vector values spanning far more than the fixed-point grid
+    # resolves: the plan switches to direct FP32 atomics, so outputs fed only
+    # by tiny terms keep 1e-5 of their own sum of |terms| (on the grid alone
+    # they would round to a few units)
+    m, n = 2000, 3000
+    indices, indptr = bm_connect_csr(m, n, 0.01, 31, True)
+    rng = np.random.default_rng(12)
+    w = rng.normal(0.5, 0.1, indices.size).astype(np.float32)
+    w[indices >= n // 2] *= np.float32(1e-12)          # the right half of the outputs sees only tiny weights
+    plan = bm.event.TiledCSR(w, indices, indptr, shape=(m, n))
+    D = dense_of(w, indices, indptr, (m, n)).astype(np.float64)
+    ev = bernoulli(m, 0.2, 13)
+    y = plan.csrmv(ev).cpu().numpy().astype(np.float64)
+    ref = ev.astype(np.float64) @ D
+    mag = ev.astype(np.float64) @ np.abs(D)
+    assert np.all(np.abs(y - ref) <= 1e-5 * mag + 1e-45)
+    assert np.abs(y[n // 2:]).max() > 0
+    # a dense vector with a few huge entries: outputs away from their rows
+    # are sums of O(1) terms only
+    w2 = rng.normal(0.5, 0.1, indices.size).astype(np.float32)
+    plan2 = bm.event.TiledCSR(w2, indices, indptr, shape=(m, n))
+    D2 = dense_of(w2, indices, indptr, (m, n)).astype(np.float64)
+    x = rng.standard_normal(m).astype(np.float32)
+    x[:10] = np.float32(1e20)
+    y2 = plan2.matvec(x).cpu().numpy().astype(np.float64)
+    ref2 = x.astype(np.float64) @ D2
+    mag2 = np.abs(x.astype(np.float64)) @ np.abs(D2)
+    assert np.all(np.abs(y2 - ref2) <= 1e-5 * mag2 + 1e-30)
+    small = mag2 < 1e6
+    assert small.sum() > n // 2 and np.abs(y2[small]).max() > 0

@@ -138,3 +138,51 @@ def test_capacity_evicts_least_recent():
     assert bm.event.plan_cache_info()["plans"] == 2
     for (idx, ptr, w), t in zip(mats, ts):
         _check(bm.event.csrmv(*t, ev, shape=(500, 700), transpose=True), w, idx, ptr, ev.cpu().numpy(), (500, 700))
+
+
+def test_sparse_transpose_runs_on_the_plan():
+    # bm.sparse.csrmv(..., transpose=True) shares the cache: the second call
+    # on the same arrays builds the plan, later calls hit it; every call
+    # within 1e-5 of the FP64 product's term magnitudes; f32 and bf16 data
+    idx, ptr, w = _inputs(seed=11)
+    for wd in (torch.float32, torch.bfloat16):
+        bm.event.clear_plan_cache()
+        tw = torch.from_numpy(w).to(DEV).to(wd)
+        wr = tw.float().cpu().numpy()
+        ti, tp = torch.from_numpy(idx).to(DEV), torch.from_numpy(ptr).to(DEV)
+        D = dense_of(wr, idx, ptr, (2000, 3000))
+        for k in range(4):
+            x = np.random.default_rng(k).standard_normal(2000).astype(np.float32)
+            x[::3] = 0.0
+            y = bm.sparse.csrmv(tw, ti, tp, torch.from_numpy(x).to(DEV), shape=(2000, 3000), transpose=True)
+            assert y.dtype == torch.float32
+            ref = x.astype(np.float64) @ D
+            mag = np.abs(x.astype(np.float64)) @ np.abs(D)
+            assert np.all(np.abs(y.cpu().numpy() - ref) <= 1e-5 * mag + 1e-30)
+            info = bm.event.plan_cache_info()
+            assert info["builds"] == (0 if k == 0 else 1) and info["hits"] == max(0, k - 1)
+        # an event call on the same arrays reuses the plan
+        ev = torch.from_numpy(bernoulli(2000, 0.1, 8)).to(DEV)
+        bm.event.csrmv(tw, ti, tp, ev, shape=(2000, 3000), transpose=True)
+        assert bm.event.plan_cache_info()["builds"] == 1
+
+
+def test_sparse_transpose_nonfinite_on_the_plan_equals_stateless():
+    # inf / NaN in the vector: the plan's outputs have the stateless
+    # scatter's NaN / +inf / -inf pattern and agree elsewhere
+    idx, ptr, w = _inputs(seed=12)
+    t = [torch.from_numpy(a).to(DEV) for a in (w, idx, ptr)]
+    x = np.random.default_rng(3).standard_normal(2000).astype(np.float32)
+    x[5], x[9], x[11] = np.inf, -np.inf, np.nan
+    xt = torch.from_numpy(x).to(DEV)
+    bm.event.set_plan_cache(enabled=False)
+    y0 = bm.sparse.csrmv(*t, xt, shape=(2000, 3000), transpose=True).cpu().numpy()
+    bm.event.set_plan_cache(enabled=True)
+    for _ in range(3):
+        y = bm.sparse.csrmv(*t, xt, shape=(2000, 3000), transpose=True).cpu().numpy()
+    assert bm.event.plan_cache_info()["hits"] >= 1
+    for f in (np.isnan, np.isposinf, np.isneginf):
+        np.testing.assert_array_equal(f(y), f(y0))
+    fin = np.isfinite(y0)
+    assert (~fin).any() and fin.any()
+    np.testing.assert_allclose(y[fin], y0[fin], rtol=1e-5, atol=1e-5)
