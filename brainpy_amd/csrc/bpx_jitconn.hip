@@ -370,6 +370,133 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
 }
 
 // ---------------------------------------------------------------------------
+// lane32 scatter (outdim_parallel=False), bool events: exclusive row tiles.
+//
+// Stream (col, l) of the reference's event kernel covers the rows
+// [step*l, step*l + step - 1) of the output (jitconn/_event_matvec_taichi.py
+// event kernels: i_row = step*index - 1, end = min(i_row + step, num_row)).
+// Block (l, k) owns the rows [lo, hi) = step*l + [k*T, (k+1)*T) (within the
+// stream's range and num_row) and, for every active column, regenerates
+// stream l from its start up to hi, adding the connections that fall in
+// [lo, hi) in LDS: each output row belongs to one block, so the results are
+// stored once (no global atomics, no zero-fill).  Homogeneous weights count
+// the connections (u32) and store S(count) = ((0 + w) + w) + ..., what the
+// reference's `out[i_row] += weight` gives in any order; uniform and normal
+// weights are drawn per connection in the reference's order (a skipped
+// connection still consumes its draws) and summed with LDS float atomics.
+// The price is the regeneration of the rows before lo: the last tile of a
+// stream walks the whole stream.
+//
+// The active columns come as a list (k_list_events, any order); each lane
+// walks the list entries tid, tid + 1024, ... in one flattened loop (one
+// draw per iteration: a connection of the current stream, or the first
+// draw of the next entry's stream), with the next entry's column loaded
+// while the current stream runs.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_list_events(const uint8_t* __restrict__ ev, int64_t n,
+                                                      uint32_t* __restrict__ count, uint32_t* __restrict__ list) {
+  // 4 columns per thread (one 32-bit load when aligned), one counter atomic per block
+  __shared__ uint32_t s_wave[16];
+  __shared__ uint32_t s_base;
+  const int64_t c0 = ((int64_t)blockIdx.x * 1024 + threadIdx.x) * 4;
+  uint32_t on = 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (c0 + u < n && ev[c0 + u] != 0) on |= 1u << u;
+  const uint32_t cnt = (uint32_t)__popc(on);
+  // inclusive wave scan of the counts
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, 64);
+    if (lane_id() >= (uint32_t)o) incl += y;
+  }
+  const uint32_t wv = threadIdx.x >> 6;
+  if (lane_id() == 63) s_wave[wv] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int q = 0; q < 16; ++q) {
+      const uint32_t t = s_wave[q];
+      s_wave[q] = tot;
+      tot += t;
+    }
+    s_base = tot > 0 ? atomicAdd(count, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t pos = s_base + s_wave[wv] + incl - cnt;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (on & (1u << u)) list[pos++] = (uint32_t)(c0 + u);
+}
+
+__global__ void k_zero_count(uint32_t* count) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *count = 0u;
+}
+
+template <int DIST, int MK>
+__global__ __launch_bounds__(kJitThreads) void k_jit_scatter_tiles(const uint32_t* __restrict__ count,
+                                                                   const uint32_t* __restrict__ list, float w0,
+                                                                   float w1, DivU32 dv, DivShort ds, uint32_t seed,
+                                                                   int64_t num_row, uint32_t step, int K, int T,
+                                                                   float* __restrict__ out) {
+  extern __shared__ uint32_t s_acc[];            // homo: counts; else f32 sums
+  const uint32_t l = blockIdx.x / (uint32_t)K, k = blockIdx.x % (uint32_t)K;
+  const uint64_t s0 = (uint64_t)step * l;        // first row of stream l's range
+  const uint64_t lo64 = s0 + (uint64_t)k * (uint64_t)T;
+  uint64_t hi64 = lo64 + (uint64_t)T;
+  if (hi64 > s0 + step) hi64 = s0 + step;
+  if (hi64 > (uint64_t)num_row) hi64 = (uint64_t)num_row;
+  if (lo64 < hi64) {
+    const uint32_t lo = (uint32_t)lo64, hi = (uint32_t)hi64;
+    // connections reach rows below min(step*l - 1 + step, num_row)
+    const uint32_t send = (uint32_t)min((int64_t)(s0 + step - 1), num_row);
+    const uint32_t hie = min(hi, send);
+    const uint32_t r0 = (uint32_t)s0 - 1u;       // step*l - 1 (u32 wrap at l = 0)
+    for (int i = threadIdx.x; i < T; i += kJitThreads) s_acc[i] = 0u;
+    __syncthreads();
+    const uint32_t cnt = *count;
+    uint32_t e = threadIdx.x;
+    bool live = e < cnt;
+    uint32_t col = live ? list[e] : 0u;
+    uint32_t ncol = e + kJitThreads < cnt ? list[e + kJitThreads] : 0u;
+    Lfsr88 key(seed + col * 32u + l);
+    uint32_t r = r0 + mod_clen<MK>(key.next(), dv, ds) + 1u;
+    while (live) {
+      if (r < hie) {
+        float v = w0;
+        if constexpr (DIST != BPX_DIST_HOMO) v = draw_weight<DIST>(key, w0, w1);
+        if (r >= lo) {
+          if constexpr (DIST == BPX_DIST_HOMO)
+            __hip_atomic_fetch_add(s_acc + (r - lo), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          else
+            __hip_atomic_fetch_add(reinterpret_cast<float*>(s_acc) + (r - lo), v, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      } else {
+        // the next entry: its column was loaded one entry ahead
+        e += kJitThreads;
+        live = e < cnt;
+        col = ncol;
+        if (e + kJitThreads < cnt) ncol = list[e + kJitThreads];
+        key = Lfsr88(seed + col * 32u + l);
+        r = r0;
+      }
+      r += mod_clen<MK>(key.next(), dv, ds) + 1u;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; lo + i < hi; i += kJitThreads) {
+      if constexpr (DIST == BPX_DIST_HOMO) out[lo + i] = repeat_sum_f(s_acc[i], w0);
+      else out[lo + i] = __uint_as_float(s_acc[i]);
+    }
+  }
+  // rows past every stream's range (32 * step <= row < num_row) get no
+  // connection: the last block stores their zeros
+  if (blockIdx.x == gridDim.x - 1)
+    for (int64_t row = (int64_t)step * 32 + threadIdx.x; row < num_row; row += kJitThreads) out[row] = 0.f;
+}
+
+// ---------------------------------------------------------------------------
 // row layout
 // ---------------------------------------------------------------------------
 template <int DIST, int VM>
@@ -448,6 +575,9 @@ static bool bits_fit(int64_t nwords) {
   return nwords * 4 <= shm;
 }
 
+// workspace of the scatter form's event list: a count, then one u32 per column
+static size_t scatter_ws_bytes(int64_t num_col) { return 256 + (size_t)num_col * 4; }
+
 template <int DIST, int VM>
 static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t clen, uint32_t seed, int64_t num_row,
                    int64_t num_col, int outdim, int layout, float* out, void* ws, size_t ws_bytes) {
@@ -511,6 +641,30 @@ static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t c
       BPX_LAUNCH(KID_JIT_OUTDIM, (k_jit_row_outdim<DIST, VM>), dim3((unsigned)blocks), dim3(256), 0, s, vec, w0, w1,
                  dv, seed, num_row, num_col, out);
     }
+  } else if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && ws && num_col > 0 && num_row > 0 &&
+             ws_bytes >= scatter_ws_bytes(num_col) && num_row < 0xFFFFFFFFll && num_col < (1ll << 27)) {
+    // exclusive row tiles: K tiles per stream range (at least 8, so the 32
+    // ranges give ~256 blocks), each at most 32768 rows of LDS accumulators;
+    // the active columns listed first (count at ws, list at ws + 256)
+    uint32_t* cnt = (uint32_t*)ws;
+    uint32_t* list = (uint32_t*)((char*)ws + 256);
+    hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, s, cnt);
+    hipLaunchKernelGGL(k_list_events, dim3((unsigned)((num_col + 4095) / 4096)), dim3(1024), 0, s,
+                       (const uint8_t*)vec, num_col, cnt, list);
+    int64_t K = ((int64_t)step + 32767) / 32768;
+    if (K < 8) K = 8;
+    if (K > (int64_t)step) K = step;
+    const int64_t T = ((int64_t)step + K - 1) / K;
+    DivShort ds = {1u, 0u, 0u};
+    const bool shrt = DivShort::make((uint32_t)clen, &ds) && DivShort::mad24_ok((uint32_t)clen);
+#define JST(MKV)                                                                                              \
+  BPX_LAUNCH(KID_JIT_SCATTER, (k_jit_scatter_tiles<DIST, MKV>), dim3((unsigned)(32 * K)), dim3(kJitThreads),      \
+             (size_t)T * 4, s, (const uint32_t*)cnt, (const uint32_t*)list, w0, w1, dv, ds, seed, num_row, step,  \
+             (int)K, (int)T, out)
+    if (shrt) JST(MOD_SHORT24);
+    else if (dv.small_quotient()) JST(MOD_GM24);
+    else JST(MOD_GM);
+#undef JST
   } else {
     int64_t zb = (num_row + 255) / 256;
     if (zb > 4096) zb = 4096;
@@ -547,7 +701,9 @@ using namespace bpx;
 extern "C" size_t bpx_jitconn_workspace(int64_t num_row, int64_t num_col) {
   (void)num_row;
   if (num_col <= 0) return 0;
-  return (size_t)((num_col + 31) / 32) * 4;
+  const size_t bits = (size_t)((num_col + 31) / 32) * 4;
+  const size_t lst = scatter_ws_bytes(num_col);
+  return bits > lst ? bits : lst;
 }
 
 extern "C" int bpx_jitconn_mv(void* stream, const void* vec, int vec_dtype, int event, int dist, float w0, float w1,

@@ -248,9 +248,14 @@ int bpx_jitconn_mv_batch(void* stream, const void* events, int ev_dtype, int dis
  *   dist: BPX_DIST_*; (w0, w1) = weight | (low, high) | (mu, sigma).
  *   clen = ceil(1/p)*2-1; seed as u32.
  * out: f32 [num_row].
- * workspace (optional, bpx_jitconn_workspace bytes): with bool events and
- * outdim_parallel the events are packed into it as a bitmask that every
- * block keeps in LDS; NULL falls back to reading the event bytes.
+ * workspace (optional, bpx_jitconn_workspace bytes = max(num_col / 8,
+ * 256 + 4 num_col)): with bool events and outdim_parallel the events are
+ * packed into it as a bitmask that every block keeps in LDS; with bool events
+ * and outdim_parallel=0 (lane32) it holds the list of active columns, and
+ * each block owns a tile of output rows, regenerating every active column's
+ * stream up to its tile and summing in LDS (no global atomics; homogeneous
+ * weights exact).  NULL falls back to reading the event bytes (outdim) or
+ * to one FP32 atomic per connection (scatter).
  */
 size_t bpx_jitconn_workspace(int64_t num_row, int64_t num_col);
 int bpx_jitconn_mv(void* stream, const void* vec, int vec_dtype, int event,

@@ -181,3 +181,32 @@ def test_rows_kernel_nonfinite_and_overflowing_weights(dist, w0, w1):
                          np_ref.conn_len(2e-3), 11)
     assert not np.all(np.isfinite(ref))
     np.testing.assert_array_equal(y, ref)
+
+
+@pytest.mark.parametrize("dist,w0,w1", [("homo", np.inf, 0.0), ("homo", np.nan, 0.0), ("homo", -0.75, 0.0),
+                                        ("uniform", -3e38, 3e38), ("uniform", 1e38, 3.3e38),
+                                        ("uniform", -0.5, 2.0), ("normal", 0.5, 0.1)])
+@pytest.mark.parametrize("shape", [(6000, 6000), (1000, 3), (37, 900), (40000, 20000)])
+def test_event_scatter_form_vs_oracle(dist, w0, w1, shape):
+    # outdim_parallel=False with bool events (the row-tiled kernel: each row
+    # summed in one block): homogeneous weights bit-exact (any order of
+    # equal terms gives the same sum), non-finite results where the oracle
+    # has them, others within f32 summation order (normal: device log/sin)
+    ev = np.random.default_rng(shape[1]).random(shape[1]) < 0.1
+    kw = dict(shape=shape, outdim_parallel=False)
+    if dist == "homo":
+        y = bm.jitconn.event_mv_prob_homo(ev, w0, 0.05, 19, **kw).cpu().numpy()
+    elif dist == "uniform":
+        y = bm.jitconn.event_mv_prob_uniform(ev, w0, w1, 0.05, 19, **kw).cpu().numpy()
+    else:
+        y = bm.jitconn.event_mv_prob_normal(ev, w0, w1, 0.05, 19, **kw).cpu().numpy()
+    with np.errstate(all="ignore"):
+        ref = orc.jitconn_mv(ev, dist, 1, False, orc.LAYOUT_LANE32, shape[0], shape[1], np.float32(w0),
+                             np.float32(w1), np_ref.conn_len(0.05), 19)
+    if dist == "homo":
+        np.testing.assert_array_equal(y, ref)
+        return
+    for f in (np.isnan, np.isposinf, np.isneginf):
+        np.testing.assert_array_equal(f(y), f(ref))
+    fin = np.isfinite(ref)
+    np.testing.assert_allclose(y[fin], ref[fin], rtol=1e-5, atol=1e-5 * max(1.0, abs(w0), abs(w1)))
