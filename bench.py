@@ -1048,7 +1048,7 @@ def main():
                     help="run only this leg (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run 3 plain device copies of this many MiB (HBM counter calibration)")
-    ap.add_argument("--detail", default=os.path.join(ROOT, "gpurun_out", "bench_detail.json"),
+    ap.add_argument("--detail", default=os.path.join(ROOT, "bench_detail.json"),
                     help="file that receives the sweep and the other legs (the last stdout line is the headline)")
     args = ap.parse_args()
     if args.gpus != 1:
