@@ -66,6 +66,10 @@ const char* bpx_version(void);
  * events[batch, n] -> ids[batch, n] (ascending indices of truthy events,
  * then -1), num[batch].  Float events are truthy if != 0.
  */
+/* Workspace: bpx_event_info_workspace bytes (tile counts and offsets, then
+ * 512 B of event bits per 4096-event tile for bool rows starting 16-byte
+ * aligned, which are then read once instead of twice); the minimum accepted
+ * is the first part, 8 * batch * ceil(n / 4096) bytes. */
 size_t bpx_event_info_workspace(int64_t batch, int64_t n);
 int bpx_event_info(void* stream, const void* events, int ev_dtype, int64_t batch, int64_t n,
                    int32_t* ids, int32_t* num, void* workspace, size_t workspace_bytes);
@@ -75,9 +79,10 @@ int bpx_event_info(void* stream, const void* events, int ev_dtype, int64_t batch
  * counterpart in the reference: it is for callers that cannot allocate
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  The workspace lives in a static device array of
- * BPX_EVENT_INFO_SCRATCH_BYTES per GPU, held until the process ends;
- * sizes whose bpx_event_info_workspace(batch, n) exceeds it return
- * BPX_ERR_WORKSPACE (e.g. a batch of 65535 rows of 100,000 events).  Every
+ * BPX_EVENT_INFO_SCRATCH_BYTES per GPU, held until the process ends; it
+ * holds the full workspace when that fits, else the minimum, and sizes whose
+ * minimum exceeds it return BPX_ERR_WORKSPACE (e.g. a batch of 65535 rows of
+ * 100,000 events).  Every
  * call on a GPU shares that GPU's array: calls that overlap on one GPU (on
  * different streams) race.  Results equal bpx_event_info's.
  */

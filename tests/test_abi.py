@@ -61,8 +61,10 @@ def test_host_only_entry_points():
     assert b"gfx950" in lib.bpx_version()
     lib.bpx_event_info_workspace.restype = ctypes.c_size_t
     lib.bpx_event_info_workspace.argtypes = [ctypes.c_int64, ctypes.c_int64]
-    assert lib.bpx_event_info_workspace(1, 4096) == 8
-    assert lib.bpx_event_info_workspace(2, 4097) == 32
+    # tile counts and offsets (8 B per 4096-event tile and row), then the
+    # bool fast path's event bits (512 B per tile and row)
+    assert lib.bpx_event_info_workspace(1, 4096) == 8 + 512
+    assert lib.bpx_event_info_workspace(2, 4097) == 32 + 2048
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libbpx.so not built")

@@ -189,6 +189,27 @@ def test_event_info_bool_16_aligned_rows(n):
     rid1, rnum1 = orc.event_info(ev[1])
     np.testing.assert_array_equal(ids1.cpu().numpy(), rid1)
     np.testing.assert_array_equal(num1.cpu().numpy(), rnum1)
+    # the minimum workspace (tile counts and offsets only) re-reads the events
+    # instead of the stored event bits: same result
+    import ctypes
+    from brainpy_amd import _lib
+    L = _lib.lib()
+    evt = torch.from_numpy(ev).cuda()
+    B = ev.shape[0]
+    full = int(L.bpx_event_info_workspace(B, n))
+    ntiles = (n + 4095) // 4096
+    for wsb in (full, 8 * B * ntiles):
+        ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+        ids2 = torch.full((B, n), 5, dtype=torch.int32, device="cuda")
+        num2 = torch.full((B,), 5, dtype=torch.int32, device="cuda")
+        rc = L.bpx_event_info(_lib.stream_ptr(), _lib.ptr(evt), _lib.BPX_BOOL, B, n, _lib.ptr(ids2), _lib.ptr(num2),
+                              _lib.ptr(ws), wsb)
+        assert rc == 0
+        np.testing.assert_array_equal(ids2.cpu().numpy(), rid)
+        np.testing.assert_array_equal(num2.cpu().numpy(), rnum)
+    ws = torch.empty(8 * B * ntiles - 1, dtype=torch.uint8, device="cuda")
+    assert L.bpx_event_info(_lib.stream_ptr(), _lib.ptr(evt), _lib.BPX_BOOL, B, n, _lib.ptr(ids2), _lib.ptr(num2),
+                            _lib.ptr(ws), ws.numel()) == -3          # BPX_ERR_WORKSPACE
 
 
 def test_fixed_order_bit_exact_vs_serial_scatter():

@@ -41,7 +41,9 @@ def test_noalloc_equals_workspace_entry_point(dtype):
     g = torch.Generator(device=dev)
     g.manual_seed(5)
     # a large call, then a smaller one: the second must not see the first's scratch
-    for batch, n, rate in ((3, 2_000_000, 0.1), (2, 50_000, 0.3), (1, 4_096 * 7 + 5, 0.02)):
+    # (the 5e7-event row's fast-path bits exceed the scratch: the no-allocation
+    # call takes the minimum-workspace path, the workspace call the fast path)
+    for batch, n, rate in ((3, 2_000_000, 0.1), (1, 50_000_000, 0.01), (2, 50_000, 0.3), (1, 4_096 * 7 + 5, 0.02)):
         ev = (torch.rand((batch, n), generator=g, device=dev) < rate).to(dtype)
         ids_a = torch.full((batch, n), 7, dtype=torch.int32, device=dev)
         num_a = torch.full((batch,), 7, dtype=torch.int32, device=dev)
