@@ -342,6 +342,23 @@ def bench_forms(_lib, torch, steps=30, warmup=3):
     print(f"# bm.sparse.csrmv transpose=1 (public path, cached plan) {r['us_per_call']:8.1f} us/call kernel "
           f"{r['kernel_us']:8.1f} us {r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%) {r['synapses_per_s']:.3e} "
           f"synapses/s", file=sys.stderr, flush=True)
+    # the public sparse gather (transpose=False) on the same cached plan:
+    # each column tile of the vector staged in LDS
+    for name, vec, kw in (("bm.sparse.csrmv transpose=0 (public path, cached plan)", xs, {}),):
+        fn = bm.sparse.csrmv
+        for _ in range(2):
+            fn(w, idx, ptr, vec, shape=(n, n), **kw)
+
+        def gcall():
+            fn(w, idx, ptr, vec, shape=(n, n), **kw)
+        wall, kern = _time_calls(_lib, torch, gcall, _lib.KID["tiled_gather"], steps, warmup)
+        alg = nnz * 8 + (n + 1) * 4 + n * 4 + n * (1 if vec.dtype == torch.bool else 4)
+        r = {"op": name, "transpose": False, "nnz": nnz, "us_per_call": wall * 1e6, "kernel_us": kern * 1e6,
+             "alg_bytes": alg, "gbs": alg / kern / 1e9, "frac": alg / kern / 1e9 / HBM_PEAK_GBS,
+             "synapses_per_s": nnz / wall}
+        res.append(r)
+        print(f"# {name} {r['us_per_call']:8.1f} us/call kernel {r['kernel_us']:8.1f} us {r['gbs']:7.0f} GB/s "
+              f"({100 * r['frac']:.1f}%) {r['synapses_per_s']:.3e} synapses/s", file=sys.stderr, flush=True)
     bm.event.clear_plan_cache()
     del xs
     probe_out = torch.empty(32768 * 256, dtype=torch.float32, device=dev)

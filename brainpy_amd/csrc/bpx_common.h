@@ -95,6 +95,7 @@ enum KernelId {
   KID_COO = 12,
   KID_BSR = 13,
   KID_NET_RUN = 14,
+  KID_TILED_GATHER = 15,
   KID_COUNT = 16
 };
 

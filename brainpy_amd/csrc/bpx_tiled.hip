@@ -491,6 +491,133 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
   }
 }
 
+// ---------------------------------------------------------------------------
+// gather on the plan: y[m] = A @ v (bm.sparse.csrmv / bm.event.csrmv with
+// transpose=False).  Block (row chunk, tile p) stages v's tile p (T values,
+// or the events as 0/1 by the rule) in LDS, and groups of 8 lanes walk the
+// tile-p segment of each row of the chunk (4 tile-local indices and weights
+// per lane and pass), each lane summing w * v[c] in element order, the 8
+// lanes reduced by a fixed tree; the row's partial of tile p goes to
+// partial[p][row], and k_tiled_gather_sum adds the P partials of a row in
+// tile order.  The matrix is read once at 2 + w bytes per synapse and every
+// v read is an LDS read; the result is deterministic.  Blocks of one row
+// chunk take consecutive indices (p fastest), so the tiles read the
+// segments of the same rows at about the same time.
+// XM: 0 dense f32 v, 1 bool events, 2 f32 events (ev_rule)
+// ---------------------------------------------------------------------------
+constexpr int kGatherRows = 256;    // rows per block
+
+template <int WK, int XM>
+__global__ __launch_bounds__(256) void k_tiled_gather(const int32_t* __restrict__ rowtile,
+                                                      const uint16_t* __restrict__ lidx,
+                                                      const void* __restrict__ wptr, const float* __restrict__ homo_w,
+                                                      const void* __restrict__ v, int rule, int64_t m, int64_t n,
+                                                      int P, int T, float* __restrict__ partial) {
+  extern __shared__ float s_v[];
+  const int p = (int)(blockIdx.x % (unsigned)P);
+  const int64_t chunk = blockIdx.x / (unsigned)P;
+  const int64_t t0 = (int64_t)p * T;
+  const int tlen = (int)((n - t0) < (int64_t)T ? (n - t0) : (int64_t)T);
+  for (int k = threadIdx.x; k < tlen; k += 256) {
+    float x;
+    if constexpr (XM == 0) x = ((const float*)v)[t0 + k];
+    else if constexpr (XM == 1) x = ((const uint8_t*)v)[t0 + k] ? 1.f : 0.f;
+    else x = ev_true(((const float*)v)[t0 + k], rule) ? 1.f : 0.f;
+    s_v[k] = x;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int lv = lane & 7;                       // lane within its group of 8
+  const int gbase = lane & ~7;                   // the group's first lane
+  const int grp = threadIdx.x >> 3;              // 32 groups per block
+  const int64_t r0 = chunk * kGatherRows;
+  const int64_t r_end = min(r0 + kGatherRows, m);
+  float hw = 0.f;
+  if constexpr (WK == 0) hw = *homo_w;
+  // the group's 8 rows r0 + grp + 32k (k = 0..7): lane k loads row k's
+  // segment bounds, so all 8 are one memory round trip
+  int32_t sb = 0, se = 0;
+  {
+    const int64_t row = r0 + grp + 32 * lv;
+    if (row < r_end) {
+      const int32_t* rt = rowtile + row * (P + 1) + p;
+      sb = rt[0];
+      se = rt[1];
+    }
+  }
+  auto term = [&](uint32_t c, float wv, float& acc) {
+    // padding entries carry 0xFFFF, past every tile
+    if (c < (uint32_t)tlen) {
+      if constexpr (WK == 0) acc += s_v[c];
+      else acc += wv * s_v[c];
+    }
+  };
+  auto load_w = [&](int32_t j, float (&w4)[4]) {
+    if constexpr (WK == 1) {
+      const float4 f = *reinterpret_cast<const float4*>((const float*)wptr + j);
+      w4[0] = f.x; w4[1] = f.y; w4[2] = f.z; w4[3] = f.w;
+    } else if constexpr (WK == 2) {
+      const uint2 h = *reinterpret_cast<const uint2*>((const bf16_t*)wptr + j);
+      w4[0] = __uint_as_float(h.x << 16); w4[1] = __uint_as_float(h.x & 0xFFFF0000u);
+      w4[2] = __uint_as_float(h.y << 16); w4[3] = __uint_as_float(h.y & 0xFFFF0000u);
+    } else {
+      w4[0] = w4[1] = w4[2] = w4[3] = 1.f;
+    }
+  };
+  constexpr int RB = 4;                          // rows whose first pass is in flight together
+#pragma unroll
+  for (int k0 = 0; k0 < 8; k0 += RB) {
+    int32_t s[RB], e[RB];
+    uint2 a[RB];
+    float w4[RB][4];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+      s[q] = __shfl(sb, gbase + k0 + q, 64);
+      e[q] = __shfl(se, gbase + k0 + q, 64);
+      const int32_t j = s[q] + 4 * lv;
+      const bool ok = j < e[q];
+      const int32_t js = ok ? j : 0;
+      a[q] = *reinterpret_cast<const uint2*>(lidx + js);
+      load_w(js, w4[q]);
+      if (!ok) a[q] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+    }
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+      const int64_t row = r0 + grp + 32 * (k0 + q);
+      float acc = 0.f;
+      term(a[q].x & 0xFFFFu, w4[q][0], acc);
+      term(a[q].x >> 16, w4[q][1], acc);
+      term(a[q].y & 0xFFFFu, w4[q][2], acc);
+      term(a[q].y >> 16, w4[q][3], acc);
+      // the rest of a segment longer than 32 entries
+      for (int32_t j = s[q] + 32 + 4 * lv; j < e[q]; j += 32) {
+        const uint2 a2 = *reinterpret_cast<const uint2*>(lidx + j);
+        float v4[4];
+        load_w(j, v4);
+        term(a2.x & 0xFFFFu, v4[0], acc);
+        term(a2.x >> 16, v4[1], acc);
+        term(a2.y & 0xFFFFu, v4[2], acc);
+        term(a2.y >> 16, v4[3], acc);
+      }
+      // fixed reduction tree over the group's 8 lanes
+      acc += __shfl_xor(acc, 1, 64);
+      acc += __shfl_xor(acc, 2, 64);
+      acc += __shfl_xor(acc, 4, 64);
+      if constexpr (WK == 0) acc = hw * acc;
+      if (lv == 0 && row < r_end) partial[(int64_t)p * m + row] = acc;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_tiled_gather_sum(const float* __restrict__ partial, int64_t m, int P,
+                                                          float* __restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= m) return;
+  float acc = 0.f;
+  for (int p = 0; p < P; ++p) acc += partial[(int64_t)p * m + r];
+  out[r] = acc;
+}
+
 template <typename O>
 __global__ __launch_bounds__(256) void k_fill_zero_t(O* __restrict__ out, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -785,6 +912,53 @@ int bpx_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* x, int 
   else TLDR(2);
 #undef TLDR
 #undef TLD
+  BPX_CHECK_LAUNCH();
+  return BPX_OK;
+}
+
+size_t bpx_tiled_csr_gather_workspace(const bpx_tiled_csr* plan) {
+  if (!plan || plan->m <= 0) return 0;
+  return (size_t)plan->tiles * (size_t)plan->m * sizeof(float);
+}
+
+int bpx_tiled_csr_gather(void* stream, const bpx_tiled_csr* plan, const void* vec, int vec_dtype, int event,
+                         int ev_rule, float* out, void* workspace, size_t workspace_bytes) {
+  if (!plan || !plan->mem) return BPX_ERR_ARG;
+  if (event ? (vec_dtype != BPX_BOOL && vec_dtype != BPX_F32) : vec_dtype != BPX_F32) return BPX_ERR_DTYPE;
+  if (event && ev_rule != BPX_RULE_POSITIVE && ev_rule != BPX_RULE_NONZERO) return BPX_ERR_ARG;
+  const int64_t m = plan->m, n = plan->n;
+  if (m == 0) return BPX_OK;
+  if (!out || (n > 0 && !vec)) return BPX_ERR_ARG;
+  if (!workspace || workspace_bytes < bpx_tiled_csr_gather_workspace(plan)) return BPX_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  const char* base = (const char*)plan->mem;
+  const int32_t* rowtile = (const int32_t*)(base + plan->rowtile_offset);
+  const uint16_t* lidx = (const uint16_t*)(base + plan->lidx_offset);
+  const void* w = base + plan->weights_offset;
+  const float* hw = (const float*)(base + plan->homo_offset);
+  const int P = plan->tiles, T = plan->tile_len;
+  float* partial = (float*)workspace;
+  const int64_t chunks = (m + kGatherRows - 1) / kGatherRows;
+  const int64_t blocks = chunks * P;
+  if (blocks > 0x7FFFFFFFll) return BPX_ERR_SHAPE;
+  const size_t shm = (size_t)T * sizeof(float);
+  const int xm = event ? (vec_dtype == BPX_BOOL ? 1 : 2) : 0;
+#define TG(WK, XM)                                                                                          \
+  BPX_LAUNCH(KID_TILED_GATHER, (k_tiled_gather<WK, XM>), dim3((unsigned)blocks), dim3(256), shm, s, rowtile, lidx, w, \
+             hw, vec, ev_rule, m, n, P, T, partial)
+#define TG_X(WK)                   \
+  do {                             \
+    if (xm == 0) TG(WK, 0);        \
+    else if (xm == 1) TG(WK, 1);   \
+    else TG(WK, 2);                \
+  } while (0)
+  if (plan->weight_dtype == BPX_TILED_HOMO) TG_X(0);
+  else if (plan->weight_dtype == BPX_F32) TG_X(1);
+  else TG_X(2);
+#undef TG_X
+#undef TG
+  hipLaunchKernelGGL(k_tiled_gather_sum, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, (const float*)partial, m,
+                     P, out);
   BPX_CHECK_LAUNCH();
   return BPX_OK;
 }

@@ -46,13 +46,14 @@ def _launch(d, idx, ptr, v, shape, transpose):
         raise ValueError(f'Shape mismatch, vector ({v.shape[0]},) with matrix {shape}, transpose={transpose}.')
     if ptr.shape[0] != m + 1:
         raise ValueError(f'indptr should have shape ({m + 1},), but we got {tuple(ptr.shape)}.')
-    if transpose and v.dtype == torch.float32:
+    if v.dtype == torch.float32:
         # repeated calls on the same arrays run on the cached tiled plan
-        # (shared with bm.event.csrmv): products summed in LDS, no global
-        # atomic per synapse
+        # (shared with bm.event.csrmv): the transpose sums its products in
+        # LDS with no global atomic per synapse, the gather reads each column
+        # tile of v from LDS instead of one random gather per synapse
         plan = _PLANS.lookup(d, idx, ptr, m, n, v, None)
         if plan is not None:
-            return plan.matvec(v)
+            return plan.matvec(v) if transpose else plan._gather(v, False, _lib.RULE_POSITIVE)
     out = torch.empty(n if transpose else m, dtype=acc_dtype(d.dtype), device=v.device)
     rc = _lib.lib().bpx_csrmv(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), d.shape[0], _lib.ptr(idx),
                               _lib.ptr(ptr), _lib.bpx_dtype(idx), idx.shape[0], _lib.ptr(v), _lib.bpx_dtype(v), m, n,

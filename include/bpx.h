@@ -215,6 +215,22 @@ int bpx_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* x, int 
                     void* workspace, size_t workspace_bytes);
 
 /*
+ * The same plan read row-wise: out[m] = A @ vec, the transpose=0 gather of
+ * bm.sparse.csrmv (dense f32 vec; sparse/_csr_mv.py:29-123) and of
+ * bm.event.csrmv (event=1: bool events, or f32 events with ev_rule; the
+ * weights of rows' truthy columns summed; event/_csr_matvec.py:38-110).
+ * Each block stages one column tile of vec in LDS and sums the tile's
+ * segment of each of its rows (w * vec[c] in element order, then a fixed
+ * 8-lane tree); the per-tile row partials (workspace of
+ * bpx_tiled_csr_gather_workspace bytes = 4 * tiles * m) are added in tile
+ * order: deterministic, no atomics, the matrix read once at 2 + w bytes per
+ * synapse.  No reference counterpart as a plan (the reference re-reads CSR).
+ */
+size_t bpx_tiled_csr_gather_workspace(const bpx_tiled_csr* plan);
+int bpx_tiled_csr_gather(void* stream, const bpx_tiled_csr* plan, const void* vec, int vec_dtype, int event,
+                         int ev_rule, float* out, void* workspace, size_t workspace_bytes);
+
+/*
  * Batched event operators: `batch` event vectors in one launch, in place of
  * the reference's batching rule for these primitives, a sequential scan over
  * the batch (op_register/utils.py:16-37; event/_csr_matvec.py:549-556,

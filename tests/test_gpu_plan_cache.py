@@ -186,3 +186,77 @@ def test_sparse_transpose_nonfinite_on_the_plan_equals_stateless():
     fin = np.isfinite(y0)
     assert (~fin).any() and fin.any()
     np.testing.assert_allclose(y[fin], y0[fin], rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("wkind", ["f32", "bf16", "homo"])
+@pytest.mark.parametrize("shape", [(2000, 3000), (777, 5000), (3000, 150000)])
+def test_plan_gather_dense_and_events(wkind, shape):
+    # A @ v on the plan (transpose=False): dense f32 v within 1e-5 of each
+    # row's sum of |w v| against FP64; events (bool, f32 by both rules)
+    # summing the truthy columns' weights; calls are deterministic
+    m, n = shape
+    idx, ptr = fixed_prob_csr(m, n, min(0.02, 60.0 / n), 21)
+    rng = np.random.default_rng(4)
+    w = np.float32(0.7) if wkind == "homo" else rng.normal(0.5, 0.3, idx.size).astype(np.float32)
+    plan = bm.event.TiledCSR(w, idx, ptr, shape=shape, weight_dtype=torch.bfloat16 if wkind == "bf16" else None)
+    if wkind == "bf16":
+        w = torch.from_numpy(w).to(torch.bfloat16).float().numpy()
+    D = dense_of(w, idx, ptr, shape).astype(np.float64)
+    x = rng.standard_normal(n).astype(np.float32)
+    x[rng.random(n) < 0.3] = 0.0
+    y = plan.gather(x)
+    assert torch.equal(y, plan.gather(x))
+    ref, mag = D @ x.astype(np.float64), np.abs(D) @ np.abs(x.astype(np.float64))
+    assert np.all(np.abs(y.cpu().numpy() - ref) <= 1e-5 * mag + 1e-30)
+    evb = bernoulli(n, 0.1, 5)
+    ye = plan.gather(evb, event=True).cpu().numpy()
+    on = evb.astype(np.float64)
+    assert np.all(np.abs(ye - D @ on) <= 1e-5 * (np.abs(D) @ on) + 1e-30)
+    evf = np.where(evb, rng.choice([1.5, -2.0], n), 0.0).astype(np.float32)
+    for rule, truthy in ((bm.event._lib.RULE_POSITIVE, evf > 0), (bm.event._lib.RULE_NONZERO, evf != 0)):
+        yf = plan.gather(evf, event=True, rule=rule).cpu().numpy()
+        on = truthy.astype(np.float64)
+        assert np.all(np.abs(yf - D @ on) <= 1e-5 * (np.abs(D) @ on) + 1e-30)
+
+
+def test_public_gather_paths_run_on_the_plan():
+    # bm.sparse.csrmv with transpose=False: the second call on the same
+    # arrays builds the plan, later calls hit it; bm.event.csrmv's gather
+    # keeps its CSR kernel (no plan), exact integer counts either way
+    idx, ptr, w = _inputs(seed=13)
+    ti, tp = torch.from_numpy(idx).to(DEV), torch.from_numpy(ptr).to(DEV)
+    tw = torch.from_numpy(w).to(DEV)
+    D = dense_of(w, idx, ptr, (2000, 3000)).astype(np.float64)
+    for k in range(4):
+        x = np.random.default_rng(k).standard_normal(3000).astype(np.float32)
+        y = bm.sparse.csrmv(tw, ti, tp, torch.from_numpy(x).to(DEV), shape=(2000, 3000))
+        assert np.all(np.abs(y.cpu().numpy() - D @ x) <= 1e-5 * (np.abs(D) @ np.abs(x)) + 1e-30)
+    info = bm.event.plan_cache_info()
+    assert info["builds"] == 1 and info["hits"] == 2
+    ev = bernoulli(3000, 0.2, 9)
+    ref = orc.event_csrmv(np.float32(1.0), idx, ptr, ev, (2000, 3000), False)
+    for _ in range(3):
+        y = bm.event.csrmv(1.0, ti, tp, torch.from_numpy(ev).to(DEV), shape=(2000, 3000))
+        np.testing.assert_array_equal(y.cpu().numpy(), ref)
+    assert bm.event.plan_cache_info()["builds"] == 1
+    # the plan's event gather (TiledCSR.gather) gives the same integers
+    plan = bm.event.TiledCSR(1.0, ti, tp, shape=(2000, 3000))
+    np.testing.assert_array_equal(plan.gather(ev, event=True).cpu().numpy(), ref)
+
+
+def test_plan_gather_nonfinite_vector_matches_stateless():
+    idx, ptr, w = _inputs(seed=14)
+    t = [torch.from_numpy(a).to(DEV) for a in (w, idx, ptr)]
+    x = np.random.default_rng(2).standard_normal(3000).astype(np.float32)
+    x[3], x[40], x[77] = np.inf, -np.inf, np.nan
+    xt = torch.from_numpy(x).to(DEV)
+    bm.event.set_plan_cache(enabled=False)
+    y0 = bm.sparse.csrmv(*t, xt, shape=(2000, 3000)).cpu().numpy()
+    bm.event.set_plan_cache(enabled=True)
+    for _ in range(3):
+        y = bm.sparse.csrmv(*t, xt, shape=(2000, 3000)).cpu().numpy()
+    for f in (np.isnan, np.isposinf, np.isneginf):
+        np.testing.assert_array_equal(f(y), f(y0))
+    fin = np.isfinite(y0)
+    assert (~fin).any() and fin.any()
+    np.testing.assert_allclose(y[fin], y0[fin], rtol=1e-5, atol=1e-5)
