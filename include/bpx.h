@@ -216,9 +216,11 @@ int bpx_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* x, int 
 
 /*
  * The same plan read row-wise: out[m] = A @ vec, the transpose=0 gather of
- * bm.sparse.csrmv (dense f32 vec; sparse/_csr_mv.py:29-123) and of
- * bm.event.csrmv (event=1: bool events, or f32 events with ev_rule; the
- * weights of rows' truthy columns summed; event/_csr_matvec.py:38-110).
+ * bm.sparse.csrmv (dense f32 vec; sparse/_csr_mv.py:29-123), and with
+ * event=1 the event gather of bm.event.csrmv (bool events, or f32 events
+ * with ev_rule; the weights of rows' truthy columns summed;
+ * event/_csr_matvec.py:38-110) -- for events bpx_event_csrmv's bitmask
+ * kernel reads whole rows and is the faster of the two.
  * Each block stages one column tile of vec in LDS and sums the tile's
  * segment of each of its rows (w * vec[c] in element order, then a fixed
  * 8-lane tree); the per-tile row partials (workspace of
