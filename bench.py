@@ -407,6 +407,75 @@ def bench_forms(_lib, torch, steps=30, warmup=3):
     return {"workload": "csrmv_forms_100kx100k_p0.01_heter_f32 / event_info", "results": res, "info": info}
 
 
+def bench_bign(_lib, torch, steps=30, warmup=3):
+    """The transpose event csrmv beyond 262,144 outputs, where the plan has
+    more than 64 output tiles (fewer output copies per call): the public
+    bm.event.csrmv(transpose=True) on its cached plan against the stateless
+    scatter on the same arrays, heterogeneous f32 weights, bool events.
+    Workloads: 100,000 x 1,000,000 with 1000 synapses per row (nnz 1e8) at
+    10 % events, and 1,000,000 x 1,000,000 with 100 per row (nnz 1e8) at
+    2 %.  Algorithmic bytes as the headline's: m + 8 per active row + 8 per
+    synaptic event + 4n."""
+    from brainpy_amd.connect import fixed_prob_csr_device
+    import brainpy_amd.math as bm
+    dev = _lib.device()
+    L = _lib.lib()
+    sp = _lib.stream_ptr()
+    res = []
+    for m, n, k, rate in ((100_000, 1_000_000, 1000, 0.1), (1_000_000, 1_000_000, 100, 0.02)):
+        idx, ptr = fixed_prob_csr_device(m, n, (k + 0.5) / n, seed=77, device=dev)
+        nnz = idx.numel()
+        g = torch.Generator(device=dev)
+        g.manual_seed(13)
+        w = torch.empty(nnz, dtype=torch.float32, device=dev).normal_(0.5, 0.1, generator=g)
+        ev = torch.rand(m, generator=g, device=dev) < rate
+        active = int(ev.sum())
+        out = torch.empty(n, dtype=torch.float32, device=dev)
+        alg = m + 8 * active + 8 * active * k + 4 * n
+        bm.event.clear_plan_cache()
+        bm.event.csrmv(w, idx, ptr, ev, shape=(m, n), transpose=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bm.event.csrmv(w, idx, ptr, ev, shape=(m, n), transpose=True)      # builds the plan
+        torch.cuda.synchronize()
+        build_ms = (time.perf_counter() - t0) * 1e3
+        info = bm.event.plan_cache_info()
+        if info["plans"] != 1:
+            raise RuntimeError(f"no plan for {m} x {n}: {info}")
+        hits0 = info["hits"]
+
+        def pcall():
+            bm.event.csrmv(w, idx, ptr, ev, shape=(m, n), transpose=True)
+        wall, kern = _time_calls(_lib, torch, pcall, _lib.KID["tiled"], steps, warmup)
+        if bm.event.plan_cache_info()["hits"] - hits0 < steps:
+            raise RuntimeError("bm.event.csrmv(transpose=True) did not run on its cached plan")
+        y_plan = bm.event.csrmv(w, idx, ptr, ev, shape=(m, n), transpose=True).clone()
+
+        def scall():
+            rc = L.bpx_event_csrmv(sp, _lib.ptr(w), _lib.BPX_F32, nnz, _lib.ptr(idx), _lib.ptr(ptr), _lib.BPX_I32,
+                                   nnz, _lib.ptr(ev), _lib.BPX_BOOL, _lib.RULE_POSITIVE, m, n, 1, _lib.ptr(out))
+            if rc != 0:
+                raise RuntimeError(f"bpx_event_csrmv failed ({rc})")
+        swall, skern = _time_calls(_lib, torch, scall, _lib.KID["event_scatter"], steps, warmup)
+        diff = float((y_plan - out).abs().max())
+        plan = next(iter(bm.event._PLANS._plans.values()))[0]
+        r = {"m": m, "n": n, "nnz": nnz, "rate": rate, "events": active * k, "tiles": plan.plan.tiles,
+             "tile_len": plan.plan.tile_len, "plan_build_ms": build_ms, "plan_bytes": plan.mem.numel(),
+             "us_per_call": wall * 1e6, "kernel_us": kern * 1e6, "alg_bytes": alg, "gbs": alg / kern / 1e9,
+             "frac": alg / kern / 1e9 / HBM_PEAK_GBS, "stateless_us_per_call": swall * 1e6,
+             "stateless_kernel_us": skern * 1e6, "speedup_kernel": skern / kern, "max_abs_diff_vs_stateless": diff}
+        res.append(r)
+        print(f"# event csrmv transpose {m}x{n} k={k} rate={rate}: plan ({r['tiles']} tiles of {r['tile_len']}, "
+              f"built in {build_ms:.1f} ms, {r['plan_bytes'] / 2**20:.0f} MiB) {r['us_per_call']:.1f} us/call, kernel "
+              f"{r['kernel_us']:.1f} us = {r['gbs']:.0f} GB/s ({100 * r['frac']:.1f}%); stateless kernel "
+              f"{r['stateless_kernel_us']:.1f} us ({r['speedup_kernel']:.1f}x); max|diff| {diff:.2e}",
+              file=sys.stderr, flush=True)
+        bm.event.clear_plan_cache()
+        del idx, ptr, w, ev, out, y_plan, plan
+        torch.cuda.empty_cache()
+    return {"workload": "event_csrmv_transpose_large_n", "results": res}
+
+
 def bench_conn(_lib, torch, reps=5):
     """FixedProb.build_csr of the headline matrix (100,000 x 100,000,
     p = 0.01, 1e8 targets): the device builder (bpx_fixed_prob_csr: uniform
@@ -1060,7 +1129,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the jitconn and network legs")
     ap.add_argument("--only", choices=["jitconn", "batched", "coba", "smallbatch", "served", "coo", "forms", "conn", "stdp",
-                                       "bsr", "batchops"],
+                                       "bsr", "batchops", "bign"],
                     default=None,
                     help="run only this leg (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -1087,7 +1156,8 @@ def main():
               "served": lambda: bench_served(_lib, torch),
               "coo": lambda: bench_coo(_lib, torch), "forms": lambda: bench_forms(_lib, torch),
               "conn": lambda: bench_conn(_lib, torch), "stdp": lambda: bench_stdp(_lib, torch),
-              "bsr": lambda: bench_bsr(_lib, torch), "batchops": lambda: bench_batch_ops(_lib, torch)}[args.only]
+              "bsr": lambda: bench_bsr(_lib, torch), "batchops": lambda: bench_batch_ops(_lib, torch),
+              "bign": lambda: bench_bign(_lib, torch)}[args.only]
         print(json.dumps({"only": args.only, "result": fn()}), flush=True)
         return
     sp = _lib.stream_ptr()
@@ -1289,6 +1359,7 @@ def main():
         extra["batch_ops"] = bench_batch_ops(_lib, torch)
         extra["coo"] = bench_coo(_lib, torch)
         extra["forms"] = bench_forms(_lib, torch)
+        extra["bign"] = bench_bign(_lib, torch)
         extra["conn"] = bench_conn(_lib, torch)
         extra["stdp"] = bench_stdp(_lib, torch)
         extra["bsr"] = bench_bsr(_lib, torch)

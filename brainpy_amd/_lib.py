@@ -79,6 +79,9 @@ SIGNATURES = {
     "bpx_tiled_csr_gather_workspace": ([ctypes.POINTER(TiledCSRPlan)], _sz),
     "bpx_tiled_csr_gather": ([_vp, ctypes.POINTER(TiledCSRPlan), _vp, _i32, _i32, _i32, _vp, _vp, _sz], _i32),
     "bpx_event_csrmv_tiled_batch": ([_vp, ctypes.POINTER(TiledCSRPlan), _vp, _i32, _i32, _i64, _i32, _vp], _i32),
+    "bpx_tiled_csr_call_workspace": ([ctypes.POINTER(TiledCSRPlan), _i64], _sz),
+    "bpx_event_csrmv_tiled_ws": ([_vp, ctypes.POINTER(TiledCSRPlan), _vp, _i32, _i32, _i64, _i32, _vp, _vp, _sz],
+                                 _i32),
     "bpx_batch_workspace": ([_i64], _sz),
     "bpx_event_csrmv_batch": ([_vp, _vp, _i32, _i64, _vp, _vp, _i32, _i64, _vp, _i32, _i32, _i64, _i64, _i64, _i32,
                                _vp, _vp, _sz], _i32),
@@ -214,4 +217,4 @@ def timer_read(kernel_id):
 
 
 KID = dict(event_scatter=0, event_gather=1, csr_scatter=2, csr_gather=3, jit_outdim=4, jit_scatter=5, info=6,
-           lif=7, fixed_order=8, net=10, tiled=11, coo=12, bsr=13, net_run=14, tiled_gather=15)
+           lif=7, fixed_order=8, tiled_list=9, net=10, tiled=11, coo=12, bsr=13, net_run=14, tiled_gather=15)

@@ -89,7 +89,7 @@ enum KernelId {
   KID_INFO = 6,
   KID_LIF = 7,
   KID_FIXED_ORDER = 8,
-  KID_UNUSED_9 = 9,
+  KID_TILED_LIST = 9,
   KID_NET = 10,
   KID_TILED = 11,
   KID_COO = 12,

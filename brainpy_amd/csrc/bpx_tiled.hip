@@ -35,7 +35,8 @@ namespace bpx {
 
 constexpr int kTiledThreads = 1024;
 constexpr int kBuildThreads = 128;
-constexpr int kMaxTiles = 64;
+constexpr int kMaxTiles = 64;        // tiles whose build counts fit the LDS of a build thread
+constexpr int kMaxTilesBig = 1024;    // tiles built through global cursors (n <= 1024 * 4096)
 
 static int device_cus() {
   int dev = 0, cus = 0;
@@ -171,6 +172,121 @@ __global__ __launch_bounds__(kBuildThreads) void k_tiled_fill(const V* __restric
   }
 }
 
+// More than kMaxTiles tiles: a block of tb threads (one row each) keeps
+// every thread's P counters in dynamic LDS (tb * P * 4 <= 64 KiB), writes
+// the counts into the rowtile row, k_tiled_offsets_big turns a row's counts
+// into segment starts (one wave per row, coalesced), and k_tiled_fill_big
+// places the elements in order from LDS cursors.  Padding slots were filled
+// with 0xFFFF beforehand (k_tiled_prefill).
+constexpr int kBigBuildLds = 65536;
+
+static int big_build_threads(int P) {
+  int tb = kBigBuildLds / (4 * P);
+  return tb > 64 ? 64 : (tb < 1 ? 1 : tb);
+}
+
+template <typename I>
+__global__ __launch_bounds__(64) void k_tiled_count_big(const I* __restrict__ indices, const I* __restrict__ indptr,
+                                                        int64_t nnz, int64_t m, int64_t n, int P, DivU32 dT,
+                                                        int32_t* __restrict__ rowtile,
+                                                        int32_t* __restrict__ rowsize) {
+  extern __shared__ int32_t s_big[];
+  int32_t* cnt = s_big + threadIdx.x * P;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  for (int p = 0; p < P; ++p) cnt[p] = 0;
+  int64_t s = (int64_t)indptr[r], t = (int64_t)indptr[r + 1];
+  s = s < 0 ? 0 : s;
+  t = t > nnz ? nnz : t;
+  for (int64_t j = s; j < t; ++j) {
+    const int64_t c = (int64_t)indices[j];
+    if ((uint64_t)c < (uint64_t)n) cnt[dT.div((uint32_t)c)] += 1;
+  }
+  int32_t* rt = rowtile + r * (P + 1);
+  int32_t tot = 0;
+  for (int p = 0; p < P; ++p) {
+    rt[p] = cnt[p];
+    tot += (cnt[p] + kVec - 1) & ~(kVec - 1);
+  }
+  rowsize[r] = tot;
+}
+
+// one wave per row: rt[p] = the start of segment p (counts padded to kVec),
+// rt[P] = the row's end
+__global__ __launch_bounds__(256) void k_tiled_offsets_big(const int32_t* __restrict__ rowbase, int64_t m, int P,
+                                                           int32_t* __restrict__ rowtile) {
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= m) return;
+  const int lane = threadIdx.x & 63;
+  int32_t* rt = rowtile + r * (P + 1);
+  int32_t carry = rowbase[r];
+  for (int p0 = 0; p0 < P; p0 += 64) {
+    const int p = p0 + lane;
+    const int32_t c = p < P ? rt[p] : 0;
+    const int32_t sz = (c + kVec - 1) & ~(kVec - 1);
+    int32_t incl = sz;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    if (p < P) rt[p] = carry + incl - sz;
+    carry += __shfl(incl, 63, 64);
+  }
+  if (lane == 0) rt[P] = carry;
+}
+
+// every slot below the plan's total (rowsize[m], read on the device) holds
+// the padding index and a zero weight
+template <typename W>
+__global__ __launch_bounds__(256) void k_tiled_prefill(const int32_t* __restrict__ total, int heter,
+                                                       uint16_t* __restrict__ lidx, W* __restrict__ w) {
+  const int64_t len = *total;
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (; i < len; i += stride) {
+    lidx[i] = 0xFFFF;
+    if (heter) {
+      if constexpr (sizeof(W) == 2) w[i].bits = 0;
+      else w[i] = 0.f;
+    }
+  }
+}
+
+template <typename I, typename V, typename W>
+__global__ __launch_bounds__(64) void k_tiled_fill_big(const V* __restrict__ data, int heter,
+                                                       const I* __restrict__ indices, const I* __restrict__ indptr,
+                                                       int64_t nnz, int64_t m, int64_t n, int P, DivU32 dT,
+                                                       const int32_t* __restrict__ rowtile,
+                                                       uint16_t* __restrict__ lidx, W* __restrict__ w) {
+  extern __shared__ int32_t s_big[];
+  int32_t* cur = s_big + threadIdx.x * P;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  const int32_t* rt = rowtile + r * (P + 1);
+  for (int p = 0; p < P; ++p) cur[p] = rt[p];
+  int64_t s = (int64_t)indptr[r], t = (int64_t)indptr[r + 1];
+  s = s < 0 ? 0 : s;
+  t = t > nnz ? nnz : t;
+  for (int64_t j = s; j < t; ++j) {
+    const int64_t c = (int64_t)indices[j];
+    if ((uint64_t)c >= (uint64_t)n) continue;
+    const uint32_t p = dT.div((uint32_t)c);
+    const int32_t pos = cur[p]++;
+    lidx[pos] = (uint16_t)((uint32_t)c - p * dT.d);
+    if (heter) {
+      const float v = to_f32(data[j]);
+      if constexpr (sizeof(W) == 2) {
+        uint32_t u = __float_as_uint(v);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        w[pos].bits = (uint16_t)(u >> 16);
+      } else {
+        w[pos] = v;
+      }
+    }
+  }
+}
+
 // out[0] = max |v| (NaN propagates, or with FINITE_ONLY the max over finite
 // values), out[1] = min |v| over the finite non-zero values (+inf if none);
 // out must hold {0, +inf} before (k_init_minmax)
@@ -264,15 +380,22 @@ __device__ __forceinline__ long long fixed_of(float w, double scale) {
 // min non-zero |w|} and `xmax` at x's {max finite |x|, min non-zero |x|}.
 // COARSE (events; the plan's flag): every term goes to out with an FP32
 // atomic; DENSE decides the same per call on the device.
-template <int WK, typename E, int R, bool DENSE, bool COARSE>
+// LISTED: the active rows were listed once for the call (k_tiled_list:
+// lcount[v] rows of vector v in lrows[v][...], any order), and the waves walk
+// that list instead of scanning every event of their copy's rows: with many
+// tiles per copy each row's event would otherwise be read by every tile
+// block.  Sums are the same (fixed point is exact in any order).
+template <int WK, typename E, int R, bool DENSE, bool COARSE, bool LISTED = false>
 __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* __restrict__ rowtile,
                                                                  const uint16_t* __restrict__ lidx,
                                                                  const void* __restrict__ wptr,
                                                                  const float* __restrict__ homo_w,
                                                                  const E* __restrict__ ev, int rule, int64_t m,
-                                                                 int64_t n, int P, int T, int cpx, double fx_scale,
+                                                                 int64_t n, int P, int T, int ncopy, double fx_scale,
                                                                  double fx_inv, const uint32_t* __restrict__ xmax,
-                                                                 float* __restrict__ out) {
+                                                                 float* __restrict__ out,
+                                                                 const int32_t* __restrict__ lcount = nullptr,
+                                                                 const int32_t* __restrict__ lrows = nullptr) {
   extern __shared__ unsigned long long s_acc[];
   __shared__ int2 s_seg[kWavesPerBlock][kListCap];
   __shared__ float s_xv[DENSE ? kWavesPerBlock : 1][DENSE ? kListCap : 1];
@@ -296,12 +419,23 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
   // a batch of event vectors: vector blockIdx.y of [B][m] into [B][n]
   ev += (int64_t)blockIdx.y * m;
   out += (int64_t)blockIdx.y * n;
+  // rows to walk: the listed active rows, or every row
+  int64_t nrows = m;
+  if constexpr (LISTED) {
+    nrows = lcount[blockIdx.y];
+    lrows += (int64_t)blockIdx.y * m;
+  }
+  // XCD x (blocks are dispatched to the XCDs round-robin) takes the
+  // (copy, tile) pairs [x * ncopy * P / 8, (x + 1) * ncopy * P / 8): whole
+  // copies while ncopy >= 8, else a range of one copy's tiles, so the blocks
+  // of an XCD read the same rows' events and the same part of their
+  // row-tile offsets through its L2
   const int b = blockIdx.x;
-  const int x = b & 7;                 // XCD of this block (round-robin dispatch)
+  const int x = b & 7;
   const int jb = b >> 3;
-  const int p = jb % P;                // output tile
-  const int copy = x * cpx + jb / P;   // output copy (cpx copies per XCD)
-  const int ncopy = 8 * cpx;
+  const int pair = x * (ncopy * P / 8) + jb;
+  const int p = pair % P;              // output tile
+  const int copy = pair / P;           // output copy
   const int64_t t0 = (int64_t)p * T;
   const int tlen = (int)((n - t0) < (int64_t)T ? (n - t0) : (int64_t)T);
   for (int k = threadIdx.x; k < T; k += kTiledThreads) s_acc[k] = 0ull;
@@ -317,29 +451,44 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
   constexpr int NGRP = 64 / GV;         // lane groups per wave
   const int g = lane / GV, lv = lane % GV;
   int2* seg = s_seg[wave];
-  const int64_t nchunks = (m + 63) >> 6;
+  const int64_t nchunks = (nrows + 63) >> 6;
   const int64_t cstep = (int64_t)ncopy * kWavesPerBlock;    // chunk stride of one wave
   int64_t q0 = copy + (int64_t)ncopy * wave;                // first chunk of this wave
 
   // loads are issued for every lane (rows past the end read row m-1 and are
-  // masked), so the four chunks' loads are in flight together
-  auto load_batch = [&](int64_t qb, bool* act, float* xv) {
-    E v[kChunksPerBatch];
+  // masked), so the four chunks' loads are in flight together.  LISTED:
+  // position k of the list holds the row; rid keeps it for the offsets
+  auto load_batch = [&](int64_t qb, bool* act, float* xv, int32_t* rid) {
+    if constexpr (LISTED) {
 #pragma unroll
-    for (int u = 0; u < kChunksPerBatch; ++u) {
-      const int64_t r = ((qb + cstep * u) << 6) + lane;
-      v[u] = ev[r < m ? r : m - 1];
-    }
+      for (int u = 0; u < kChunksPerBatch; ++u) {
+        const int64_t k = ((qb + cstep * u) << 6) + lane;
+        act[u] = k < nrows;
+        rid[u] = act[u] ? lrows[k] : 0;
+      }
+      if constexpr (DENSE) {
 #pragma unroll
-    for (int u = 0; u < kChunksPerBatch; ++u) {
-      const int64_t r = ((qb + cstep * u) << 6) + lane;
-      act[u] = r < m && ev_true(v[u], rule);
-      if constexpr (DENSE) xv[u] = (float)v[u];
+        for (int u = 0; u < kChunksPerBatch; ++u) xv[u] = (float)ev[rid[u]];
+      }
+    } else {
+      E v[kChunksPerBatch];
+#pragma unroll
+      for (int u = 0; u < kChunksPerBatch; ++u) {
+        const int64_t r = ((qb + cstep * u) << 6) + lane;
+        v[u] = ev[r < m ? r : m - 1];
+      }
+#pragma unroll
+      for (int u = 0; u < kChunksPerBatch; ++u) {
+        const int64_t r = ((qb + cstep * u) << 6) + lane;
+        act[u] = r < m && ev_true(v[u], rule);
+        if constexpr (DENSE) xv[u] = (float)v[u];
+      }
     }
   };
   bool act[kChunksPerBatch];
   float xv[kChunksPerBatch];
-  load_batch(q0, act, xv);
+  int32_t rid[kChunksPerBatch];
+  load_batch(q0, act, xv, rid);
   while (q0 < nchunks) {
     // list the batch's active rows with their tile-p segments
     // (inactive lanes read row 0's pair: one shared line, no branch, so the
@@ -352,7 +501,7 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
       int2 sgl[kChunksPerBatch];
 #pragma unroll
       for (int u = 0; u < kChunksPerBatch; ++u) {
-        const int64_t r = act[u] ? ((q0 + cstep * u) << 6) + lane : 0;
+        const int64_t r = !act[u] ? 0 : LISTED ? (int64_t)rid[u] : ((q0 + cstep * u) << 6) + lane;
         const int32_t* rt = rowtile + r * (P + 1) + p;
         sgl[u] = make_int2(rt[0], rt[1]);
       }
@@ -371,7 +520,8 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
     const int64_t qn = q0 + cstep * kChunksPerBatch;
     bool nact[kChunksPerBatch];
     float nxv[kChunksPerBatch];
-    load_batch(qn, nact, nxv);
+    int32_t nrid[kChunksPerBatch];
+    load_batch(qn, nact, nxv, nrid);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -472,6 +622,7 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
     for (int u = 0; u < kChunksPerBatch; ++u) {
       act[u] = nact[u];
       xv[u] = nxv[u];
+      rid[u] = nrid[u];
     }
     q0 = qn;
   }
@@ -625,9 +776,75 @@ __global__ __launch_bounds__(256) void k_fill_zero_t(O* __restrict__ out, int64_
   for (; i < n; i += stride) out[i] = O(0);
 }
 
-// output copies per XCD: two 1024-thread blocks per CU hide the memory
-// latency of the row walk
-static int tiled_copies_per_xcd() { return 2; }
+// zero-fill of the output (n floats, none when accumulating) and of the
+// list counters (nc ints) in one launch
+__global__ __launch_bounds__(256) void k_fill_zero_out_counts(float* __restrict__ out, int64_t n,
+                                                              int32_t* __restrict__ cnt, int64_t nc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = i; k < nc; k += stride) cnt[k] = 0;
+  for (; i < n; i += stride) out[i] = 0.f;
+}
+
+// the active rows of event vector blockIdx.y (ev_true by rule; for a dense
+// vector the rule != 0) appended to lrows[y][...]: a block takes 4096 rows
+// (4 per thread), scans its counts and takes its place in the list with one
+// counter atomic (a counter shared by every wave serialises its atomics:
+// ~11k of them cost ~130 us on a million rows); blocks land in arrival order
+constexpr int kListThreads = 1024;
+constexpr int kListRows = 4 * kListThreads;
+
+template <typename E>
+__global__ __launch_bounds__(kListThreads) void k_tiled_list(const E* __restrict__ ev, int rule, int64_t m,
+                                                             int32_t* __restrict__ lcount,
+                                                             int32_t* __restrict__ lrows) {
+  __shared__ int32_t s_wave[kListThreads / 64];
+  __shared__ int32_t s_base;
+  ev += (int64_t)blockIdx.y * m;
+  lrows += (int64_t)blockIdx.y * m;
+  const int64_t r0 = (int64_t)blockIdx.x * kListRows + 4 * (int64_t)threadIdx.x;
+  uint32_t on = 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (r0 + u < m && ev_true(ev[r0 + u], rule)) on |= 1u << u;
+  const int32_t cnt = __popc(on);
+  int32_t incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(incl, o, 64);
+    if (lane_id() >= (uint32_t)o) incl += y;
+  }
+  const int wv = threadIdx.x >> 6;
+  if (lane_id() == 63) s_wave[wv] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t tot = 0;
+    for (int q = 0; q < kListThreads / 64; ++q) {
+      const int32_t t = s_wave[q];
+      s_wave[q] = tot;
+      tot += t;
+    }
+    s_base = tot > 0 ? atomicAdd(lcount + blockIdx.y, tot) : 0;
+  }
+  __syncthreads();
+  int32_t pos = s_base + s_wave[wv] + incl - cnt;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (on & (1u << u)) lrows[pos++] = (int32_t)(r0 + u);
+}
+
+// output copies of a call: two per XCD up to 64 tiles (two 1024-thread
+// blocks per CU hide the memory latency of the row walk), then fewer as the
+// tiles grow, so that a call keeps ~1024 blocks and its flush (one atomic
+// per output and copy) stays a small part of its traffic; ncopy * P is a
+// multiple of 8 (P > 64 is rounded up to a multiple of 8)
+static int tiled_copies(int P) {
+  if (P <= kMaxTiles) return 16;
+  if (P <= 128) return 8;
+  if (P <= 256) return 4;
+  if (P <= 512) return 2;
+  return 1;
+}
 
 
 struct TiledShape {
@@ -637,15 +854,18 @@ struct TiledShape {
 
 static int max_tile_len();
 
-// one tile per CU of an XCD, or more tiles (up to kMaxTiles) when a tile
-// would not fit the block's LDS
+// one tile per CU of an XCD, or more tiles when a tile would not fit the
+// block's LDS (beyond kMaxTiles a multiple of 8, at most kMaxTilesBig)
 static TiledShape tiled_shape(int64_t n) {
   int cus = device_cus();
   int P = cus / 8;
   if (P < 1) P = 1;
   const int64_t tmax = max_tile_len();
-  if (tmax > 0 && (n + P - 1) / P > tmax) P = (int)((n + tmax - 1) / tmax);
-  if (P > kMaxTiles) P = kMaxTiles;
+  if (tmax > 0 && (n + P - 1) / P > tmax) {
+    const int64_t pn = (n + tmax - 1) / tmax;
+    P = (int)(pn > kMaxTilesBig + 1 ? kMaxTilesBig + 1 : pn);
+  }
+  if (P > kMaxTiles) P = (P + 7) & ~7;
   int64_t T = (n + P - 1) / P;
   if (T < 1) T = 1;
   return {P, (int)T};
@@ -677,7 +897,11 @@ int bpx_tiled_csr_layout(int64_t m, int64_t n, int64_t nnz, int weight_dtype, bp
   if (m < 0 || n < 1 || nnz < 0 || nnz > INT32_MAX) return BPX_ERR_SHAPE;
   if (weight_dtype != BPX_F32 && weight_dtype != BPX_BF16 && weight_dtype != BPX_TILED_HOMO) return BPX_ERR_DTYPE;
   TiledShape ts = tiled_shape(n);
-  if (ts.T > max_tile_len()) return BPX_ERR_SHAPE;
+  if (ts.P > kMaxTilesBig || ts.T > max_tile_len()) return BPX_ERR_SHAPE;
+  // beyond kMaxTiles tiles a row's offsets (4 (P + 1) bytes) and read of
+  // them per active row and tile grow with P: refuse rows shorter than P / 4
+  // synapses on average, for which the stateless kernels do as well
+  if (ts.P > kMaxTiles && m > 0 && nnz * 4 < m * (int64_t)ts.P) return BPX_ERR_SHAPE;
   plan->m = m;
   plan->n = n;
   plan->nnz = nnz;
@@ -770,6 +994,33 @@ int bpx_tiled_csr_build(void* stream, const void* data, int data_dtype, int64_t 
     BPX_CHECK_LAUNCH();
     return BPX_OK;
   }
+#define FILL_BIG(I, V, W)                                                                                     \
+  do {                                                                                                        \
+    hipLaunchKernelGGL((k_tiled_prefill<W>), dim3(2048), dim3(256), 0, s, (const int32_t*)(aux + m), homo ? 0 : 1, \
+                       lidx, (W*)w);                                                                          \
+    hipLaunchKernelGGL((k_tiled_fill_big<I, V, W>), dim3(bblocks), dim3(tb), bshm, s, (const V*)data,         \
+                       homo ? 0 : 1, (const I*)indices, (const I*)indptr, nnz, m, n, P, dT, (const int32_t*)rowtile, \
+                       lidx, (W*)w);                                                                          \
+  } while (0)
+#define BUILD_BIG(I)                                                                                          \
+  do {                                                                                                        \
+    const int tb = big_build_threads(P);                                                                      \
+    const unsigned bblocks = (unsigned)((m + tb - 1) / tb);                                                   \
+    const size_t bshm = (size_t)tb * P * sizeof(int32_t);                                                     \
+    hipLaunchKernelGGL(k_tiled_count_big<I>, dim3(bblocks), dim3(tb), bshm, s, (const I*)indices,             \
+                       (const I*)indptr, nnz, m, n, P, dT, rowtile, aux);                                     \
+    hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(1024), 0, s, aux, m);                                       \
+    hipLaunchKernelGGL(k_tiled_offsets_big, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, s,                   \
+                       (const int32_t*)aux, m, P, rowtile);                                                   \
+    const bool wf = homo || plan->weight_dtype == BPX_F32;                                                    \
+    if (data_dtype == BPX_F64) {                                                                              \
+      if (wf) FILL_BIG(I, double, float); else FILL_BIG(I, double, bf16_t);                                   \
+    } else if (data_dtype == BPX_BF16) {                                                                      \
+      if (wf) FILL_BIG(I, bf16_t, float); else FILL_BIG(I, bf16_t, bf16_t);                                   \
+    } else {                                                                                                  \
+      if (wf) FILL_BIG(I, float, float); else FILL_BIG(I, float, bf16_t);                                     \
+    }                                                                                                         \
+  } while (0)
 #define BUILD_IDX(I)                                                                                           \
   do {                                                                                                         \
     hipLaunchKernelGGL(k_tiled_count<I>, dim3(blocks), dim3(kBuildThreads), 0, s, (const I*)indices,          \
@@ -805,10 +1056,17 @@ int bpx_tiled_csr_build(void* stream, const void* data, int data_dtype, int64_t 
                            rowtile, lidx, (bf16_t*)w);                                                          \
     }                                                                                                          \
   } while (0)
-  if (index_dtype == BPX_I32) BUILD_IDX(int32_t);
-  else if (index_dtype == BPX_I64) BUILD_IDX(int64_t);
-  else return BPX_ERR_DTYPE;
+  if (index_dtype != BPX_I32 && index_dtype != BPX_I64) return BPX_ERR_DTYPE;
+  if (P > kMaxTiles) {
+    if (index_dtype == BPX_I32) BUILD_BIG(int32_t);
+    else BUILD_BIG(int64_t);
+  } else {
+    if (index_dtype == BPX_I32) BUILD_IDX(int32_t);
+    else BUILD_IDX(int64_t);
+  }
 #undef BUILD_IDX
+#undef BUILD_BIG
+#undef FILL_BIG
   BPX_CHECK_LAUNCH();
   return BPX_OK;
 }
@@ -820,6 +1078,19 @@ int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* e
 
 int bpx_event_csrmv_tiled_batch(void* stream, const bpx_tiled_csr* plan, const void* events, int ev_dtype,
                                 int ev_rule, int64_t batch, int accumulate, float* out) {
+  return bpx_event_csrmv_tiled_ws(stream, plan, events, ev_dtype, ev_rule, batch, accumulate, out, nullptr, 0);
+}
+
+// list counters at the start (a 256-byte multiple), then [batch][m] rows
+static size_t list_count_bytes(int64_t batch) { return ((size_t)batch * sizeof(int32_t) + 255) & ~(size_t)255; }
+
+size_t bpx_tiled_csr_call_workspace(const bpx_tiled_csr* plan, int64_t batch) {
+  if (!plan || plan->tiles <= kMaxTiles || plan->m <= 0 || batch < 1) return 0;
+  return list_count_bytes(batch) + (size_t)batch * (size_t)plan->m * sizeof(int32_t);
+}
+
+int bpx_event_csrmv_tiled_ws(void* stream, const bpx_tiled_csr* plan, const void* events, int ev_dtype, int ev_rule,
+                             int64_t batch, int accumulate, float* out, void* workspace, size_t workspace_bytes) {
   if (!plan || !plan->mem) return BPX_ERR_ARG;
   if (ev_rule != BPX_RULE_POSITIVE && ev_rule != BPX_RULE_NONZERO) return BPX_ERR_ARG;
   if (batch < 0 || batch > 65535) return BPX_ERR_SHAPE;
@@ -827,14 +1098,34 @@ int bpx_event_csrmv_tiled_batch(void* stream, const bpx_tiled_csr* plan, const v
   hipStream_t s = (hipStream_t)stream;
   const int64_t m = plan->m, n = plan->n;
   if (!out || (m > 0 && !events)) return BPX_ERR_ARG;
-  if (!accumulate) {
-    int64_t zb = (batch * n + 255) / 256;
+  // plans of more than kMaxTiles tiles list each vector's active rows once
+  // when the caller passes the workspace for it
+  const size_t lws = bpx_tiled_csr_call_workspace(plan, batch);
+  const bool listed = lws > 0 && workspace && workspace_bytes >= lws;
+  int32_t* lcount = listed ? (int32_t*)workspace : nullptr;
+  int32_t* lrows = listed ? (int32_t*)((char*)workspace + list_count_bytes(batch)) : nullptr;
+  if (!accumulate || listed) {
+    const int64_t zn = accumulate ? 0 : batch * n;
+    int64_t zb = ((zn > batch ? zn : batch) + 255) / 256;
     if (zb > 4096) zb = 4096;
-    hipLaunchKernelGGL(k_fill_zero_t<float>, dim3((unsigned)zb), dim3(256), 0, s, out, batch * n);
+    hipLaunchKernelGGL(k_fill_zero_out_counts, dim3((unsigned)zb), dim3(256), 0, s, out, zn, lcount,
+                       listed ? batch : 0);
   }
   if (m == 0) {
     BPX_CHECK_LAUNCH();
     return BPX_OK;
+  }
+  if (listed) {
+    const int64_t lb = (m + kListRows - 1) / kListRows;
+    if (lb > 0x7FFFFFFFll) return BPX_ERR_SHAPE;
+    if (ev_dtype == BPX_BOOL)
+      BPX_LAUNCH(KID_TILED_LIST, k_tiled_list<uint8_t>, dim3((unsigned)lb, (unsigned)batch), dim3(kListThreads), 0,
+                 s, (const uint8_t*)events, ev_rule, m, lcount, lrows);
+    else if (ev_dtype == BPX_F32)
+      BPX_LAUNCH(KID_TILED_LIST, k_tiled_list<float>, dim3((unsigned)lb, (unsigned)batch), dim3(kListThreads), 0,
+                 s, (const float*)events, ev_rule, m, lcount, lrows);
+    else
+      return BPX_ERR_DTYPE;
   }
   const char* base = (const char*)plan->mem;
   const int32_t* rowtile = (const int32_t*)(base + plan->rowtile_offset);
@@ -842,18 +1133,23 @@ int bpx_event_csrmv_tiled_batch(void* stream, const bpx_tiled_csr* plan, const v
   const void* w = base + plan->weights_offset;
   const float* hw = (const float*)(base + plan->homo_offset);
   const int P = plan->tiles, T = plan->tile_len;
-  const int cpx = tiled_copies_per_xcd();
-  const unsigned blocks = (unsigned)(8 * cpx * P);
+  const int ncopy = tiled_copies(P);
+  const unsigned blocks = (unsigned)(ncopy * P);
   const size_t shm = (size_t)T * sizeof(unsigned long long);
   const double fx_scale = ldexp(1.0, plan->fx_shift), fx_inv = ldexp(1.0, -plan->fx_shift);
-#define TLR(WK, E, CO)                                                                                    \
-  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E, 1, false, CO>), dim3(blocks, (unsigned)batch), dim3(kTiledThreads), shm, s, rowtile, \
-             lidx, w, hw, (const E*)events, ev_rule, m, n, P, T, cpx, fx_scale, fx_inv,                         \
-             (const uint32_t*)nullptr, out)
-#define TL(WK, E)                          \
-  do {                                     \
-    if (plan->coarse) TLR(WK, E, true);    \
-    else TLR(WK, E, false);                \
+#define TLR(WK, E, CO, LI)                                                                                \
+  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, E, 1, false, CO, LI>), dim3(blocks, (unsigned)batch), dim3(kTiledThreads), shm, s, rowtile, \
+             lidx, w, hw, (const E*)events, ev_rule, m, n, P, T, ncopy, fx_scale, fx_inv,                         \
+             (const uint32_t*)nullptr, out, (const int32_t*)lcount, (const int32_t*)lrows)
+#define TL(WK, E)                                \
+  do {                                           \
+    if (listed) {                                \
+      if (plan->coarse) TLR(WK, E, true, true);  \
+      else TLR(WK, E, false, true);              \
+    } else {                                     \
+      if (plan->coarse) TLR(WK, E, true, false); \
+      else TLR(WK, E, false, false);             \
+    }                                            \
   } while (0)
 #define TL_E(WK)                                     \
   do {                                               \
@@ -878,16 +1174,24 @@ int bpx_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* x, int 
   if (!workspace || workspace_bytes < 2 * sizeof(uint32_t)) return BPX_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
   const int64_t m = plan->m, n = plan->n;
-  if (!accumulate) {
-    int64_t zb = (n + 255) / 256;
+  // {max finite |x|, min non-zero finite |x|} first; with 256 more bytes
+  // than bpx_tiled_csr_call_workspace(plan, 1), the rows with x != 0 are
+  // listed once (plans of more than kMaxTiles tiles)
+  const size_t lws = bpx_tiled_csr_call_workspace(plan, 1);
+  const bool listed = lws > 0 && workspace_bytes >= 256 + lws;
+  int32_t* lcount = listed ? (int32_t*)((char*)workspace + 256) : nullptr;
+  int32_t* lrows = listed ? (int32_t*)((char*)workspace + 256 + list_count_bytes(1)) : nullptr;
+  if (!accumulate || listed) {
+    const int64_t zn = accumulate ? 0 : n;
+    int64_t zb = ((zn > 1 ? zn : 1) + 255) / 256;
     if (zb > 4096) zb = 4096;
-    hipLaunchKernelGGL(k_fill_zero_t<float>, dim3((unsigned)zb), dim3(256), 0, s, out, n);
+    hipLaunchKernelGGL(k_fill_zero_out_counts, dim3((unsigned)zb), dim3(256), 0, s, out, zn, lcount, listed ? 1 : 0);
   }
   if (m == 0) {
     BPX_CHECK_LAUNCH();
     return BPX_OK;
   }
-  uint32_t* xmax = (uint32_t*)workspace;      // {max finite |x|, min non-zero finite |x|}
+  uint32_t* xmax = (uint32_t*)workspace;
   hipLaunchKernelGGL(k_init_minmax, dim3(1), dim3(64), 0, s, xmax);
   int64_t ab = (m + 255) / 256;
   if (ab > 1024) ab = 1024;
@@ -898,15 +1202,26 @@ int bpx_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* x, int 
   const void* w = base + plan->weights_offset;
   const float* hw = (const float*)(base + plan->homo_offset);
   const int P = plan->tiles, T = plan->tile_len;
-  const int cpx = tiled_copies_per_xcd();
-  const unsigned blocks = (unsigned)(8 * cpx * P);
+  const int ncopy = tiled_copies(P);
+  const unsigned blocks = (unsigned)(ncopy * P);
   // the accumulators, then the outputs' non-finite flags (one byte each)
   const size_t shm = (size_t)T * sizeof(unsigned long long) + (size_t)((T + 3) / 4) * sizeof(uint32_t);
   const double fx_scale = ldexp(1.0, plan->fx_shift), fx_inv = ldexp(1.0, -plan->fx_shift);
-#define TLD(WK, R)                                                                                            \
-  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, float, R, true, false>), dim3(blocks), dim3(kTiledThreads), shm, s, \
-             rowtile, lidx, w, hw, (const float*)x, BPX_RULE_NONZERO, m, n, P, T, cpx, fx_scale, fx_inv, xmax, out)
-#define TLDR(WK) TLD(WK, 1)
+  if (listed) {
+    const int64_t lb = (m + kListRows - 1) / kListRows;
+    if (lb > 0x7FFFFFFFll) return BPX_ERR_SHAPE;
+    BPX_LAUNCH(KID_TILED_LIST, k_tiled_list<float>, dim3((unsigned)lb), dim3(kListThreads), 0, s, (const float*)x,
+               BPX_RULE_NONZERO, m, lcount, lrows);
+  }
+#define TLD(WK, LI)                                                                                           \
+  BPX_LAUNCH(KID_TILED, (k_tiled_scatter<WK, float, 1, true, false, LI>), dim3(blocks), dim3(kTiledThreads), shm, s, \
+             rowtile, lidx, w, hw, (const float*)x, BPX_RULE_NONZERO, m, n, P, T, ncopy, fx_scale, fx_inv, xmax, out, \
+             (const int32_t*)lcount, (const int32_t*)lrows)
+#define TLDR(WK)                 \
+  do {                           \
+    if (listed) TLD(WK, true);   \
+    else TLD(WK, false);         \
+  } while (0)
   if (plan->weight_dtype == BPX_TILED_HOMO) TLDR(0);
   else if (plan->weight_dtype == BPX_F32) TLDR(1);
   else TLDR(2);

@@ -275,10 +275,7 @@ def _launch_batch(d, ev, idx, ptr, shape, transpose, rule):
             plan = _PLANS.lookup(d, idx, ptr, m, n, ev, None)     # a batch counts as repeated use
         if plan is not None:
             out = torch.empty((B, n), dtype=torch.float32, device=ev.device)
-            rc = _lib.lib().bpx_event_csrmv_tiled_batch(_lib.stream_ptr(), plan._plan_ref, _lib.ptr(ev),
-                                                        _EV_CODE[ev.dtype], rule, B, 0, _lib.ptr(out))
-            _lib.check(rc, "bpx_event_csrmv_tiled_batch")
-            return out
+            return plan._exec_batch(ev, rule, B, False, out)
     L = _lib.lib()
     ev_len = m if transpose else n
     out = torch.empty((B, n if transpose else m), dtype=torch.float32, device=ev.device)
@@ -551,6 +548,9 @@ class TiledCSR:
         self.weight_dtype = wcode
         self._plan_ref = ctypes.byref(self.plan)
         self._fn = L.bpx_event_csrmv_tiled
+        # plans of more than 64 tiles list a call's active rows once, in a
+        # workspace of this many bytes per event vector (0: the call scans)
+        self._list_ws = int(L.bpx_tiled_csr_call_workspace(self._plan_ref, 1))
 
     def csrmv(self, events, *, out: torch.Tensor = None, rule: int = _lib.RULE_POSITIVE) -> torch.Tensor:
         """``events @ A`` (a new vector), or ``out += events @ A`` when
@@ -571,10 +571,25 @@ class TiledCSR:
         accumulate = out is not None
         if out is None:
             out = torch.empty(self.shape[1], dtype=torch.float32, device=ev.device)
+        if self._list_ws:
+            return self._exec_batch(ev, rule, 1, accumulate, out)
         rc = self._fn(_lib.raw_stream(ev.get_device()), self._plan_ref, ev.data_ptr(),
                       _EV_CODE[ev.dtype], rule, accumulate, out.data_ptr())
         if rc != 0:
             _lib.check(rc, "bpx_event_csrmv_tiled")
+        return out
+
+    def _exec_batch(self, ev, rule, batch, accumulate, out):
+        """`batch` contiguous event vectors [batch][m] into out [batch][n],
+        with the active-row list workspace when the plan takes one."""
+        L = _lib.lib()
+        wsb = int(L.bpx_tiled_csr_call_workspace(self._plan_ref, batch)) if self._list_ws else 0
+        ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=ev.device)
+        rc = L.bpx_event_csrmv_tiled_ws(_lib.raw_stream(ev.get_device()), self._plan_ref, ev.data_ptr(),
+                                        _EV_CODE[ev.dtype], rule, batch, int(bool(accumulate)), out.data_ptr(),
+                                        ws.data_ptr(), wsb)
+        if rc != 0:
+            _lib.check(rc, "bpx_event_csrmv_tiled_ws")
         return out
 
     def _gather(self, v, event, rule):
@@ -629,8 +644,11 @@ class TiledCSR:
             out = torch.empty(n, dtype=torch.float32, device=xt.device)
         elif out.dtype != torch.float32 or out.shape != (n,) or not out.is_contiguous():
             raise ValueError(f'out must be a contiguous float32 vector of shape ({n},).')
-        ws = torch.empty(8, dtype=torch.uint8, device=xt.device)     # max and min |x| of this call
+        # max and min |x| of this call (8 bytes), and the list of the rows
+        # with x != 0 for plans of more than 64 tiles
+        wsb = 256 + self._list_ws if self._list_ws else 8
+        ws = torch.empty(wsb, dtype=torch.uint8, device=xt.device)
         rc = _lib.lib().bpx_csrmv_tiled(_lib.stream_ptr(), ctypes.byref(self.plan), _lib.ptr(xt),
-                                        _lib.BPX_F32, int(accumulate), _lib.ptr(out), _lib.ptr(ws), 8)
+                                        _lib.BPX_F32, int(accumulate), _lib.ptr(out), _lib.ptr(ws), wsb)
         _lib.check(rc, "bpx_csrmv_tiled")
         return out

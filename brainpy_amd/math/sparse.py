@@ -50,10 +50,14 @@ def _launch(d, idx, ptr, v, shape, transpose):
         # repeated calls on the same arrays run on the cached tiled plan
         # (shared with bm.event.csrmv): the transpose sums its products in
         # LDS with no global atomic per synapse, the gather reads each column
-        # tile of v from LDS instead of one random gather per synapse
+        # tile of v from LDS instead of one random gather per synapse (up to
+        # 64 column tiles: its per-(tile, row) partials grow with the tiles)
         plan = _PLANS.lookup(d, idx, ptr, m, n, v, None)
         if plan is not None:
-            return plan.matvec(v) if transpose else plan._gather(v, False, _lib.RULE_POSITIVE)
+            if transpose:
+                return plan.matvec(v)
+            if plan.plan.tiles <= 64:
+                return plan._gather(v, False, _lib.RULE_POSITIVE)
     out = torch.empty(n if transpose else m, dtype=acc_dtype(d.dtype), device=v.device)
     rc = _lib.lib().bpx_csrmv(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), d.shape[0], _lib.ptr(idx),
                               _lib.ptr(ptr), _lib.bpx_dtype(idx), idx.shape[0], _lib.ptr(v), _lib.bpx_dtype(v), m, n,

@@ -159,7 +159,11 @@ int bpx_event_csrmv_add(void* stream, const void* data, int data_dtype, int64_t 
  * bpx_event_csrmv_tiled per call.  Requires nnz < 2^31 and a tile that fits
  * 32 KiB of LDS next to the kernel's row lists (tile_len <= 4096): one tile
  * per CU of an XCD up to n = 4096 * CUs-per-XCD = 131072 on MI355X, then up
- * to 64 tiles per copy, n <= 262144.  Accumulation inside a tile
+ * to 64 tiles (n <= 262144, 16 output copies per call), then up to 1024
+ * tiles, a multiple of 8 (n <= 4194304; 8, 4, 2 or 1 copies for up to 128,
+ * 256, 512 or 1024 tiles), for which the layout refuses (BPX_ERR_SHAPE)
+ * matrices averaging fewer than tiles / 4 synapses per row: their per-row
+ * offsets would outweigh the synapses.  Accumulation inside a tile
  * is in 64-bit fixed point (LDS integer atomics; gfx950's LDS float atomics
  * are ~20x slower): fx_shift = min(61 - ceil(log2(max|w| * m)),
  * 50 - ceil(log2 max|w|)) keeps every sum below 2^62 and every term below
@@ -197,13 +201,24 @@ int bpx_event_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* e
  * (op_register/utils.py:16-37). */
 int bpx_event_csrmv_tiled_batch(void* stream, const bpx_tiled_csr* plan, const void* events, int ev_dtype,
                                 int ev_rule, int64_t batch, int accumulate, float* out);
+/* Plans of more than 64 tiles: bytes of workspace with which a call lists
+ * each vector's active rows once (one pass over the events) and the tile
+ * blocks walk that list, instead of every tile block scanning every event
+ * of its rows; 0 for smaller plans, which always scan.  Calls that share a
+ * workspace must not overlap. */
+size_t bpx_tiled_csr_call_workspace(const bpx_tiled_csr* plan, int64_t batch);
+/* bpx_event_csrmv_tiled_batch with that workspace (a smaller one, or none,
+ * scans as bpx_event_csrmv_tiled_batch does); same results either way. */
+int bpx_event_csrmv_tiled_ws(void* stream, const bpx_tiled_csr* plan, const void* events, int ev_dtype, int ev_rule,
+                             int64_t batch, int accumulate, float* out, void* workspace, size_t workspace_bytes);
 /*
  * The same plan with a dense f32 vector: out (+)= x @ A, the transpose
  * `csrmv` of sparse/_csr_mv.py:29-123 (cuSPARSE transpose SpMV) for a
  * static matrix.  Rows with x != 0 are streamed; each product w * x[row] is
  * rounded to f32 and summed exactly in fixed point (shift lowered by
  * ceil(log2 max|x|) over the finite x, computed on the device: workspace >= 8
- * bytes; calls that share a workspace must not overlap).  Products that are
+ * bytes, or 256 + bpx_tiled_csr_call_workspace(plan, 1) bytes to list the
+ * rows with x != 0 once; calls that share a workspace must not overlap).  Products that are
  * inf or NaN reach their outputs as IEEE addition would (an infinity, or NaN
  * for a NaN term or infinities of both signs).  When the fixed-point grid is
  * coarser than 2e-6 of the smallest non-zero term (min |w| * min |x|: values

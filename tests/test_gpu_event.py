@@ -312,7 +312,9 @@ def test_tiled_plan_dense_matvec(shape, wkind, xscale):
 
 def test_tiled_plan_output_limit():
     # more than 32 tiles per copy once a tile would not fit the LDS budget:
-    # 64 tiles of 4096 outputs hold n = 262,144; one more output is refused
+    # 64 tiles of 4096 outputs hold n = 262,144 whatever the row lengths;
+    # one more output takes 72 tiles, which need 18 synapses per row on
+    # average, so this two-synapse row is refused
     from brainpy_amd._lib import BPXError
     idx = np.array([0, 262143], np.int32)
     ptr = np.array([0, 2], np.int32)

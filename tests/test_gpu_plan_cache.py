@@ -101,10 +101,11 @@ def test_layer_steps_run_on_one_plan():
 
 
 def test_ineligible_and_disabled_run_stateless():
-    # outputs beyond the plan's limit, f64 weights, f64 events and a disabled
-    # cache all take the stateless scatter with correct results
+    # rows too short for the plan's 80 tiles (10 synapses per row), f64
+    # weights, f64 events and a disabled cache all take the stateless
+    # scatter with correct results
     m, n = 50, 300000
-    idx, ptr = fixed_prob_csr(m, n, 0.001, 11)
+    idx, ptr = fixed_prob_csr(m, n, 10.5 / n, 11)
     w = np.random.default_rng(2).normal(0.5, 0.1, idx.size).astype(np.float32)
     tw, ti, tp = (torch.from_numpy(a).to(DEV) for a in (w, idx, ptr))
     ev = torch.from_numpy(bernoulli(m, 0.5, 3)).to(DEV)
