@@ -289,6 +289,23 @@ __device__ __forceinline__ uint32_t mod_clen(uint32_t x, const DivU32& dv, const
 // never becomes -0), and a homogeneous stream's partial is the sum of its
 // `hits` active terms, ((0 + w) + w) + ...; non-finite weights take the
 // multiply.
+// c + 1 + x % clen, the next connection's column: for MOD_SHORT24 the
+// remainder x - q * clen is one 24-bit multiply-add (q < 2^23 and
+// clen < 2^23 there), which the compiler otherwise splits into a multiply
+// and an add
+template <int MK>
+__device__ __forceinline__ uint32_t next_col(uint32_t c, uint32_t x, const DivU32& dv, const DivShort& ds,
+                                             int32_t negd) {
+  if constexpr (MK == MOD_SHORT24) {
+    const uint32_t q = ds.div(x);
+    int32_t r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(q), "v"(negd), "v"(x));
+    return c + (uint32_t)r + 1u;
+  } else {
+    return c + mod_clen<MK>(x, dv, ds) + 1u;
+  }
+}
+
 template <int DIST, int MK, bool FIN>
 __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* __restrict__ gbits, int64_t nwords,
                                                                 float w0, float w1, DivU32 dv, DivShort ds,
@@ -303,24 +320,34 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
   // homogeneous weights add w0 per active connection like the others
   // (e ? w0 : +0 with finite weights): the partial's sequence of additions
   // is the reference lane's r += weight0 * e, and a stream switch is one add
-  constexpr bool HITS = false;
-  const int64_t nthreads = (int64_t)gridDim.x * kJitThreads;
-  const uint32_t ncol = (uint32_t)num_col;       // num_col < 2^32 (checked by the launcher)
-  const uint32_t c0_last = step * 32u - 1u;      // step * l - 1 at l = 32: the row is done
-  int64_t row = (int64_t)blockIdx.x * kJitThreads + threadIdx.x;
-  bool live = row < num_row;
-  uint32_t kseed = seed + (uint32_t)(row * 32);  // stream key seed + row*32 + l
+  //
+  // A thread takes a contiguous block of rows (num_row < 2^31, checked by
+  // the launcher), so the stream key seed + row * 32 + l simply counts up
+  // across its rows.  Columns are kept 32 above the reference's index
+  // (c = step * l - 1 + 32 at a stream's start): c >> 5 is then the index of
+  // c's event word in s_bits, and the bit is still c % 32.
+  constexpr uint32_t OFF = 32u;
+  const uint32_t nthreads = gridDim.x * kJitThreads;
+  const uint32_t tid = blockIdx.x * kJitThreads + threadIdx.x;
+  const uint32_t rq = (uint32_t)num_row / nthreads, rr = (uint32_t)num_row % nthreads;
+  uint32_t row = tid * rq + min(tid, rr);
+  const uint32_t row_end = row + rq + (tid < rr ? 1u : 0u);
+  bool live = row < row_end;
+  const uint32_t ncol = (uint32_t)num_col + OFF;  // num_col < 2^32 - 64 (checked by the launcher)
+  const uint32_t c_first = OFF - 1u;              // step * 0 - 1
+  const uint32_t c0_last = step * 32u - 1u + OFF; // step * l - 1 at l = 32: the row is done
+  const int32_t negd = -(int32_t)ds.d;
+  uint32_t kseed = seed + row * 32u;              // stream key seed + row*32 + l
   Lfsr88 key(kseed);
-  uint32_t c = 0xFFFFFFFFu;                       // step * l - 1
+  uint32_t c = c_first;
   uint32_t cn = c + step;                         // step * (l + 1) - 1: the next stream's start
   uint32_t end = min(cn, ncol);
   float acc = 0.f, racc = 0.f;
-  uint32_t hits = 0;
-  c = c + mod_clen<MK>(key.next(), dv, ds) + 1u;
+  c = next_col<MK>(c, key.next(), dv, ds, negd);
   while (live) {
     const bool in = c < end;
     // in flight during the draws; lanes past their stream's end read the zero word
-    const uint32_t word = s_bits[in ? (c >> 5) + 1u : 0u];
+    const uint32_t word = s_bits[in ? c >> 5 : 0u];
     const uint32_t cb = c;                               // its bit: cb % 32 (c moves on below)
     float w = w0;
     if constexpr (DIST != BPX_DIST_HOMO) {
@@ -329,36 +356,26 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
       if (FIN || in) w = draw_weight<DIST>(key, w0, w1);
     }
     if (!in) {
-      if constexpr (HITS) {
-        // a plain loop (no memory access, so no wait on the event word
-        // read in flight): 0 or 1 terms for nearly every stream
-        racc += repeat_sum_f(hits, w0);
-        hits = 0;
-      } else {
-        racc += acc;
-        acc = 0.f;
-      }
+      racc += acc;
+      acc = 0.f;
       ++kseed;
       c = cn;
       cn += step;
       if (c == c0_last) {
         out[row] = racc;
         racc = 0.f;
-        row += nthreads;
-        live = row < num_row;
-        kseed = seed + (uint32_t)(row * 32);
-        c = 0xFFFFFFFFu;
-        cn = c + step;
+        ++row;
+        live = row < row_end;
+        c = c_first;
+        cn = c_first + step;
       }
       key = Lfsr88(kseed);
       end = min(cn, ncol);
     }
-    c = c + mod_clen<MK>(key.next(), dv, ds) + 1u;
+    c = next_col<MK>(c, key.next(), dv, ds, negd);
     // bit-field extract of one bit (the offset's low 5 bits select it)
     const uint32_t bit = __builtin_amdgcn_ubfe(word, cb, 1);
-    if constexpr (HITS) {
-      hits += bit;
-    } else if constexpr (FIN) {
+    if constexpr (FIN) {
       // acc + w * e with e in {0, 1}: w * e is exact, so one fused
       // multiply-add rounds exactly as the reference's product-then-add
       // does; e = 0 (also every lane past its stream) adds a zero
@@ -586,7 +603,7 @@ static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t c
   if (outdim) {
     const int64_t nwords = (num_col + 31) / 32;
     if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && bits_fit(nwords + 1) && ws &&
-        ws_bytes >= (size_t)nwords * 4 && num_col < 0xFFFFFFFFll) {
+        ws_bytes >= (size_t)nwords * 4 && num_col < 0xFFFFFFC0ll && num_row < 0x7FFFFFFFll) {
       uint32_t* bits = (uint32_t*)ws;
       hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((num_col + 255) / 256)), dim3(256), 0, s, (const uint8_t*)vec,
                          num_col, bits);

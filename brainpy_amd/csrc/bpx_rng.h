@@ -20,13 +20,27 @@ struct Lfsr88 {
 
   // lfsr88_next_key (tifunc.py:80-86); returns k0^k1^k2 of the new state
   BPX_HD uint32_t next() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // ((s & ~(2^j - 1)) << k) ^ (t >> r) with j + k = 32 - r: the two parts
+    // occupy disjoint bits, so the update is the funnel shift of
+    // {s >> j, t} right by r (v_alignbit_b32), the same bits in 4
+    // instructions per component instead of 6
+    s0 = __builtin_amdgcn_alignbit(s0 >> 1, (s0 << 13) ^ s0, 19u);
+    s1 = __builtin_amdgcn_alignbit(s1 >> 3, (s1 << 2) ^ s1, 25u);
+    s2 = __builtin_amdgcn_alignbit(s2 >> 4, (s2 << 3) ^ s2, 11u);
+#else
     uint32_t b = ((s0 << 13) ^ s0) >> 19;
     s0 = ((s0 & 4294967294u) << 12) ^ b;
     b = ((s1 << 2) ^ s1) >> 25;
     s1 = ((s1 & 4294967288u) << 4) ^ b;
     b = ((s2 << 3) ^ s2) >> 11;
     s2 = ((s2 & 4294967280u) << 17) ^ b;
+#endif
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(s0, s1, s2, 0x96);   // s0 ^ s1 ^ s2 in one instruction
+#else
     return s0 ^ s1 ^ s2;
+#endif
   }
 };
 
