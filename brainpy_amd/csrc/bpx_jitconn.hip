@@ -306,7 +306,9 @@ __device__ __forceinline__ uint32_t next_col(uint32_t c, uint32_t x, const DivU3
   }
 }
 
-template <int DIST, int MK, bool FIN>
+// CLIP: some stream of a row ends past num_col (32 * step > num_col), so a
+// stream's end is min(next start, num_col); otherwise the next start itself
+template <int DIST, int MK, bool FIN, bool CLIP>
 __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* __restrict__ gbits, int64_t nwords,
                                                                 float w0, float w1, DivU32 dv, DivShort ds,
                                                                 uint32_t seed, int64_t num_row, int64_t num_col,
@@ -341,7 +343,7 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
   Lfsr88 key(kseed);
   uint32_t c = c_first;
   uint32_t cn = c + step;                         // step * (l + 1) - 1: the next stream's start
-  uint32_t end = min(cn, ncol);
+  uint32_t end = CLIP ? min(cn, ncol) : cn;
   float acc = 0.f, racc = 0.f;
   c = next_col<MK>(c, key.next(), dv, ds, negd);
   while (live) {
@@ -370,7 +372,7 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
         cn = c_first + step;
       }
       key = Lfsr88(kseed);
-      end = min(cn, ncol);
+      end = CLIP ? min(cn, ncol) : cn;
     }
     c = next_col<MK>(c, key.next(), dv, ds, negd);
     // bit-field extract of one bit (the offset's low 5 bits select it)
@@ -621,14 +623,20 @@ static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t c
         fin = fin && fabs((double)w0) + 8.0 * fabs((double)w1) < 3.0e38;
       DivShort ds = {1u, 0u, 0u};
       const bool shrt = DivShort::make((uint32_t)clen, &ds) && DivShort::mad24_ok((uint32_t)clen);
-#define JRB(MKV, F)                                                                                           \
-  BPX_LAUNCH(KID_JIT_OUTDIM, (k_jit_rows_bits<DIST, MKV, F>), dim3((unsigned)blocks), dim3(kJitThreads),        \
+      const bool clip = (uint64_t)step * 32u > (uint64_t)num_col;
+#define JRB(MKV, F, CL)                                                                                       \
+  BPX_LAUNCH(KID_JIT_OUTDIM, (k_jit_rows_bits<DIST, MKV, F, CL>), dim3((unsigned)blocks), dim3(kJitThreads),    \
              (size_t)(nwords + 1) * 4, s, (const uint32_t*)bits, nwords, w0, w1, dv, ds, seed, num_row, num_col, step, \
              out)
-#define JRB_F(MKV)             \
-  do {                         \
-    if (fin) JRB(MKV, true);   \
-    else JRB(MKV, false);      \
+#define JRB_F(MKV)                                 \
+  do {                                             \
+    if (clip) {                                    \
+      if (fin) JRB(MKV, true, true);               \
+      else JRB(MKV, false, true);                  \
+    } else {                                       \
+      if (fin) JRB(MKV, true, false);              \
+      else JRB(MKV, false, false);                 \
+    }                                              \
   } while (0)
       if (shrt) JRB_F(MOD_SHORT24);
       else if (dv.small_quotient()) JRB_F(MOD_GM24);
