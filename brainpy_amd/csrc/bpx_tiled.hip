@@ -1093,6 +1093,7 @@ int bpx_event_csrmv_tiled_ws(void* stream, const bpx_tiled_csr* plan, const void
                              int64_t batch, int accumulate, float* out, void* workspace, size_t workspace_bytes) {
   if (!plan || !plan->mem) return BPX_ERR_ARG;
   if (ev_rule != BPX_RULE_POSITIVE && ev_rule != BPX_RULE_NONZERO) return BPX_ERR_ARG;
+  if (ev_dtype != BPX_BOOL && ev_dtype != BPX_F32) return BPX_ERR_DTYPE;
   if (batch < 0 || batch > 65535) return BPX_ERR_SHAPE;
   if (batch == 0) return BPX_OK;
   hipStream_t s = (hipStream_t)stream;

@@ -189,6 +189,22 @@ def test_argument_validation_of_coo_conn_and_plan_entry_points():
     assert L.bpx_csrmv_tiled(None, ctypes.byref(plan), None, F32, 0, None, None, 0) == WS
     # the workspace holds max and min |x|: 8 bytes (checked before any launch)
     assert L.bpx_csrmv_tiled(None, ctypes.byref(plan), None, F32, 0, None, ctypes.c_void_p(256), 4) == WS
+    # the active-row list workspace: none up to 64 tiles; beyond, the
+    # counters (a 256-byte multiple) then [batch][m] rows
+    plan.m, plan.tiles = 10, 64
+    assert L.bpx_tiled_csr_call_workspace(ctypes.byref(plan), 1) == 0
+    plan.tiles = 72
+    assert L.bpx_tiled_csr_call_workspace(ctypes.byref(plan), 1) == 256 + 40
+    assert L.bpx_tiled_csr_call_workspace(ctypes.byref(plan), 3) == 256 + 120
+    assert L.bpx_tiled_csr_call_workspace(ctypes.byref(plan), 0) == 0
+    assert L.bpx_tiled_csr_call_workspace(None, 1) == 0
+    BOOL = _lib.BPX_BOOL
+    ws = ctypes.c_void_p(256)
+    assert L.bpx_event_csrmv_tiled_ws(None, None, None, BOOL, 0, 1, 0, None, ws, 296) == ARG
+    assert L.bpx_event_csrmv_tiled_ws(None, ctypes.byref(plan), None, BOOL, 7, 1, 0, None, ws, 296) == ARG  # rule
+    assert L.bpx_event_csrmv_tiled_ws(None, ctypes.byref(plan), None, F64, 0, 1, 0, None, ws, 296) == DTYPE
+    assert L.bpx_event_csrmv_tiled_ws(None, ctypes.byref(plan), None, BOOL, 0, -1, 0, None, ws, 296) == SHAPE
+    assert L.bpx_event_csrmv_tiled_ws(None, ctypes.byref(plan), None, BOOL, 0, 1, 0, None, ws, 296) == ARG  # no out
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libbpx.so not built")
