@@ -18,11 +18,27 @@
 //     multiple of 4, the arrays 16-byte aligned, and col values inside
 //     [0, src_len).
 //
-// Both return 0 or a negative bpx status code.
+//   int bpx_probe_valu(void* stream, int kind, int64_t iters, int blocks, int threads,
+//                      uint64_t* out);
+//     Integer issue-rate probe: every thread runs `iters` iterations of one
+//     loop body and folds its results into out[thread] (so nothing is dead
+//     code); thread 0 of each block stores its shader-clock and 100 MHz
+//     wall-clock ticks at start and end into out[total_threads + 4 * block
+//     ...] (4 words per block), from which the shader clock under load
+//     follows.  kind 0: one LFSR88 draw per iteration (one generator,
+//     three independent components); kind 1: two generators interleaved;
+//     kind 2: the draw plus the multiply-high modulo and the column add of
+//     the jitconn kernels; kind 3: one LFSR88 draw of a wave-uniform
+//     generator (scalar ALU); kind 4: kinds 0 and 3 interleaved; kinds 5,
+//     6, 7: kind 2 plus 8 independent FP32 fused multiply-adds, FP64
+//     fused multiply-adds or 32-bit selects per iteration.
+//
+// All return 0 or a negative bpx status code.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../include/bpx.h"
+#include "bpx_rng.h"
 
 namespace {
 
@@ -65,6 +81,75 @@ __global__ __launch_bounds__(256) void k_probe_coo(const int32_t* __restrict__ r
 
 bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
+// the host form of the taus88 step (shifts, masks, xors): on a wave-uniform
+// state the compiler keeps it on the scalar ALU
+__device__ __forceinline__ uint32_t taus88_scalar(uint32_t& s0, uint32_t& s1, uint32_t& s2) {
+  uint32_t b = ((s0 << 13) ^ s0) >> 19;
+  s0 = ((s0 & 4294967294u) << 12) ^ b;
+  b = ((s1 << 2) ^ s1) >> 25;
+  s1 = ((s1 & 4294967288u) << 4) ^ b;
+  b = ((s2 << 3) ^ s2) >> 11;
+  s2 = ((s2 & 4294967280u) << 17) ^ b;
+  return s0 ^ s1 ^ s2;
+}
+
+template <int KIND>
+__global__ void k_probe_valu(int64_t iters, bpx::DivShort ds, uint64_t* __restrict__ out) {
+  const uint64_t t0 = clock64(), w0 = wall_clock64();
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  bpx::Lfsr88 a(tid * 2u), b(tid * 2u + 1u);
+  uint32_t acc = 0, c = tid;
+  const int32_t negd = -(int32_t)ds.d;
+  uint32_t u0 = __builtin_amdgcn_readfirstlane(blockIdx.x + 1u), u1 = u0 + 6u, u2 = u0 + 14u, uacc = 0;
+  float f[8];
+  double g[8];
+  uint32_t sel[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    f[k] = (float)(tid + k);
+    g[k] = (double)(tid + k);
+    sel[k] = tid * (k + 1);
+  }
+  const float fm = 1.0000001f + (float)(tid & 1) * 1e-7f;
+  const double gm = 1.0000000001 + (double)(tid & 1) * 1e-12;
+  for (int64_t i = 0; i < iters; ++i) {
+    if constexpr (KIND >= 5) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if constexpr (KIND == 5) f[k] = __builtin_fmaf(f[k], fm, 0.5f);
+        else if constexpr (KIND == 6) g[k] = __builtin_fma(g[k], gm, 0.5);
+        else sel[k] = ((sel[k] ^ (uint32_t)i) & 1u) ? sel[k] + 3u : sel[k] ^ 5u;
+      }
+    }
+    if constexpr (KIND == 0) {
+      acc ^= a.next();
+    } else if constexpr (KIND == 3) {
+      uacc += taus88_scalar(u0, u1, u2);
+    } else if constexpr (KIND == 4) {
+      acc ^= a.next();
+      uacc += taus88_scalar(u0, u1, u2);
+    } else if constexpr (KIND == 1) {
+      acc ^= a.next();
+      acc += b.next();
+    } else if constexpr (KIND == 2 || KIND >= 5) {
+      const uint32_t x = a.next();
+      const uint32_t q = ds.div(x);
+      int32_t r;
+      asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(q), "v"(negd), "v"(x));
+      c = c + (uint32_t)r + 1u;
+    }
+  }
+  uint32_t extra = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) extra += (uint32_t)f[k] + (uint32_t)g[k] + sel[k];
+  out[tid] = (uint64_t)acc + c + uacc + extra;
+  const uint64_t t1 = clock64(), w1 = wall_clock64();
+  if (threadIdx.x == 0) {
+    uint64_t* o = out + (uint64_t)gridDim.x * blockDim.x + 4u * blockIdx.x;
+    o[0] = t0; o[1] = t1; o[2] = w0; o[3] = w1;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -98,6 +183,26 @@ int bpx_probe_coo(void* stream, const int32_t* row, const int32_t* col, const fl
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_probe_coo, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, row, col, data, src,
                      nse / 4, mode, out);
+  return hipGetLastError() == hipSuccess ? BPX_OK : BPX_ERR_LAUNCH;
+}
+
+int bpx_probe_valu(void* stream, int kind, int64_t iters, int blocks, int threads, uint64_t* out) {
+  if (kind < 0 || kind > 7 || iters < 0 || blocks <= 0 || threads <= 0 || threads > 1024 || (threads & 63))
+    return BPX_ERR_ARG;
+  if (!out) return BPX_ERR_ARG;
+  bpx::DivShort ds = {1u, 0u, 0u};
+  if (!bpx::DivShort::make(19999u, &ds)) return BPX_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  switch (kind) {
+    case 0: hipLaunchKernelGGL(k_probe_valu<0>, dim3(blocks), dim3(threads), 0, s, iters, ds, out); break;
+    case 1: hipLaunchKernelGGL(k_probe_valu<1>, dim3(blocks), dim3(threads), 0, s, iters, ds, out); break;
+    case 2: hipLaunchKernelGGL(k_probe_valu<2>, dim3(blocks), dim3(threads), 0, s, iters, ds, out); break;
+    case 3: hipLaunchKernelGGL(k_probe_valu<3>, dim3(blocks), dim3(threads), 0, s, iters, ds, out); break;
+    case 4: hipLaunchKernelGGL(k_probe_valu<4>, dim3(blocks), dim3(threads), 0, s, iters, ds, out); break;
+    case 5: hipLaunchKernelGGL(k_probe_valu<5>, dim3(blocks), dim3(threads), 0, s, iters, ds, out); break;
+    case 6: hipLaunchKernelGGL(k_probe_valu<6>, dim3(blocks), dim3(threads), 0, s, iters, ds, out); break;
+    default: hipLaunchKernelGGL(k_probe_valu<7>, dim3(blocks), dim3(threads), 0, s, iters, ds, out); break;
+  }
   return hipGetLastError() == hipSuccess ? BPX_OK : BPX_ERR_LAUNCH;
 }
 
