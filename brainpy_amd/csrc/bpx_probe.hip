@@ -33,6 +33,18 @@
 //     6, 7: kind 2 plus 8 independent FP32 fused multiply-adds, FP64
 //     fused multiply-adds or 32-bit selects per iteration.
 //
+//   int bpx_probe_segstream(void* stream, const int32_t* desc_off, const int32_t* desc,
+//                           const uint16_t* lidx, const float* w, int P, int ncopy, int R,
+//                           int mode, uint32_t* out);
+//     The segment reads of the tiled event scatter without its listing:
+//     block b takes the (copy, tile) pair of the scatter kernel and streams
+//     the segments desc[2 * e], desc[2 * e + 1] (start, end into lidx / w)
+//     for e in [desc_off[copy * P + tile], desc_off[copy * P + tile + 1]),
+//     8 lanes per segment, R segments in flight per lane group; mode 0
+//     folds the loaded words into one checksum per thread, mode 1 adds
+//     each weight into a 3200-entry LDS array as 64-bit integers (the
+//     scatter's accumulation).  out has ncopy * P * 1024 words.
+//
 // All return 0 or a negative bpx status code.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -91,6 +103,69 @@ __device__ __forceinline__ uint32_t taus88_scalar(uint32_t& s0, uint32_t& s1, ui
   b = ((s2 << 3) ^ s2) >> 11;
   s2 = ((s2 & 4294967280u) << 17) ^ b;
   return s0 ^ s1 ^ s2;
+}
+
+template <int R, int MODE>
+__global__ __launch_bounds__(1024) void k_probe_segstream(const int32_t* __restrict__ desc_off,
+                                                          const int32_t* __restrict__ desc,
+                                                          const uint16_t* __restrict__ lidx,
+                                                          const float* __restrict__ w, int P, int ncopy,
+                                                          uint32_t* __restrict__ out) {
+  __shared__ unsigned long long s_acc[3200];
+  if (MODE == 1)
+    for (int k = threadIdx.x; k < 3200; k += 1024) s_acc[k] = 0ull;
+  __syncthreads();
+  const int b = blockIdx.x;
+  const int pair = (b & 7) * (ncopy * P / 8) + (b >> 3);
+  const int blk = (pair / P) * P + pair % P;
+  const int d0 = desc_off[blk], d1 = desc_off[blk + 1];
+  const int g = threadIdx.x >> 3, lv = threadIdx.x & 7;
+  uint32_t acc = 0;
+  for (int e = d0 + g; e < d1; e += R * 128) {
+    int32_t st[R], en[R];
+    int32_t len = 0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const bool ok = e + q * 128 < d1;
+      st[q] = ok ? desc[2 * (e + q * 128)] : 0;
+      en[q] = ok ? desc[2 * (e + q * 128) + 1] : 0;
+      len = max(len, en[q] - st[q]);
+    }
+    for (int k = 4 * lv; k < len; k += 32) {
+      uint2 li[R];
+      float4 wv[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        li[q] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+        wv[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (st[q] + k < en[q]) {
+          li[q] = *reinterpret_cast<const uint2*>(lidx + st[q] + k);
+          wv[q] = *reinterpret_cast<const float4*>(w + st[q] + k);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        if constexpr (MODE == 0) {
+          acc ^= li[q].x ^ li[q].y ^ __float_as_uint(wv[q].x) ^ __float_as_uint(wv[q].y) ^
+                 __float_as_uint(wv[q].z) ^ __float_as_uint(wv[q].w);
+        } else {
+          const float wf[4] = {wv[q].x, wv[q].y, wv[q].z, wv[q].w};
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t c = ((u < 2 ? li[q].x : li[q].y) >> (16 * (u & 1))) & 0xFFFFu;
+            if (c < 3200u)
+              __hip_atomic_fetch_add(s_acc + c, (unsigned long long)(long long)(wf[u] * 1048576.0f),
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        }
+      }
+    }
+  }
+  if constexpr (MODE == 1) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < 3200; k += 1024) acc ^= (uint32_t)s_acc[k];
+  }
+  out[(size_t)b * 1024 + threadIdx.x] = acc;
 }
 
 template <int KIND>
@@ -203,6 +278,21 @@ int bpx_probe_valu(void* stream, int kind, int64_t iters, int blocks, int thread
     case 6: hipLaunchKernelGGL(k_probe_valu<6>, dim3(blocks), dim3(threads), 0, s, iters, ds, out); break;
     default: hipLaunchKernelGGL(k_probe_valu<7>, dim3(blocks), dim3(threads), 0, s, iters, ds, out); break;
   }
+  return hipGetLastError() == hipSuccess ? BPX_OK : BPX_ERR_LAUNCH;
+}
+
+int bpx_probe_segstream(void* stream, const int32_t* desc_off, const int32_t* desc, const uint16_t* lidx,
+                        const float* w, int P, int ncopy, int R, int mode, uint32_t* out) {
+  if (!desc_off || !desc || !lidx || !w || !out || P <= 0 || ncopy <= 0 || ((ncopy * P) & 7)) return BPX_ERR_ARG;
+  if (mode < 0 || mode > 1) return BPX_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid((unsigned)(ncopy * P));
+#define SEG(RV, MV) hipLaunchKernelGGL((k_probe_segstream<RV, MV>), grid, dim3(1024), 0, s, desc_off, desc, lidx, w, P, ncopy, out)
+  if (R == 1) { if (mode) SEG(1, 1); else SEG(1, 0); }
+  else if (R == 2) { if (mode) SEG(2, 1); else SEG(2, 0); }
+  else if (R == 4) { if (mode) SEG(4, 1); else SEG(4, 0); }
+  else return BPX_ERR_ARG;
+#undef SEG
   return hipGetLastError() == hipSuccess ? BPX_OK : BPX_ERR_LAUNCH;
 }
 
