@@ -760,6 +760,139 @@ __global__ __launch_bounds__(256) void k_tiled_gather(const int32_t* __restrict_
   }
 }
 
+// The same partials from blocks that each stage a range of NT consecutive
+// column tiles of v in LDS (up to ~150 KB) and walk, for their slice of the
+// rows, every (row, tile) segment of the range: a wave takes the items
+// (row, tile) in row-major order, 64 at a time (lane l loads item l's
+// segment bounds), and group g of 8 lanes takes items g, g + 8, ..., so the
+// 8 groups of a round read consecutive tiles' segments of one row, one
+// contiguous span of the plan, RB rounds' loads in flight together.  Each
+// (tile, row) partial is summed as in k_tiled_gather (entries in lane
+// order, the fixed 8-lane tree), so the results are bit-identical to it.
+// Block b takes range b % NR and row slice b / NR.
+constexpr int kGatherRangeThreads = 1024;
+#ifndef BPX_GATHER_RANGE
+#define BPX_GATHER_RANGE 1
+#endif
+
+template <int WK, int XM>
+__global__ __launch_bounds__(kGatherRangeThreads) void k_tiled_gather_rng(
+    const int32_t* __restrict__ rowtile, const uint16_t* __restrict__ lidx, const void* __restrict__ wptr,
+    const float* __restrict__ homo_w, const void* __restrict__ v, int rule, int64_t m, int64_t n, int P, int T,
+    int NT, int NR, int64_t rows_per_slice, float* __restrict__ partial) {
+  extern __shared__ float s_v[];
+  const int k = (int)(blockIdx.x % (unsigned)NR);
+  const int64_t slice = blockIdx.x / (unsigned)NR;
+  const int ta = k * NT;                                   // the range's first tile
+  const int nt = min(NT, P - ta);                          // its tiles
+  const int64_t c0 = (int64_t)ta * T;
+  const int64_t clen = min((int64_t)nt * T, n - c0);       // its columns
+  for (int64_t c = threadIdx.x; c < clen; c += kGatherRangeThreads) {
+    float x;
+    if constexpr (XM == 0) x = ((const float*)v)[c0 + c];
+    else if constexpr (XM == 1) x = ((const uint8_t*)v)[c0 + c] ? 1.f : 0.f;
+    else x = ev_true(((const float*)v)[c0 + c], rule) ? 1.f : 0.f;
+    s_v[c] = x;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int lv = lane & 7;
+  const int g = lane >> 3;
+  const int64_t r0 = slice * rows_per_slice;
+  const int64_t r1 = min(r0 + rows_per_slice, m);
+  if (r0 >= r1) return;
+  float hw = 0.f;
+  if constexpr (WK == 0) hw = *homo_w;
+  auto load_w = [&](int32_t j, float (&w4)[4]) {
+    if constexpr (WK == 1) {
+      const float4 f = *reinterpret_cast<const float4*>((const float*)wptr + j);
+      w4[0] = f.x; w4[1] = f.y; w4[2] = f.z; w4[3] = f.w;
+    } else if constexpr (WK == 2) {
+      const uint2 h = *reinterpret_cast<const uint2*>((const bf16_t*)wptr + j);
+      w4[0] = __uint_as_float(h.x << 16); w4[1] = __uint_as_float(h.x & 0xFFFF0000u);
+      w4[2] = __uint_as_float(h.y << 16); w4[3] = __uint_as_float(h.y & 0xFFFF0000u);
+    } else {
+      w4[0] = w4[1] = w4[2] = w4[3] = 1.f;
+    }
+  };
+  // tile-local column c of tile (ta + tl) sits at s_v[tl * T + c]; padding
+  // entries (0xFFFF) and columns past the tile's end are skipped
+  auto term = [&](uint32_t c, int tl, int tlen, float wv, float& acc) {
+    if (c < (uint32_t)tlen) {
+      if constexpr (WK == 0) acc += s_v[tl * T + c];
+      else acc += wv * s_v[tl * T + c];
+    }
+  };
+  const int64_t nitems = (r1 - r0) * nt;
+  constexpr int RB = 4;
+  for (int64_t i0 = (int64_t)wave * 64; i0 < nitems; i0 += (int64_t)(kGatherRangeThreads / 64) * 64) {
+    // lane l: bounds of item i0 + l
+    int32_t sb = 0, se = 0;
+    {
+      const int64_t it = i0 + lane;
+      if (it < nitems) {
+        const int64_t row = r0 + it / nt;
+        const int32_t* rt = rowtile + row * (P + 1) + ta + (int)(it % nt);
+        sb = rt[0];
+        se = rt[1];
+      }
+    }
+#pragma unroll
+    for (int q0 = 0; q0 < 8; q0 += RB) {
+      // the first 64 entries of each segment (segments average n / P per
+      // row; 1000 / 32 here) in flight together, the rest after
+      int32_t s_[RB], e_[RB];
+      uint2 a[RB][2];
+      float w4[RB][2][4];
+#pragma unroll
+      for (int q = 0; q < RB; ++q) {
+        const int src = 8 * (q0 + q) + g;                  // the group's item of round q0 + q
+        s_[q] = __shfl(sb, src, 64);
+        e_[q] = __shfl(se, src, 64);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int32_t j = s_[q] + 32 * h + 4 * lv;
+          const bool ok = j < e_[q];
+          const int32_t js = ok ? j : 0;
+          a[q][h] = *reinterpret_cast<const uint2*>(lidx + js);
+          load_w(js, w4[q][h]);
+          if (!ok) a[q][h] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < RB; ++q) {
+        const int64_t it = i0 + 8 * (q0 + q) + g;
+        const int tl = (int)(it % nt);
+        const int64_t row = r0 + it / nt;
+        const int tlen = (int)min((int64_t)T, n - (int64_t)(ta + tl) * T);
+        float acc = 0.f;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          term(a[q][h].x & 0xFFFFu, tl, tlen, w4[q][h][0], acc);
+          term(a[q][h].x >> 16, tl, tlen, w4[q][h][1], acc);
+          term(a[q][h].y & 0xFFFFu, tl, tlen, w4[q][h][2], acc);
+          term(a[q][h].y >> 16, tl, tlen, w4[q][h][3], acc);
+        }
+        for (int32_t j = s_[q] + 64 + 4 * lv; j < e_[q]; j += 32) {
+          const uint2 a2 = *reinterpret_cast<const uint2*>(lidx + j);
+          float v4[4];
+          load_w(j, v4);
+          term(a2.x & 0xFFFFu, tl, tlen, v4[0], acc);
+          term(a2.x >> 16, tl, tlen, v4[1], acc);
+          term(a2.y & 0xFFFFu, tl, tlen, v4[2], acc);
+          term(a2.y >> 16, tl, tlen, v4[3], acc);
+        }
+        acc += __shfl_xor(acc, 1, 64);
+        acc += __shfl_xor(acc, 2, 64);
+        acc += __shfl_xor(acc, 4, 64);
+        if constexpr (WK == 0) acc = hw * acc;
+        if (lv == 0 && it < nitems) partial[(int64_t)(ta + tl) * m + row] = acc;
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_tiled_gather_sum(const float* __restrict__ partial, int64_t m, int P,
                                                           float* __restrict__ out) {
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -869,6 +1002,18 @@ static TiledShape tiled_shape(int64_t n) {
   int64_t T = (n + P - 1) / P;
   if (T < 1) T = 1;
   return {P, (int)T};
+}
+
+// the device's LDS per block (bytes)
+static int shm_bytes() {
+  static int shm = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+      return 65536;
+    return v;
+  }();
+  return shm;
 }
 
 // largest tile the device's per-block LDS holds (floats), capped by the
@@ -1257,8 +1402,40 @@ int bpx_tiled_csr_gather(void* stream, const bpx_tiled_csr* plan, const void* ve
   const int64_t chunks = (m + kGatherRows - 1) / kGatherRows;
   const int64_t blocks = chunks * P;
   if (blocks > 0x7FFFFFFFll) return BPX_ERR_SHAPE;
-  const size_t shm = (size_t)T * sizeof(float);
   const int xm = event ? (vec_dtype == BPX_BOOL ? 1 : 2) : 0;
+  // tiles per LDS range: as many as the block's LDS holds (at most P), the
+  // ranges then evened out; the row slices keep ~one block per CU
+  const int64_t ntmax = (int64_t)shm_bytes() / ((int64_t)T * (int64_t)sizeof(float));
+  if (BPX_GATHER_RANGE && ntmax >= 2) {
+    const int NR = (int)((P + std::min<int64_t>(ntmax, P) - 1) / std::min<int64_t>(ntmax, P));
+    const int NT = (P + NR - 1) / NR;
+    int64_t slices = device_cus() / NR;
+    if (slices < 1) slices = 1;
+    const int64_t rps = (m + slices - 1) / slices;
+    slices = (m + rps - 1) / rps;
+    const int64_t rblocks = slices * NR;
+    if (rblocks > 0x7FFFFFFFll) return BPX_ERR_SHAPE;
+    const size_t rshm = (size_t)NT * T * sizeof(float);
+#define TGR(WK, XM)                                                                                              \
+  BPX_LAUNCH(KID_TILED_GATHER, (k_tiled_gather_rng<WK, XM>), dim3((unsigned)rblocks), dim3(kGatherRangeThreads), rshm, \
+             s, rowtile, lidx, w, hw, vec, ev_rule, m, n, P, T, NT, NR, rps, partial)
+#define TGR_X(WK)                  \
+  do {                             \
+    if (xm == 0) TGR(WK, 0);       \
+    else if (xm == 1) TGR(WK, 1);  \
+    else TGR(WK, 2);               \
+  } while (0)
+    if (plan->weight_dtype == BPX_TILED_HOMO) TGR_X(0);
+    else if (plan->weight_dtype == BPX_F32) TGR_X(1);
+    else TGR_X(2);
+#undef TGR_X
+#undef TGR
+    hipLaunchKernelGGL(k_tiled_gather_sum, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, (const float*)partial,
+                       m, P, out);
+    BPX_CHECK_LAUNCH();
+    return BPX_OK;
+  }
+  const size_t shm = (size_t)T * sizeof(float);
 #define TG(WK, XM)                                                                                          \
   BPX_LAUNCH(KID_TILED_GATHER, (k_tiled_gather<WK, XM>), dim3((unsigned)blocks), dim3(256), shm, s, rowtile, lidx, w, \
              hw, vec, ev_rule, m, n, P, T, partial)
