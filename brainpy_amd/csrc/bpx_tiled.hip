@@ -898,7 +898,16 @@ __global__ __launch_bounds__(256) void k_tiled_gather_sum(const float* __restric
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (r >= m) return;
   float acc = 0.f;
-  for (int p = 0; p < P; ++p) acc += partial[(int64_t)p * m + r];
+  // the loads of 8 tiles in flight together, the adds in tile order
+  int p = 0;
+  for (; p + 8 <= P; p += 8) {
+    float t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = partial[(int64_t)(p + u) * m + r];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += t[u];
+  }
+  for (; p < P; ++p) acc += partial[(int64_t)p * m + r];
   out[r] = acc;
 }
 
