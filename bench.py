@@ -360,8 +360,37 @@ def bench_forms(_lib, torch, steps=30, warmup=3):
         res.append(r)
         print(f"# {name} {r['us_per_call']:8.1f} us/call kernel {r['kernel_us']:8.1f} us {r['gbs']:7.0f} GB/s "
               f"({100 * r['frac']:.1f}%) {r['synapses_per_s']:.3e} synapses/s", file=sys.stderr, flush=True)
+    # the public event gather, bm.event.csrmv(..., transpose=False), on the
+    # cached plan of the transposed matrix: each call scatters the entries of
+    # its active columns only (16 Bernoulli(0.1) vectors in turn, so the
+    # entries come from HBM); bytes as an event-driven operator reads them:
+    # 1 per event, an 8-byte offset pair per active column, 8 per synaptic
+    # event and 4 per output
+    evs = [torch.rand(n, generator=g, device=dev) < 0.1 for _ in range(16)]
+    for k in range(2):
+        bm.event.csrmv(w, idx, ptr, evs[k], shape=(n, n))
+    hits0 = bm.event.plan_cache_info()["hits"]
+    cnt = [0]
+
+    def ecall():
+        bm.event.csrmv(w, idx, ptr, evs[cnt[0] % 16], shape=(n, n))
+        cnt[0] += 1
+    wall, kern = _time_calls(_lib, torch, ecall, _lib.KID["tiled"], 64, 16)
+    if bm.event.plan_cache_info()["hits"] - hits0 < 64:
+        raise RuntimeError("bm.event.csrmv(transpose=False) did not run on its cached transposed plan")
+    cols = sum(int(e.sum()) for e in evs) / 16
+    events = cols * (nnz / n)
+    alg = n + cols * 8 + events * 8 + n * 4
+    r = {"op": "bm.event.csrmv transpose=0 (public path, cached plan of A^T)", "transpose": False,
+         "events": "bool", "nnz": nnz, "active_columns": cols, "events_per_call": events,
+         "us_per_call": wall * 1e6, "kernel_us": kern * 1e6, "alg_bytes_event_driven": alg,
+         "gbs": alg / kern / 1e9, "frac": alg / kern / 1e9 / HBM_PEAK_GBS, "events_per_s": events / wall}
+    res.append(r)
+    print(f"# bm.event.csrmv transpose=0 (public path, cached plan of A^T, 16 event vectors) {r['us_per_call']:8.1f} "
+          f"us/call kernel {r['kernel_us']:8.1f} us {r['gbs']:7.0f} GB/s event-driven ({100 * r['frac']:.1f}%) "
+          f"{r['events_per_s']:.3e} events/s", file=sys.stderr, flush=True)
     bm.event.clear_plan_cache()
-    del xs
+    del xs, evs
     probe_out = torch.empty(32768 * 256, dtype=torch.float32, device=dev)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for k in range(13):

@@ -82,11 +82,11 @@ def _launch(d, idx, ptr, ev, shape, transpose, rule, host_w=None, use_plan=True)
     m, n = int(shape[0]), int(shape[1])
     if ptr.shape[0] != m + 1:
         raise ValueError(f'indptr should have shape ({m + 1},), but we got {tuple(ptr.shape)}.')
-    if transpose and use_plan:
-        # (the gather form keeps the CSR kernel with the events' bitmask in
-        # LDS: it reads whole rows, 163 µs on 1e8 synapses, against 239 µs
-        # for the plan's per-tile segments)
-        plan = _PLANS.lookup(d, idx, ptr, m, n, ev, host_w)
+    if use_plan:
+        # transpose: the plan of A scatters the active rows; the gather form
+        # runs on the plan of A^T, whose active rows (the events' columns)
+        # it scatters, reading only their entries instead of every row
+        plan = _PLANS.lookup(d, idx, ptr, m, n, ev, host_w, transposed=not transpose)
         if plan is not None:
             return plan._exec(ev, rule, None)
     out = torch.empty(n if transpose else m, dtype=acc_dtype(d.dtype), device=ev.device)
@@ -99,7 +99,9 @@ def _launch(d, idx, ptr, ev, shape, transpose, rule, host_w=None, use_plan=True)
 
 class _PlanCache:
     """Tiled plans (:class:`TiledCSR`) of the matrices that ``csrmv`` (event
-    or sparse, either orientation) is called with repeatedly.
+    or sparse, either orientation) is called with repeatedly; the event
+    gather (transpose=False) uses a plan of the transposed matrix instead,
+    a separate entry.
 
     The reference's layers call the operator statelessly every step with
     the same arrays (dnn/linear.py:630-634); the plan replaces one global
@@ -129,17 +131,19 @@ class _PlanCache:
         self.hits = 0
 
     @staticmethod
-    def _key(d, idx, ptr, m, n, host_w):
+    def _key(d, idx, ptr, m, n, host_w, transposed):
         dk = (("value", float(host_w)) if host_w is not None
               else (d.data_ptr(), d._version))
         return (dk, d.dtype, d.shape[0], idx.data_ptr(), idx._version, idx.dtype, idx.shape[0],
-                ptr.data_ptr(), ptr._version, m, n, d.device.index)
+                ptr.data_ptr(), ptr._version, m, n, d.device.index, transposed)
 
-    def lookup(self, d, idx, ptr, m, n, ev, host_w):
+    def lookup(self, d, idx, ptr, m, n, ev, host_w, transposed=False):
+        """The plan of A (m x n), or with ``transposed`` the plan of A^T
+        (n x m), whose scatter of active rows is the event gather A @ ev."""
         if not self.enabled or d.dtype not in (torch.float32, torch.bfloat16) or \
                 ev.dtype not in (torch.bool, torch.float32) or idx.shape[0] == 0:
             return None
-        key = self._key(d, idx, ptr, m, n, host_w)
+        key = self._key(d, idx, ptr, m, n, host_w, transposed)
         with self._lock:
             ent = self._plans.get(key)
             if ent is not None:
@@ -153,7 +157,11 @@ class _PlanCache:
                 return None
             del self._seen[key]
         try:
-            plan = TiledCSR(d, idx, ptr, shape=(m, n), weight_dtype=d.dtype if d.shape[0] > 1 else None)
+            if transposed:
+                dt, it, pt = _transpose_csr(d, idx, ptr, m, n)
+                plan = TiledCSR(dt, it, pt, shape=(n, m), weight_dtype=d.dtype if d.shape[0] > 1 else None)
+            else:
+                plan = TiledCSR(d, idx, ptr, shape=(m, n), weight_dtype=d.dtype if d.shape[0] > 1 else None)
         except _lib.BPXError as e:
             if "inconsistent shapes" not in str(e):
                 raise
@@ -187,6 +195,22 @@ class _PlanCache:
 
 
 _PLANS = _PlanCache()
+
+
+def _transpose_csr(d, idx, ptr, m, n):
+    """The CSR arrays of A^T (n x m) from those of A (m x n), on the device:
+    entries ordered by column (a stable sort, so each column's rows stay
+    ascending), A^T's indices the rows, its weights permuted alike."""
+    nnz = idx.shape[0]
+    counts = (ptr[1:] - ptr[:-1]).long()
+    rows = torch.repeat_interleave(torch.arange(m, device=idx.device, dtype=idx.dtype), counts,
+                                   output_size=nnz)
+    cols, order = torch.sort(idx, stable=True)
+    t_idx = rows[order]
+    t_d = d[order] if d.shape[0] > 1 else d
+    t_ptr = torch.zeros(n + 1, dtype=idx.dtype, device=idx.device)
+    t_ptr[1:] = torch.cumsum(torch.bincount(cols.long(), minlength=n), 0).to(idx.dtype)
+    return t_d.contiguous(), t_idx.contiguous(), t_ptr
 
 
 def plan_cache_info():
@@ -269,12 +293,13 @@ def _launch_batch(d, ev, idx, ptr, shape, transpose, rule):
     B = ev.shape[0]
     if ptr.shape[0] != m + 1:
         raise ValueError(f'indptr should have shape ({m + 1},), but we got {tuple(ptr.shape)}.')
-    if transpose and B >= 1:
-        plan = _PLANS.lookup(d, idx, ptr, m, n, ev, None)
+    if B >= 1:
+        plan = _PLANS.lookup(d, idx, ptr, m, n, ev, None, transposed=not transpose)
         if plan is None:
-            plan = _PLANS.lookup(d, idx, ptr, m, n, ev, None)     # a batch counts as repeated use
+            # a batch counts as repeated use
+            plan = _PLANS.lookup(d, idx, ptr, m, n, ev, None, transposed=not transpose)
         if plan is not None:
-            out = torch.empty((B, n), dtype=torch.float32, device=ev.device)
+            out = torch.empty((B, n if transpose else m), dtype=torch.float32, device=ev.device)
             return plan._exec_batch(ev, rule, B, False, out)
     L = _lib.lib()
     ev_len = m if transpose else n

@@ -223,7 +223,8 @@ def test_plan_gather_dense_and_events(wkind, shape):
 def test_public_gather_paths_run_on_the_plan():
     # bm.sparse.csrmv with transpose=False: the second call on the same
     # arrays builds the plan, later calls hit it; bm.event.csrmv's gather
-    # keeps its CSR kernel (no plan), exact integer counts either way
+    # builds the plan of the transposed matrix on its second call (a
+    # separate entry), exact integer counts either way
     idx, ptr, w = _inputs(seed=13)
     ti, tp = torch.from_numpy(idx).to(DEV), torch.from_numpy(ptr).to(DEV)
     tw = torch.from_numpy(w).to(DEV)
@@ -239,7 +240,7 @@ def test_public_gather_paths_run_on_the_plan():
     for _ in range(3):
         y = bm.event.csrmv(1.0, ti, tp, torch.from_numpy(ev).to(DEV), shape=(2000, 3000))
         np.testing.assert_array_equal(y.cpu().numpy(), ref)
-    assert bm.event.plan_cache_info()["builds"] == 1
+    assert bm.event.plan_cache_info()["builds"] == 2
     # the plan's event gather (TiledCSR.gather) gives the same integers
     plan = bm.event.TiledCSR(1.0, ti, tp, shape=(2000, 3000))
     np.testing.assert_array_equal(plan.gather(ev, event=True).cpu().numpy(), ref)
@@ -261,3 +262,63 @@ def test_plan_gather_nonfinite_vector_matches_stateless():
     fin = np.isfinite(y0)
     assert (~fin).any() and fin.any()
     np.testing.assert_allclose(y[fin], y0[fin], rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("shape", [(2000, 3000), (3000, 700), (1, 5000), (4000, 1)])
+@pytest.mark.parametrize("wkind", ["f32", "bf16", "homo"])
+def test_event_gather_runs_on_the_transposed_plan(shape, wkind):
+    # bm.event.csrmv(..., transpose=False) = A @ events: from the second call
+    # on the same arrays it scatters the active columns' entries on the plan
+    # of A^T (only their entries are read); against the FP64 dense product
+    # of the same (bf16-rounded) weights within 1e-5 of each output's sum of
+    # |terms|, for bool events, f32 events under the Numba rule (> 0, csrmv)
+    # and the Taichi rule (!= 0, csrmv_taichi), and a vmap over event vectors
+    m, n = shape
+    idx, ptr = fixed_prob_csr(m, n, 0.05 if n > 10 else 1.0, 21)
+    w = np.random.default_rng(22).normal(0.5, 0.3, idx.size).astype(np.float32)
+    if wkind == "bf16":
+        tw = torch.from_numpy(w).to(DEV).to(torch.bfloat16)
+        wref = tw.float().cpu().numpy()
+    elif wkind == "homo":
+        tw, wref = torch.tensor([0.75], device=DEV), np.full(idx.size, 0.75, np.float32)
+    else:
+        tw, wref = torch.from_numpy(w).to(DEV), w
+    ti, tp = torch.from_numpy(idx).to(DEV), torch.from_numpy(ptr).to(DEV)
+    D = dense_of(wref, idx, ptr, shape).astype(np.float64)
+    rng = np.random.default_rng(23)
+    for k in range(3):
+        evb = rng.random(n) < 0.2
+        evf = np.where(evb, rng.choice([-1.0, 0.5, 2.0], n), 0.0).astype(np.float32)
+        for y, on in ((bm.event.csrmv(tw, ti, tp, torch.from_numpy(evb).to(DEV), shape=shape), evb),
+                      (bm.event.csrmv(tw, ti, tp, torch.from_numpy(evf).to(DEV), shape=shape), evf > 0),
+                      (bm.event.csrmv_taichi(tw, ti, tp, torch.from_numpy(evf).to(DEV), shape=shape)[0], evf != 0)):
+            on = on.astype(np.float64)
+            assert y.shape == (m,)
+            assert np.all(np.abs(y.cpu().numpy() - D @ on) <= 1e-5 * (np.abs(D) @ on) + 1e-30)
+    info = bm.event.plan_cache_info()
+    assert info["builds"] == 1 and info["plans"] == 1 and info["hits"] >= 7
+    evs = torch.from_numpy(rng.random((5, n)) < 0.3).to(DEV)
+    ys = torch.func.vmap(lambda e: bm.event.csrmv(tw, ti, tp, e, shape=shape))(evs)
+    ref = evs.cpu().numpy().astype(np.float64) @ D.T
+    assert np.all(np.abs(ys.cpu().numpy() - ref) <= 1e-5 * (evs.cpu().numpy() @ np.abs(D).T) + 1e-30)
+    assert bm.event.plan_cache_info()["builds"] == 1
+
+
+def test_event_gather_transposed_plan_integer_exact_and_invalidated():
+    # integer-valued weights: every partial sum is exact, so the transposed
+    # plan's scatter equals the oracle's row gather bit for bit; an in-place
+    # write into the weights builds a new plan from the new values
+    idx, ptr = fixed_prob_csr(1500, 2500, 0.03, 31)
+    w = np.random.default_rng(32).integers(-4, 5, idx.size).astype(np.float32)
+    tw, ti, tp = (torch.from_numpy(a).to(DEV) for a in (w, idx, ptr))
+    ev = bernoulli(2500, 0.25, 33)
+    ref = orc.event_csrmv(w.astype(np.float64), idx, ptr, ev, (1500, 2500), False)
+    for _ in range(3):
+        y = bm.event.csrmv(tw, ti, tp, torch.from_numpy(ev).to(DEV), shape=(1500, 2500))
+        np.testing.assert_array_equal(y.cpu().numpy(), ref)
+    assert bm.event.plan_cache_info()["builds"] == 1
+    tw.mul_(2.0)
+    for _ in range(3):
+        y = bm.event.csrmv(tw, ti, tp, torch.from_numpy(ev).to(DEV), shape=(1500, 2500))
+        np.testing.assert_array_equal(y.cpu().numpy(), 2 * ref)
+    assert bm.event.plan_cache_info()["builds"] == 2
