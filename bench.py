@@ -244,6 +244,29 @@ def bench_coo(_lib, torch, steps=50, warmup=5):
     print("# coomv ceilings (probe): " + ", ".join(f"{k} {u:.1f} us" for k, u in ceil.items())
           + f"; coomv at {100 * res[0]['frac_of_read_gather_ceiling']:.0f}% of read+gather",
           file=sys.stderr, flush=True)
+    # the public bm.sparse.coomv on the same arrays: from its second call on
+    # them the matrix runs on a tiled plan (ordered by row); bytes as passed
+    import brainpy_amd.math as bm
+    for transpose in (0, 1):
+        for _ in range(2):
+            bm.sparse.coomv(w, row, idx, v, shape=(n, n), transpose=bool(transpose))
+        hits0 = bm.event.plan_cache_info()["hits"]
+
+        def pubcall():
+            bm.sparse.coomv(w, row, idx, v, shape=(n, n), transpose=bool(transpose))
+        wall, kern = _time_calls(_lib, torch, pubcall, _lib.KID["tiled"] if transpose else _lib.KID["tiled_gather"],
+                                 steps, warmup)
+        if bm.event.plan_cache_info()["hits"] - hits0 < steps:
+            raise RuntimeError("bm.sparse.coomv did not run on its cached plan")
+        alg = nse * 12 + n * 4 + n * 4
+        r = {"op": "bm.sparse.coomv (public path, cached plan)", "transpose": bool(transpose), "nse": nse,
+             "us_per_call": wall * 1e6, "kernel_us": kern * 1e6, "alg_bytes": alg, "gbs": alg / kern / 1e9,
+             "frac": alg / kern / 1e9 / HBM_PEAK_GBS, "entries_per_s": nse / wall}
+        res.append(r)
+        print(f"# bm.sparse.coomv transpose={transpose} (public path, cached plan) {r['us_per_call']:8.1f} us/call "
+              f"kernel {r['kernel_us']:8.1f} us {r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%)", file=sys.stderr,
+              flush=True)
+    bm.event.clear_plan_cache()
     del row, w, idx, ptr, probe_out
     idx, ptr = fixed_prob_csr_device(n, n, 0.001, seed=2025, device=dev)
     nse = idx.numel()

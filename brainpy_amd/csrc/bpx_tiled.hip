@@ -326,7 +326,8 @@ __global__ void k_init_minmax(uint32_t* p) {
 // fine enough when it is at most 2e-6 of the smallest non-zero term
 // |w|min (* |x|min for a dense x): each term's rounding then costs at most
 // 1e-6 of that term, so an output stays within 1e-6 of its sum of |terms|.
-// Wider ranges take the coarse path: direct FP32 atomics into out.
+// Wider ranges take the coarse path: FP32 sums in LDS, flushed with FP32
+// atomics into out.
 __device__ __forceinline__ bool grid_too_coarse(double fx_inv, float tmin) {
   return tmin <= 3.402823466e38f && fx_inv > 2e-6 * (double)tmin;
 }
@@ -378,8 +379,9 @@ __device__ __forceinline__ long long fixed_of(float w, double scale) {
 // output follows IEEE addition (inf + finite = inf, +inf + -inf = NaN).
 // `homo_w` points at the plan's scalars {homogeneous weight, max |w|,
 // min non-zero |w|} and `xmax` at x's {max finite |x|, min non-zero |x|}.
-// COARSE (events; the plan's flag): every term goes to out with an FP32
-// atomic; DENSE decides the same per call on the device.
+// COARSE (events; the plan's flag): every term is added in FP32 to the
+// copy's LDS sums (the same LDS read as floats), which the flush adds to out
+// with FP32 atomics; DENSE decides the same per call on the device.
 // LISTED: the active rows were listed once for the call (k_tiled_list:
 // lcount[v] rows of vector v in lrows[v][...], any order), and the waves walk
 // that list instead of scanning every event of their copy's rows: with many
@@ -589,9 +591,13 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
           for (int u = 0; u < kVec; ++u) {
             const uint32_t c = (l4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
             if (c < (uint32_t)tlen && (COARSE || (DENSE && coarse))) {
-              // coarse grid: the term goes straight to its output
-              if constexpr (DENSE) atomic_add(out + t0 + c, wv[q][u] * xr[q]);
-              else if constexpr (WK != 0) atomic_add(out + t0 + c, wv[q][u]);
+              // coarse grid: the term is added in FP32 to the copy's LDS sum
+              // (ds_add_f32); the flush adds that sum to out
+              float* s_accf = reinterpret_cast<float*>(s_acc);
+              if constexpr (DENSE)
+                __hip_atomic_fetch_add(s_accf + c, wv[q][u] * xr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              else if constexpr (WK != 0)
+                __hip_atomic_fetch_add(s_accf + c, wv[q][u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             } else if (c < (uint32_t)tlen) {
               long long fx;
               bool special = false;
@@ -627,6 +633,15 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
     q0 = qn;
   }
   __syncthreads();
+  if (COARSE || (DENSE && coarse)) {
+    // FP32 sums (NaN != 0 too, so NaN and inf sums are added)
+    const float* s_accf = reinterpret_cast<const float*>(s_acc);
+    for (int k = threadIdx.x; k < tlen; k += kTiledThreads) {
+      const float v = s_accf[k];
+      if (v != 0.f) atomic_add(out + t0 + k, v);
+    }
+    return;
+  }
   for (int k = threadIdx.x; k < tlen; k += kTiledThreads) {
     const long long a = (long long)s_acc[k];
     uint32_t f = 0;

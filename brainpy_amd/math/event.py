@@ -137,13 +137,15 @@ class _PlanCache:
         return (dk, d.dtype, d.shape[0], idx.data_ptr(), idx._version, idx.dtype, idx.shape[0],
                 ptr.data_ptr(), ptr._version, m, n, d.device.index, transposed)
 
-    def lookup(self, d, idx, ptr, m, n, ev, host_w, transposed=False):
+    def lookup(self, d, idx, ptr, m, n, ev, host_w, transposed=False, coo=False):
         """The plan of A (m x n), or with ``transposed`` the plan of A^T
-        (n x m), whose scatter of active rows is the event gather A @ ev."""
+        (n x m), whose scatter of active rows is the event gather A @ ev.
+        With ``coo``, ``ptr`` and ``idx`` are the row and column arrays of a
+        COO matrix, ordered by row into CSR for the plan."""
         if not self.enabled or d.dtype not in (torch.float32, torch.bfloat16) or \
                 ev.dtype not in (torch.bool, torch.float32) or idx.shape[0] == 0:
             return None
-        key = self._key(d, idx, ptr, m, n, host_w, transposed)
+        key = self._key(d, idx, ptr, m, n, host_w, ("coo" if coo else "csr", transposed))
         with self._lock:
             ent = self._plans.get(key)
             if ent is not None:
@@ -157,7 +159,11 @@ class _PlanCache:
                 return None
             del self._seen[key]
         try:
-            if transposed:
+            if coo:
+                csr = _coo_to_csr(d, ptr, idx, m)
+                plan = None if csr is None else TiledCSR(
+                    *csr, shape=(m, n), weight_dtype=d.dtype if d.shape[0] > 1 else None)
+            elif transposed:
                 dt, it, pt = _transpose_csr(d, idx, ptr, m, n)
                 plan = TiledCSR(dt, it, pt, shape=(n, m), weight_dtype=d.dtype if d.shape[0] > 1 else None)
             else:
@@ -195,6 +201,18 @@ class _PlanCache:
 
 
 _PLANS = _PlanCache()
+
+
+def _coo_to_csr(d, row, col, m):
+    """CSR arrays (data, indices, indptr) of an m-row COO matrix, entries
+    ordered by row (a stable sort, so each row keeps its entries' order);
+    None when a row index lies outside [0, m)."""
+    if int(row.min()) < 0 or int(row.max()) >= m:
+        return None
+    rows, order = torch.sort(row, stable=True)
+    ptr = torch.zeros(m + 1, dtype=col.dtype, device=col.device)
+    ptr[1:] = torch.cumsum(torch.bincount(rows.long(), minlength=m), 0).to(col.dtype)
+    return (d[order] if d.shape[0] > 1 else d).contiguous(), col[order].contiguous(), ptr
 
 
 def _transpose_csr(d, idx, ptr, m, n):

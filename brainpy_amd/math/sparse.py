@@ -213,6 +213,22 @@ _COO_DTYPES = (torch.float32, torch.float64, torch.int32)
 def _coomv_launch(d, row, col, v, shape, transpose):
     d, row, col, v = d.contiguous(), row.contiguous(), col.contiguous(), v.contiguous()
     m, n = int(shape[0]), int(shape[1])
+    if d.dtype == torch.float32 and v.dtype == torch.float32 and row.shape[0] > 0:
+        # repeated calls on the same arrays run on a tiled plan of the matrix
+        # ordered by row (shared cache with csrmv): the transpose sums its
+        # products in LDS, the gather stages column tiles of v in LDS
+        plan = _PLANS.lookup(d, col, row, m, n, v, None, coo=True)
+        if plan is not None:
+            if transpose:
+                return plan.matvec(v)
+            if plan.plan.tiles <= 64:
+                return plan._gather(v, False, _lib.RULE_POSITIVE)
+    return _coomv_stateless(d, row, col, v, shape, transpose)
+
+
+def _coomv_stateless(d, row, col, v, shape, transpose):
+    d, row, col, v = d.contiguous(), row.contiguous(), col.contiguous(), v.contiguous()
+    m, n = int(shape[0]), int(shape[1])
     out = torch.empty(n if transpose else m, dtype=d.dtype, device=v.device)
     rc = _lib.lib().bpx_coomv(_lib.stream_ptr(), _lib.ptr(d), _lib.bpx_dtype(d), d.shape[0], _lib.ptr(row),
                               _lib.ptr(col), _lib.bpx_dtype(row), row.shape[0], _lib.ptr(v), _lib.bpx_dtype(v), m, n,

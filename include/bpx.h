@@ -181,7 +181,8 @@ typedef struct {
   int32_t weight_dtype; /* BPX_F32, BPX_BF16 or BPX_TILED_HOMO */
   int32_t fx_shift;     /* LDS accumulators hold sum(w) * 2^fx_shift as int64 */
   int32_t coarse;       /* 1 when 2^-fx_shift > 2e-6 * min non-zero |w| (weights spanning ~2^40 / m or more):
-                           calls add every term to out with an FP32 atomic instead (set by build) */
+                           calls add the terms in FP32 in each copy's LDS instead, then to out with FP32
+                           atomics (set by build) */
   int32_t reserved;
   int64_t capacity;     /* element slots incl. padding (segments padded to 4) */
   int64_t rowtile_offset, aux_offset, lidx_offset, weights_offset, homo_offset; /* byte offsets in mem */
@@ -222,9 +223,9 @@ int bpx_event_csrmv_tiled_ws(void* stream, const bpx_tiled_csr* plan, const void
  * inf or NaN reach their outputs as IEEE addition would (an infinity, or NaN
  * for a NaN term or infinities of both signs).  When the fixed-point grid is
  * coarser than 2e-6 of the smallest non-zero term (min |w| * min |x|: values
- * spanning more than ~2^40 / m), the terms are added with FP32 atomics
- * instead, so every output stays within 1e-6 of its sum of |terms| plus the
- * FP32 rounding of the sum.  bpx_event_csrmv_tiled does the same with min |w|.
+ * spanning more than ~2^40 / m), the terms are added in FP32 instead (LDS
+ * atomics into each output copy, whose sums go to out with FP32 atomics), so
+ * an output is an FP32 sum of its terms in some order.  bpx_event_csrmv_tiled does the same with min |w|.
  */
 int bpx_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* x, int x_dtype, int accumulate, float* out,
                     void* workspace, size_t workspace_bytes);

@@ -322,3 +322,46 @@ def test_event_gather_transposed_plan_integer_exact_and_invalidated():
         y = bm.event.csrmv(tw, ti, tp, torch.from_numpy(ev).to(DEV), shape=(1500, 2500))
         np.testing.assert_array_equal(y.cpu().numpy(), 2 * ref)
     assert bm.event.plan_cache_info()["builds"] == 2
+
+
+@pytest.mark.parametrize("shape", [(2000, 3000), (3000, 700)])
+@pytest.mark.parametrize("homo", [False, True])
+def test_coomv_runs_on_the_plan(shape, homo):
+    # bm.sparse.coomv: from the second call on the same (data, row, col) the
+    # matrix, ordered by row, runs on a tiled plan (both orientations);
+    # unsorted entries with duplicates, against the FP64 dense product
+    # within 1e-5 of each output's sum of |terms|
+    m, n = shape
+    rng = np.random.default_rng(41)
+    nse = m * n // 40
+    row = rng.integers(0, m, nse).astype(np.int32)
+    col = rng.integers(0, n, nse).astype(np.int32)
+    row[:50], col[:50] = row[50:100], col[50:100]          # duplicated (row, col) pairs
+    data = np.float32([0.6]) if homo else rng.normal(0.2, 1.0, nse).astype(np.float32)
+    D = np.zeros(shape, np.float64)
+    np.add.at(D, (row, col), np.broadcast_to(data.astype(np.float64), row.shape))
+    td, tr, tc = (torch.from_numpy(a).to(DEV) for a in (data, row, col))
+    for k in range(3):
+        for transpose in (False, True):
+            x = rng.standard_normal(m if transpose else n).astype(np.float32)
+            y = bm.sparse.coomv(td, tr, tc, torch.from_numpy(x).to(DEV), shape=shape, transpose=transpose)
+            A = D.T if transpose else D
+            assert np.all(np.abs(y.cpu().numpy() - A @ x) <= 1e-5 * (np.abs(A) @ np.abs(x)) + 1e-30)
+    info = bm.event.plan_cache_info()
+    assert info["builds"] == 1 and info["plans"] == 1 and info["hits"] == 4
+
+
+def test_coomv_rows_outside_the_matrix_stay_stateless():
+    # a row index outside [0, m) leaves the matrix to the stateless kernel
+    # (no plan: its CSR would not have m rows); the results agree with it
+    rng = np.random.default_rng(43)
+    row = rng.integers(0, 100, 500).astype(np.int32)
+    row[7] = 100
+    col = rng.integers(0, 80, 500).astype(np.int32)
+    data = rng.normal(size=500).astype(np.float32)
+    td, tr, tc = (torch.from_numpy(a).to(DEV) for a in (data, row, col))
+    x = torch.from_numpy(rng.standard_normal(80).astype(np.float32)).to(DEV)
+    ys = [bm.sparse.coomv(td, tr, tc, x, shape=(100, 80)).cpu().numpy() for _ in range(3)]
+    info = bm.event.plan_cache_info()
+    assert info["plans"] == 0 and info["ineligible"] == 1
+    np.testing.assert_allclose(ys[1], ys[0], rtol=1e-6, atol=1e-6)
