@@ -322,15 +322,14 @@ __global__ void k_init_minmax(uint32_t* p) {
   }
 }
 
-// The fixed-point grid (fx_inv, one unit in weight or product units) is
-// fine enough when it is at most 2e-6 of the smallest non-zero term
-// |w|min (* |x|min for a dense x): each term's rounding then costs at most
-// 1e-6 of that term, so an output stays within 1e-6 of its sum of |terms|.
-// Wider ranges take the coarse path: FP32 sums in LDS, flushed with FP32
-// atomics into out.
-__device__ __forceinline__ bool grid_too_coarse(double fx_inv, float tmin) {
-  return tmin <= 3.402823466e38f && fx_inv > 2e-6 * (double)tmin;
-}
+// The fixed-point grid (fx_inv, one unit in weight or product units)
+// rounds a term to within 1e-6 of itself when the term is at least 5e5
+// units.  Smaller terms (and, for a dense x, products that are inf or NaN)
+// are not put on the grid: they are added to their output directly with an
+// FP32 atomic.  For events the plan's flag says whether any weight is that
+// small (the COARSE kernels test each weight); for a dense x every product
+// is tested.
+__device__ __forceinline__ float grid_floor(double fx_inv) { return (float)(5e5 * fx_inv); }
 
 __global__ void k_store_scalar(const void* src, int src_dtype, float* dst) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -373,15 +372,14 @@ __device__ __forceinline__ long long fixed_of(float w, double scale) {
 // is multiplied by it (one f32 rounding, as the plain product), and the
 // fixed-point shift comes from the plan's less ceil(log2 max|x|) over the
 // finite x (device scalar xmax: the f32 bits of that max), so the sums still
-// stay below 2^61.  A product that is inf or NaN (non-finite x, or an f32
-// overflow) is not summed in fixed point: it sets a flag of its output (+inf,
-// -inf, NaN) in LDS, and the flush adds that special value instead, so the
-// output follows IEEE addition (inf + finite = inf, +inf + -inf = NaN).
-// `homo_w` points at the plan's scalars {homogeneous weight, max |w|,
-// min non-zero |w|} and `xmax` at x's {max finite |x|, min non-zero |x|}.
-// COARSE (events; the plan's flag): every term is added in FP32 to the
-// copy's LDS sums (the same LDS read as floats), which the flush adds to out
-// with FP32 atomics; DENSE decides the same per call on the device.
+// stay below 2^61.  Products below the grid floor, and products that are inf
+// or NaN (non-finite x, or an f32 overflow), go to their outputs with FP32
+// atomics, so an output follows IEEE addition (inf + finite = inf, +inf +
+// -inf = NaN).  `homo_w` points at the plan's scalars {homogeneous weight,
+// max |w|, min non-zero |w|} and `xmax` at x's {max finite |x|, min non-zero
+// |x|}.
+// COARSE (events; the plan's flag): weights below the grid floor go to out
+// with FP32 atomics, the others to the fixed-point sums.
 // LISTED: the active rows were listed once for the call (k_tiled_list:
 // lcount[v] rows of vector v in lrows[v][...], any order), and the waves walk
 // that list instead of scanning every event of their copy's rows: with many
@@ -401,12 +399,8 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
   extern __shared__ unsigned long long s_acc[];
   __shared__ int2 s_seg[kWavesPerBlock][kListCap];
   __shared__ float s_xv[DENSE ? kWavesPerBlock : 1][DENSE ? kListCap : 1];
-  // DENSE: non-finite flags, 8 bits per output (1 +inf, 2 -inf, 4 NaN)
-  uint32_t* s_flag = reinterpret_cast<uint32_t*>(s_acc + T);
-  bool coarse = COARSE;
   if constexpr (DENSE) {
     const float mx = __uint_as_float(xmax[0]);
-    const float tmin = __uint_as_float(reinterpret_cast<const uint32_t*>(homo_w)[2]) * __uint_as_float(xmax[1]);
     int ex = 0;
     if (mx > 0.f && mx <= 3.402823466e38f) frexpf(mx, &ex);   // mx < 2^ex
     // products |w * x| < max|w| * 2^ex: scaling by 2^(shift - ex) keeps every
@@ -416,8 +410,8 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
       fx_scale = ldexp(fx_scale, -ex);
       fx_inv = ldexp(fx_inv, ex);
     }
-    coarse = WK != 0 && grid_too_coarse(fx_inv, tmin);
   }
+  const float floor_term = grid_floor(fx_inv);
   // a batch of event vectors: vector blockIdx.y of [B][m] into [B][n]
   ev += (int64_t)blockIdx.y * m;
   out += (int64_t)blockIdx.y * n;
@@ -441,8 +435,6 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
   const int64_t t0 = (int64_t)p * T;
   const int tlen = (int)((n - t0) < (int64_t)T ? (n - t0) : (int64_t)T);
   for (int k = threadIdx.x; k < T; k += kTiledThreads) s_acc[k] = 0ull;
-  if constexpr (DENSE)
-    for (int k = threadIdx.x; k < (T + 3) / 4; k += kTiledThreads) s_flag[k] = 0u;
   __syncthreads();
   float hw = 0.f;
   if constexpr (WK == 0) hw = *homo_w;
@@ -590,34 +582,25 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
 #pragma unroll
           for (int u = 0; u < kVec; ++u) {
             const uint32_t c = (l4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
-            if (c < (uint32_t)tlen && (COARSE || (DENSE && coarse))) {
-              // coarse grid: the term is added in FP32 to the copy's LDS sum
-              // (ds_add_f32); the flush adds that sum to out
-              float* s_accf = reinterpret_cast<float*>(s_acc);
-              if constexpr (DENSE)
-                __hip_atomic_fetch_add(s_accf + c, wv[q][u] * xr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-              else if constexpr (WK != 0)
-                __hip_atomic_fetch_add(s_accf + c, wv[q][u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            } else if (c < (uint32_t)tlen) {
-              long long fx;
-              bool special = false;
-              if constexpr (DENSE) {
-                const float prod = (WK == 0) ? hw * xr[q] : wv[q][u] * xr[q];
-                special = !(fabsf(prod) <= 3.402823466e38f);
-                if (special) {
-                  const uint32_t bits = prod != prod ? 4u : (prod > 0.f ? 1u : 2u);
-                  __hip_atomic_fetch_or(s_flag + (c >> 2), bits << (8 * (c & 3)), __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (c < (uint32_t)tlen) {
+              if constexpr (DENSE || COARSE) {
+                // on the grid when at least its floor; else (small, inf or
+                // NaN) an FP32 atomic to the output
+                float t = (WK == 0) ? hw : wv[q][u];
+                if constexpr (DENSE) t *= xr[q];
+                const float at = fabsf(t);
+                if (at >= floor_term && at <= 3.402823466e38f) {
+                  const long long fx = (WK == 0 && DENSE) ? hq[q] : (WK == 0) ? hfx : fixed_of(t, fx_scale);
+                  __hip_atomic_fetch_add(s_acc + c, (unsigned long long)fx, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else if (t != 0.f) {
+                  atomic_add(out + t0 + c, t);
                 }
-                fx = (WK == 0) ? hq[q] : fixed_of(prod, fx_scale);
-              } else if constexpr (WK == 0) {
-                fx = hq[q];
               } else {
-                fx = fixed_of(wv[q][u], fx_scale);
-              }
-              if (!special)
+                const long long fx = (WK == 0) ? hfx : fixed_of(wv[q][u], fx_scale);
                 __hip_atomic_fetch_add(s_acc + c, (unsigned long long)fx, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+              }
             }
           }
         }
@@ -633,27 +616,9 @@ __global__ __launch_bounds__(kTiledThreads) void k_tiled_scatter(const int32_t* 
     q0 = qn;
   }
   __syncthreads();
-  if (COARSE || (DENSE && coarse)) {
-    // FP32 sums (NaN != 0 too, so NaN and inf sums are added)
-    const float* s_accf = reinterpret_cast<const float*>(s_acc);
-    for (int k = threadIdx.x; k < tlen; k += kTiledThreads) {
-      const float v = s_accf[k];
-      if (v != 0.f) atomic_add(out + t0 + k, v);
-    }
-    return;
-  }
   for (int k = threadIdx.x; k < tlen; k += kTiledThreads) {
     const long long a = (long long)s_acc[k];
-    uint32_t f = 0;
-    if constexpr (DENSE) f = (s_flag[k >> 2] >> (8 * (k & 3))) & 7u;
-    if (f != 0) {
-      // +inf and -inf together, or a NaN term: NaN; else the one infinity
-      const float sv = ((f & 4u) || f == 3u) ? __uint_as_float(0x7FC00000u)
-                                               : __uint_as_float((f & 1u) ? 0x7F800000u : 0xFF800000u);
-      atomic_add(out + t0 + k, sv);
-    } else if (a != 0) {
-      atomic_add(out + t0 + k, (float)((double)a * fx_inv));
-    }
+    if (a != 0) atomic_add(out + t0 + k, (float)((double)a * fx_inv));
   }
 }
 
@@ -1374,8 +1339,7 @@ int bpx_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* x, int 
   const int P = plan->tiles, T = plan->tile_len;
   const int ncopy = tiled_copies(P);
   const unsigned blocks = (unsigned)(ncopy * P);
-  // the accumulators, then the outputs' non-finite flags (one byte each)
-  const size_t shm = (size_t)T * sizeof(unsigned long long) + (size_t)((T + 3) / 4) * sizeof(uint32_t);
+  const size_t shm = (size_t)T * sizeof(unsigned long long);
   const double fx_scale = ldexp(1.0, plan->fx_shift), fx_inv = ldexp(1.0, -plan->fx_shift);
   if (listed) {
     const int64_t lb = (m + kListRows - 1) / kListRows;

@@ -181,8 +181,8 @@ typedef struct {
   int32_t weight_dtype; /* BPX_F32, BPX_BF16 or BPX_TILED_HOMO */
   int32_t fx_shift;     /* LDS accumulators hold sum(w) * 2^fx_shift as int64 */
   int32_t coarse;       /* 1 when 2^-fx_shift > 2e-6 * min non-zero |w| (weights spanning ~2^40 / m or more):
-                           calls add the terms in FP32 in each copy's LDS instead, then to out with FP32
-                           atomics (set by build) */
+                           calls then add each weight below 5e5 grid units to its output with an FP32
+                           atomic instead of the fixed-point sum (set by build) */
   int32_t reserved;
   int64_t capacity;     /* element slots incl. padding (segments padded to 4) */
   int64_t rowtile_offset, aux_offset, lidx_offset, weights_offset, homo_offset; /* byte offsets in mem */
@@ -219,13 +219,13 @@ int bpx_event_csrmv_tiled_ws(void* stream, const bpx_tiled_csr* plan, const void
  * rounded to f32 and summed exactly in fixed point (shift lowered by
  * ceil(log2 max|x|) over the finite x, computed on the device: workspace >= 8
  * bytes, or 256 + bpx_tiled_csr_call_workspace(plan, 1) bytes to list the
- * rows with x != 0 once; calls that share a workspace must not overlap).  Products that are
- * inf or NaN reach their outputs as IEEE addition would (an infinity, or NaN
- * for a NaN term or infinities of both signs).  When the fixed-point grid is
- * coarser than 2e-6 of the smallest non-zero term (min |w| * min |x|: values
- * spanning more than ~2^40 / m), the terms are added in FP32 instead (LDS
- * atomics into each output copy, whose sums go to out with FP32 atomics), so
- * an output is an FP32 sum of its terms in some order.  bpx_event_csrmv_tiled does the same with min |w|.
+ * rows with x != 0 once; calls that share a workspace must not overlap).
+ * A product below 5e5 units of that grid (where rounding to the grid would
+ * cost more than 1e-6 of it), and one that is inf or NaN, is added to its
+ * output with an FP32 atomic instead, so tiny terms keep f32 accuracy and
+ * non-finite ones reach their outputs as IEEE addition would (an infinity,
+ * or NaN for a NaN term or infinities of both signs).  bpx_event_csrmv_tiled
+ * does the same for weights below the grid's floor (the plan's coarse flag).
  */
 int bpx_csrmv_tiled(void* stream, const bpx_tiled_csr* plan, const void* x, int x_dtype, int accumulate, float* out,
                     void* workspace, size_t workspace_bytes);

@@ -459,9 +459,9 @@ def bm_connect_csr(m, n, p, seed, sort):
 
 def test_tiled_plan_wide_value_ranges_stay_accurate():
     # weights or vector values spanning far more than the fixed-point grid
-    # resolves: the plan switches to direct FP32 atomics, so outputs fed only
-    # by tiny terms keep 1e-5 of their own sum of |terms| (on the grid alone
-    # they would round to a few units)
+    # resolves: terms below the grid's floor go to their outputs with FP32
+    # atomics, so outputs fed only by tiny terms keep 1e-5 of their own sum
+    # of |terms| (on the grid alone they would round to a few units)
     m, n = 2000, 3000
     indices, indptr = bm_connect_csr(m, n, 0.01, 31, True)
     rng = np.random.default_rng(12)
@@ -488,3 +488,18 @@ def test_tiled_plan_wide_value_ranges_stay_accurate():
     assert np.all(np.abs(y2 - ref2) <= 1e-5 * mag2 + 1e-30)
     small = mag2 < 1e6
     assert small.sum() > n // 2 and np.abs(y2[small]).max() > 0
+    # huge and tiny entries only, f32 and homogeneous weights: the outputs
+    # that no huge row reaches are sums of products far below the floor of a
+    # grid scaled for 1e20, and keep their own 1e-5
+    x3 = np.zeros(m, np.float32)
+    x3[:10] = np.float32(1e20)
+    x3[10:20] = np.float32(-1e-30)
+    for plan3, D3 in ((plan2, D2),
+                      (bm.event.TiledCSR(np.float32(0.7), indices, indptr, shape=(m, n)),
+                       dense_of(np.float32(0.7), indices, indptr, (m, n)).astype(np.float64))):
+        y3 = plan3.matvec(x3).cpu().numpy().astype(np.float64)
+        ref3 = x3.astype(np.float64) @ D3
+        mag3 = np.abs(x3.astype(np.float64)) @ np.abs(D3)
+        assert np.all(np.abs(y3 - ref3) <= 1e-5 * mag3 + 1e-45)
+        only_tiny = (mag3 > 0) & (mag3 < 1.0)
+        assert only_tiny.sum() > 50 and np.all(y3[only_tiny] < 0)
