@@ -160,7 +160,7 @@ class _PlanCache:
             del self._seen[key]
         try:
             if coo:
-                csr = _coo_to_csr(d, ptr, idx, m)
+                csr = _coo_to_csr(d, ptr, idx, m, n)
                 plan = None if csr is None else TiledCSR(
                     *csr, shape=(m, n), weight_dtype=d.dtype if d.shape[0] > 1 else None)
             elif transposed:
@@ -203,11 +203,13 @@ class _PlanCache:
 _PLANS = _PlanCache()
 
 
-def _coo_to_csr(d, row, col, m):
-    """CSR arrays (data, indices, indptr) of an m-row COO matrix, entries
+def _coo_to_csr(d, row, col, m, n):
+    """CSR arrays (data, indices, indptr) of an m x n COO matrix, entries
     ordered by row (a stable sort, so each row keeps its entries' order);
-    None when a row index lies outside [0, m)."""
-    if int(row.min()) < 0 or int(row.max()) >= m:
+    None when a row index lies outside [0, m) or a column index outside
+    [0, n) (the stateless COO kernel keeps JAX's semantics for those: output
+    keys out of shape dropped, gather indices clamped)."""
+    if int(row.min()) < 0 or int(row.max()) >= m or int(col.min()) < 0 or int(col.max()) >= n:
         return None
     rows, order = torch.sort(row, stable=True)
     ptr = torch.zeros(m + 1, dtype=col.dtype, device=col.device)

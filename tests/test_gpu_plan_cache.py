@@ -351,17 +351,27 @@ def test_coomv_runs_on_the_plan(shape, homo):
     assert info["builds"] == 1 and info["plans"] == 1 and info["hits"] == 4
 
 
-def test_coomv_rows_outside_the_matrix_stay_stateless():
-    # a row index outside [0, m) leaves the matrix to the stateless kernel
-    # (no plan: its CSR would not have m rows); the results agree with it
+@pytest.mark.parametrize("bad", ["row", "col", "negative col"])
+@pytest.mark.parametrize("transpose", [False, True])
+def test_coomv_keys_outside_the_matrix_stay_stateless(bad, transpose):
+    # a row index outside [0, m) or a column index outside [0, n) leaves the
+    # matrix to the stateless kernel (no plan: the plan would drop the entry,
+    # where the gather clamps its column as JAX does); the results agree
+    # with the first (stateless) call bit for bit
     rng = np.random.default_rng(43)
     row = rng.integers(0, 100, 500).astype(np.int32)
-    row[7] = 100
     col = rng.integers(0, 80, 500).astype(np.int32)
+    if bad == "row":
+        row[7] = 100
+    elif bad == "col":
+        col[7] = 80
+    else:
+        col[7] = -1
     data = rng.normal(size=500).astype(np.float32)
     td, tr, tc = (torch.from_numpy(a).to(DEV) for a in (data, row, col))
-    x = torch.from_numpy(rng.standard_normal(80).astype(np.float32)).to(DEV)
-    ys = [bm.sparse.coomv(td, tr, tc, x, shape=(100, 80)).cpu().numpy() for _ in range(3)]
+    x = torch.from_numpy(rng.standard_normal(100 if transpose else 80).astype(np.float32)).to(DEV)
+    ys = [bm.sparse.coomv(td, tr, tc, x, shape=(100, 80), transpose=transpose).cpu().numpy() for _ in range(3)]
     info = bm.event.plan_cache_info()
     assert info["plans"] == 0 and info["ineligible"] == 1
-    np.testing.assert_allclose(ys[1], ys[0], rtol=1e-6, atol=1e-6)
+    np.testing.assert_array_equal(ys[1], ys[0])
+    np.testing.assert_array_equal(ys[2], ys[0])
