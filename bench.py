@@ -248,8 +248,14 @@ def bench_coo(_lib, torch, steps=50, warmup=5):
     # them the matrix runs on a tiled plan (ordered by row); bytes as passed
     import brainpy_amd.math as bm
     for transpose in (0, 1):
-        for _ in range(2):
-            bm.sparse.coomv(w, row, idx, v, shape=(n, n), transpose=bool(transpose))
+        # first call stateless; the second builds the plan (transpose=0) and
+        # runs on it, timed as the plan build (sort by row, offsets, tiling)
+        bm.sparse.coomv(w, row, idx, v, shape=(n, n), transpose=bool(transpose))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bm.sparse.coomv(w, row, idx, v, shape=(n, n), transpose=bool(transpose))
+        torch.cuda.synchronize()
+        second_ms = (time.perf_counter() - t0) * 1e3
         hits0 = bm.event.plan_cache_info()["hits"]
 
         def pubcall():
@@ -262,9 +268,12 @@ def bench_coo(_lib, torch, steps=50, warmup=5):
         r = {"op": "bm.sparse.coomv (public path, cached plan)", "transpose": bool(transpose), "nse": nse,
              "us_per_call": wall * 1e6, "kernel_us": kern * 1e6, "alg_bytes": alg, "gbs": alg / kern / 1e9,
              "frac": alg / kern / 1e9 / HBM_PEAK_GBS, "entries_per_s": nse / wall}
+        if transpose == 0:
+            r["plan_build_ms"] = second_ms
         res.append(r)
         print(f"# bm.sparse.coomv transpose={transpose} (public path, cached plan) {r['us_per_call']:8.1f} us/call "
-              f"kernel {r['kernel_us']:8.1f} us {r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%)", file=sys.stderr,
+              f"kernel {r['kernel_us']:8.1f} us {r['gbs']:7.0f} GB/s ({100 * r['frac']:.1f}%)"
+              + (f"; plan built in the second call, {second_ms:.1f} ms" if transpose == 0 else ""), file=sys.stderr,
               flush=True)
     bm.event.clear_plan_cache()
     del row, w, idx, ptr, probe_out
@@ -390,8 +399,12 @@ def bench_forms(_lib, torch, steps=30, warmup=3):
     # 1 per event, an 8-byte offset pair per active column, 8 per synaptic
     # event and 4 per output
     evs = [torch.rand(n, generator=g, device=dev) < 0.1 for _ in range(16)]
-    for k in range(2):
-        bm.event.csrmv(w, idx, ptr, evs[k], shape=(n, n))
+    bm.event.csrmv(w, idx, ptr, evs[0], shape=(n, n))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bm.event.csrmv(w, idx, ptr, evs[1], shape=(n, n))      # builds the plan of A^T
+    torch.cuda.synchronize()
+    build_t_ms = (time.perf_counter() - t0) * 1e3
     hits0 = bm.event.plan_cache_info()["hits"]
     cnt = [0]
 
@@ -407,11 +420,13 @@ def bench_forms(_lib, torch, steps=30, warmup=3):
     r = {"op": "bm.event.csrmv transpose=0 (public path, cached plan of A^T)", "transpose": False,
          "events": "bool", "nnz": nnz, "active_columns": cols, "events_per_call": events,
          "us_per_call": wall * 1e6, "kernel_us": kern * 1e6, "alg_bytes_event_driven": alg,
-         "gbs": alg / kern / 1e9, "frac": alg / kern / 1e9 / HBM_PEAK_GBS, "events_per_s": events / wall}
+         "gbs": alg / kern / 1e9, "frac": alg / kern / 1e9 / HBM_PEAK_GBS, "events_per_s": events / wall,
+         "plan_build_ms": build_t_ms}
     res.append(r)
     print(f"# bm.event.csrmv transpose=0 (public path, cached plan of A^T, 16 event vectors) {r['us_per_call']:8.1f} "
           f"us/call kernel {r['kernel_us']:8.1f} us {r['gbs']:7.0f} GB/s event-driven ({100 * r['frac']:.1f}%) "
-          f"{r['events_per_s']:.3e} events/s", file=sys.stderr, flush=True)
+          f"{r['events_per_s']:.3e} events/s; plan of A^T built in the second call, {build_t_ms:.1f} ms",
+          file=sys.stderr, flush=True)
     bm.event.clear_plan_cache()
     del xs, evs
     probe_out = torch.empty(32768 * 256, dtype=torch.float32, device=dev)

@@ -164,8 +164,9 @@ class _PlanCache:
                 plan = None if csr is None else TiledCSR(
                     *csr, shape=(m, n), weight_dtype=d.dtype if d.shape[0] > 1 else None)
             elif transposed:
-                dt, it, pt = _transpose_csr(d, idx, ptr, m, n)
-                plan = TiledCSR(dt, it, pt, shape=(n, m), weight_dtype=d.dtype if d.shape[0] > 1 else None)
+                tr = _transpose_csr(d, idx, ptr, m, n)
+                plan = None if tr is None else TiledCSR(
+                    *tr, shape=(n, m), weight_dtype=d.dtype if d.shape[0] > 1 else None)
             else:
                 plan = TiledCSR(d, idx, ptr, shape=(m, n), weight_dtype=d.dtype if d.shape[0] > 1 else None)
         except _lib.BPXError as e:
@@ -212,25 +213,33 @@ def _coo_to_csr(d, row, col, m, n):
     if int(row.min()) < 0 or int(row.max()) >= m or int(col.min()) < 0 or int(col.max()) >= n:
         return None
     rows, order = torch.sort(row, stable=True)
-    ptr = torch.zeros(m + 1, dtype=col.dtype, device=col.device)
-    ptr[1:] = torch.cumsum(torch.bincount(rows.long(), minlength=m), 0).to(col.dtype)
-    return (d[order] if d.shape[0] > 1 else d).contiguous(), col[order].contiguous(), ptr
+    return (d[order] if d.shape[0] > 1 else d).contiguous(), col[order].contiguous(), _sorted_offsets(rows, m,
+                                                                                                   col.dtype)
+
+
+def _sorted_offsets(keys, k, dtype):
+    """Offsets (k + 1) of the runs of the sorted keys 0..k-1: entry r is the
+    number of keys below r (one binary search per offset)."""
+    bounds = torch.arange(k + 1, device=keys.device, dtype=keys.dtype)
+    return torch.searchsorted(keys, bounds, out_int32=dtype == torch.int32).to(dtype)
 
 
 def _transpose_csr(d, idx, ptr, m, n):
     """The CSR arrays of A^T (n x m) from those of A (m x n), on the device:
     entries ordered by column (a stable sort, so each column's rows stay
-    ascending), A^T's indices the rows, its weights permuted alike."""
+    ascending), A^T's indices the rows, its weights permuted alike; None
+    when an index lies outside [0, n) (the stateless kernels skip such
+    entries)."""
     nnz = idx.shape[0]
+    if int(idx.min()) < 0 or int(idx.max()) >= n:
+        return None
     counts = (ptr[1:] - ptr[:-1]).long()
     rows = torch.repeat_interleave(torch.arange(m, device=idx.device, dtype=idx.dtype), counts,
                                    output_size=nnz)
     cols, order = torch.sort(idx, stable=True)
     t_idx = rows[order]
     t_d = d[order] if d.shape[0] > 1 else d
-    t_ptr = torch.zeros(n + 1, dtype=idx.dtype, device=idx.device)
-    t_ptr[1:] = torch.cumsum(torch.bincount(cols.long(), minlength=n), 0).to(idx.dtype)
-    return t_d.contiguous(), t_idx.contiguous(), t_ptr
+    return t_d.contiguous(), t_idx.contiguous(), _sorted_offsets(cols, n, idx.dtype)
 
 
 def plan_cache_info():

@@ -324,6 +324,29 @@ def test_event_gather_transposed_plan_integer_exact_and_invalidated():
     assert bm.event.plan_cache_info()["builds"] == 2
 
 
+@pytest.mark.parametrize("bad", [3000, -1])
+def test_event_gather_index_outside_the_matrix_stays_stateless(bad):
+    # a column index outside [0, n): the stateless gather skips that entry;
+    # no transposed plan is built (it could not hold the entry), and every
+    # call equals the oracle's gather without it
+    idx, ptr = fixed_prob_csr(2000, 3000, 0.02, 35)
+    w = np.random.default_rng(36).integers(-4, 5, idx.size).astype(np.float32)
+    idx = idx.copy()
+    idx[17] = bad
+    keep = (idx >= 0) & (idx < 3000)
+    rows = np.repeat(np.arange(2000), np.diff(ptr))
+    D = np.zeros((2000, 3000))
+    np.add.at(D, (rows[keep], idx[keep]), w[keep])
+    ev = bernoulli(3000, 0.2, 37)
+    ref = D @ ev.astype(np.float64)
+    tw, ti, tp = (torch.from_numpy(a).to(DEV) for a in (w, idx, ptr))
+    for _ in range(3):
+        y = bm.event.csrmv(tw, ti, tp, torch.from_numpy(ev).to(DEV), shape=(2000, 3000))
+        np.testing.assert_array_equal(y.cpu().numpy(), ref)
+    info = bm.event.plan_cache_info()
+    assert info["plans"] == 0 and info["ineligible"] == 1
+
+
 @pytest.mark.parametrize("shape", [(2000, 3000), (3000, 700)])
 @pytest.mark.parametrize("homo", [False, True])
 def test_coomv_runs_on_the_plan(shape, homo):
