@@ -329,6 +329,14 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
   // (c = step * l - 1 + 32 at a stream's start): c >> 5 is then the index of
   // c's event word in s_bits, and the bit is still c % 32.
   constexpr uint32_t OFF = 32u;
+  // Draws per loop iteration: the stream switch, which some lane of a wave
+  // takes in nearly every iteration, is tested after the last of them only.
+  // With ~3 connections per stream, two slots cost a lane ~0.5 idle slot
+  // per stream and save the switch block every other draw (homogeneous
+  // 138 -> 128 us, uniform 193 -> 187 us on 1M x 1M at p = 1e-4); normal
+  // weights keep one (the idle slot's weight draw costs more than the
+  // switch).  Same draws, same sums, so the same bits.
+  constexpr int kSlots = DIST == BPX_DIST_NORMAL ? 1 : 2;
   const uint32_t nthreads = gridDim.x * kJitThreads;
   const uint32_t tid = blockIdx.x * kJitThreads + threadIdx.x;
   const uint32_t rq = (uint32_t)num_row / nthreads, rr = (uint32_t)num_row % nthreads;
@@ -347,6 +355,26 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
   float acc = 0.f, racc = 0.f;
   c = next_col<MK>(c, key.next(), dv, ds, negd);
   while (live) {
+#pragma unroll
+    for (int sl = 1; sl < kSlots; ++sl) {
+      // a draw with no switch after it: a lane whose stream ends on this
+      // test fails the next test too (c only grows within a stream) and
+      // switches there, the slot between drawing from the key it drops
+      const bool in = c < end;
+      const uint32_t word = s_bits[in ? c >> 5 : 0u];
+      const uint32_t cb = c;
+      float w = w0;
+      if constexpr (DIST != BPX_DIST_HOMO) {
+        if (FIN || in) w = draw_weight<DIST>(key, w0, w1);
+      }
+      c = next_col<MK>(c, key.next(), dv, ds, negd);
+      const uint32_t bit = __builtin_amdgcn_ubfe(word, cb, 1);
+      if constexpr (FIN) {
+        acc = __builtin_fmaf(w, (float)bit, acc);
+      } else if (in) {
+        acc += w * (float)bit;
+      }
+    }
     const bool in = c < end;
     // in flight during the draws; lanes past their stream's end read the zero word
     const uint32_t word = s_bits[in ? c >> 5 : 0u];
@@ -605,7 +633,10 @@ static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t c
   if (outdim) {
     const int64_t nwords = (num_col + 31) / 32;
     if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && bits_fit(nwords + 1) && ws &&
-        ws_bytes >= (size_t)nwords * 4 && num_col < 0xFFFFFFC0ll && num_row < 0x7FFFFFFFll) {
+        ws_bytes >= (size_t)nwords * 4 && num_col < 0xFFFFFFC0ll && num_row < 0x7FFFFFFFll &&
+        (uint64_t)num_col + 64u + 2u * (uint64_t)clen < (1ull << 32)) {
+      // (a lane's column counter, 32 above the column, passes its stream's
+      // end by at most two increments of up to clen: no u32 wrap)
       uint32_t* bits = (uint32_t*)ws;
       hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((num_col + 255) / 256)), dim3(256), 0, s, (const uint8_t*)vec,
                          num_col, bits);
