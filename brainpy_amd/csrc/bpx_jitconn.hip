@@ -299,7 +299,7 @@ __device__ __forceinline__ uint32_t next_col(uint32_t c, uint32_t x, const DivU3
   if constexpr (MK == MOD_SHORT24) {
     const uint32_t q = ds.div(x);
     int32_t r;
-    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(q), "v"(negd), "v"(x));
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(q), "s"(negd), "v"(x));
     return c + (uint32_t)r + 1u;
   } else {
     return c + mod_clen<MK>(x, dv, ds) + 1u;
@@ -389,18 +389,25 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_rows_bits(const uint32_t* _
       racc += acc;
       acc = 0.f;
       ++kseed;
-      c = cn;
-      cn += step;
+      // unclipped, a stream's end is the next stream's start (one register)
+      if constexpr (CLIP) {
+        c = cn;
+        cn += step;
+      } else {
+        c = end;
+        end += step;
+      }
       if (c == c0_last) {
         out[row] = racc;
         racc = 0.f;
         ++row;
         live = row < row_end;
         c = c_first;
-        cn = c_first + step;
+        if constexpr (CLIP) cn = c_first + step;
+        else end = c_first + step;
       }
       key = Lfsr88(kseed);
-      end = CLIP ? min(cn, ncol) : cn;
+      if constexpr (CLIP) end = min(cn, ncol);
     }
     c = next_col<MK>(c, key.next(), dv, ds, negd);
     // bit-field extract of one bit (the offset's low 5 bits select it)
