@@ -101,6 +101,10 @@ def bench_jitconn(L, _lib, torch, steps, warmup, cpu=True):
     sp = _lib.stream_ptr()
     wsb = int(L.bpx_jitconn_workspace(n, n))
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    # the scatter form's two passes take per-tile connection lists (the
+    # public call's workspace); with the smaller workspace it runs one pass
+    wsb2 = max(int(L.bpx_jitconn_workspace_ex(n, n, clen, d)) for d in (0, 1, 2))
+    ws2 = torch.empty(wsb2, dtype=torch.uint8, device=dev)
     res = []
     for od in (1, 0):
         # exact connection counts of this layout: M @ ones (all streams) and
@@ -115,11 +119,27 @@ def bench_jitconn(L, _lib, torch, steps, warmup, cpu=True):
         streams = n * 32 if od else n_active * 32
         conns = conns_all if od else events
         for dist, (w0, w1), extra in ((0, (1.0, 0.0), 0), (1, (0.0, 1.0), 1), (2, (0.5, 0.1), 2)):
-            def call():
+            def call(two_pass=True):
+                wp, wn = (ws2, wsb2) if (two_pass and not od) else (ws, wsb)
                 rc = L.bpx_jitconn_mv(sp, _lib.ptr(ev), _lib.BPX_BOOL, 1, dist, w0, w1, clen, 42, n, n, od, 0,
-                                      _lib.ptr(out), _lib.ptr(ws), wsb)
+                                      _lib.ptr(out), _lib.ptr(wp), wn)
                 if rc != 0:
                     raise RuntimeError(f"bpx_jitconn_mv failed ({rc})")
+
+            def device_time(two_pass=True):
+                # device seconds per call: the kernel (row-parallel), or the
+                # scatter form's list pass plus its sum pass
+                _lib.timer_reset()
+                _lib.timer_enable(True)
+                for _ in range(steps):
+                    call(two_pass)
+                torch.cuda.synchronize()
+                _lib.timer_enable(False)
+                ms, cnt = _lib.timer_read(_lib.KID["jit_outdim"] if od else _lib.KID["jit_scatter"])
+                emit = _lib.timer_read(_lib.KID["jit_emit"])[0] if not od else 0.0
+                split[two_pass] = (emit / 1e3 / max(cnt, 1), ms / 1e3 / max(cnt, 1))
+                return (ms + emit) / 1e3 / max(cnt, 1)
+            split = {}
             for _ in range(warmup):
                 call()
             torch.cuda.synchronize()
@@ -128,19 +148,16 @@ def bench_jitconn(L, _lib, torch, steps, warmup, cpu=True):
                 call()
             torch.cuda.synchronize()
             wall = (time.perf_counter() - t0) / steps
-            _lib.timer_reset()
-            _lib.timer_enable(True)
-            for _ in range(steps):
-                call()
-            torch.cuda.synchronize()
-            _lib.timer_enable(False)
-            ms, cnt = _lib.timer_read(_lib.KID["jit_outdim"] if od else _lib.KID["jit_scatter"])
-            kern = ms / 1e3 / max(cnt, 1)
+            kern = device_time()
+            one_pass = None if od else device_time(False)
             draws = streams + conns * (1 + extra)
             ops = draws * OPS_PER_DRAW
             r = {"dist": ["homo", "uniform", "normal"][dist], "outdim_parallel": bool(od), "us_per_call": wall * 1e6,
                  "kernel_us": kern * 1e6, "events_per_call": events, "events_per_s": events / wall,
                  "draws": draws, "int_tops": ops / kern / 1e12, "int_frac": ops / kern / 1e12 / INT_PEAK_TOPS}
+            if one_pass is not None:
+                r["one_pass_kernel_us"] = one_pass * 1e6
+                r["list_pass_us"], r["sum_pass_us"] = (x * 1e6 for x in split[True])
             if cpu:
                 # the oracle's serial restatement of the same lane32 kernel
                 # (the reference's Taichi loop order), one call, 1 thread
@@ -150,6 +167,7 @@ def bench_jitconn(L, _lib, torch, steps, warmup, cpu=True):
                 y_cpu = orc.jitconn_mv(ev_h, dist, 1, od, orc.LAYOUT_LANE32, n, n, np.float32(w0), np.float32(w1),
                                        clen, 42)
                 t_cpu = time.perf_counter() - t0
+                call()                           # the two-pass form's output for the scatter form
                 y_gpu = out.cpu().numpy()
                 r["cpu_port_s"] = t_cpu
                 r["cpu_port_events_per_s"] = events / t_cpu
@@ -158,6 +176,8 @@ def bench_jitconn(L, _lib, torch, steps, warmup, cpu=True):
             res.append(r)
             print(f"# jitconn od={od} {r['dist']:7s} {r['us_per_call']:9.1f} us/call kernel {r['kernel_us']:9.1f} us "
                   f"{r['int_tops']:6.2f} Tops ({100*r['int_frac']:.1f}%) {r['events_per_s']:.3e} ev/s"
+                  + (f" (list {r['list_pass_us']:.1f} + sum {r['sum_pass_us']:.1f} us; one pass "
+                     f"{r['one_pass_kernel_us']:.1f} us)" if one_pass is not None else "")
                   + (f" | cpu port {r['cpu_port_s']:.3f} s/call (1 thread), equal={r['gpu_equals_cpu']} "
                      f"max|diff|={r['gpu_vs_cpu_max_abs_diff']:.2e}" if cpu else ""),
                   file=sys.stderr, flush=True)

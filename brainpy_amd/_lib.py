@@ -55,6 +55,7 @@ SIGNATURES = {
     "bpx_jitconn_mv": ([_vp, _vp, _i32, _i32, _i32, _f32, _f32, ctypes.c_int32, _u32, _i64, _i64, _i32, _i32, _vp,
                         _vp, _sz], _i32),
     "bpx_jitconn_workspace": ([_i64, _i64], _sz),
+    "bpx_jitconn_workspace_ex": ([_i64, _i64, ctypes.c_int32, ctypes.c_int], _sz),
     "bpx_lif_coba_step": ([_vp, _i64, _i32, _i32, _vp, _vp, _vp, ctypes.POINTER(_vp), _vp, _vp, _vp, _f32, _vp,
                            _f32, _f32, _f32, _f32, _f32, _f32, _f32, _f32], _i32),
     "bpx_csr_on_pre_update": ([_vp, _vp, _i32, _vp, _vp, _i32, _i64, _vp, _i32, _i64, _vp, _i64,
@@ -217,4 +218,4 @@ def timer_read(kernel_id):
 
 
 KID = dict(event_scatter=0, event_gather=1, csr_scatter=2, csr_gather=3, jit_outdim=4, jit_scatter=5, info=6,
-           lif=7, fixed_order=8, tiled_list=9, net=10, tiled=11, coo=12, bsr=13, net_run=14, tiled_gather=15)
+           lif=7, fixed_order=8, tiled_list=9, net=10, tiled=11, coo=12, bsr=13, net_run=14, tiled_gather=15, jit_emit=16)

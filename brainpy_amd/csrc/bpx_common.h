@@ -96,7 +96,8 @@ enum KernelId {
   KID_BSR = 13,
   KID_NET_RUN = 14,
   KID_TILED_GATHER = 15,
-  KID_COUNT = 16
+  KID_JIT_EMIT = 16,
+  KID_COUNT = 17
 };
 
 }  // namespace bpx

@@ -551,6 +551,287 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_scatter_tiles(const uint32_
 }
 
 // ---------------------------------------------------------------------------
+// lane32 scatter (outdim_parallel=False), bool events, in two passes.
+//
+// k_jit_scatter_tiles regenerates every active stream up to the end of its
+// block's tile, so the last tile of each stream range walks whole streams
+// and bounds the call.  Here each active stream is walked once:
+//   pass 1, k_jit_emit, block (l, j): stream l of the active columns of
+//     column slice j, each to its end; every connection appends its
+//     tile-local row (and, for uniform and normal weights, its weight, drawn
+//     in the reference's order) to list (l, j, k) of its row tile k;
+//   pass 2, k_jit_sum_lists, block (l, k): adds the kEmitSlices lists of
+//     tile k into LDS (counts for homogeneous weights, FP32 sums otherwise)
+//     and stores the tile's rows once.
+// The lists have a fixed capacity and their counts are stored whole: a tile
+// with an overflowing list regenerates its streams instead (as
+// k_jit_scatter_tiles does), so every input is served.
+// ---------------------------------------------------------------------------
+constexpr int kEmitSlices = 8;          // column slices per stream range
+// w * scale rounded to the nearest integer (ties to even) as int64, for
+// |w * scale| < 2^51: one fma adding 1.5 * 2^52 leaves the integer in the
+// low mantissa bits
+__device__ __forceinline__ long long jit_fixed(float w, double scale) {
+  const double y = fma((double)w, scale, 6755399441055744.0);
+  return (long long)__double_as_longlong(y) - 0x4338000000000000ll;
+}
+constexpr int kEmitList = 16384;        // active columns held in LDS per round
+
+// f(c) for every column c of thread t's share of the slice [sb, se) with
+// ev[c] != 0: the 16-byte pieces sb + 16 t + 16384 v (v < nv), so a wave's
+// loads cover 1 KB of consecutive events; 8 pieces are loaded together.
+template <typename F>
+__device__ __forceinline__ void for_active(const uint8_t* __restrict__ ev, int64_t sb, int64_t se, int64_t nv,
+                                           F&& f) {
+  const bool al = (reinterpret_cast<uintptr_t>(ev + sb) & 15u) == 0;
+  for (int64_t v0 = 0; v0 < nv; v0 += 8) {
+    uint4 q[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t b = sb + (v0 + u) * (16 * kJitThreads) + 16 * (int64_t)threadIdx.x;
+      q[u] = make_uint4(0u, 0u, 0u, 0u);
+      if (al && v0 + u < nv && b + 16 <= se) q[u] = *reinterpret_cast<const uint4*>(ev + b);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t b = sb + (v0 + u) * (16 * kJitThreads) + 16 * (int64_t)threadIdx.x;
+      if (v0 + u >= nv || b >= se) continue;
+      if (al && b + 16 <= se) {
+        const uint32_t w[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          // the nonzero bytes of the word, lowest first
+          uint32_t x = w[h];
+          while (x != 0u) {
+            const uint32_t y = (uint32_t)__builtin_ctz(x) >> 3;
+            f(b + 4 * h + y);
+            x &= ~(0xFFu << (8 * y));
+          }
+        }
+      } else {
+        for (int64_t c = b; c < min(b + 16, se); ++c)
+          if (ev[c] != 0) f(c);
+      }
+    }
+  }
+}
+
+template <int DIST, int MK>
+__global__ __launch_bounds__(kJitThreads) void k_jit_emit(const uint8_t* __restrict__ ev, int64_t num_col, float w0,
+                                                          float w1, DivU32 dv, DivShort ds, uint32_t seed,
+                                                          int64_t num_row, uint32_t step, int K, int T, DivU32 dT,
+                                                          int64_t span, uint32_t cap, uint32_t* __restrict__ counts,
+                                                          uint16_t* __restrict__ lrow, float* __restrict__ lw) {
+  __shared__ uint32_t s_list[kEmitList];
+  __shared__ uint32_t s_w[kJitThreads / 64];
+  __shared__ uint32_t s_tot;
+  extern __shared__ uint32_t s_cnt[];     // K list counters
+  const uint32_t l = blockIdx.x / (uint32_t)kEmitSlices, j = blockIdx.x % (uint32_t)kEmitSlices;
+  const uint64_t s0 = (uint64_t)step * l;
+  const uint32_t send = (uint32_t)min((int64_t)(s0 + step - 1), num_row);   // rows below it
+  const uint32_t r0 = (uint32_t)s0 - 1u;                                    // step*l - 1 (u32 wrap at l = 0)
+  const int32_t negd = -(int32_t)ds.d;
+  for (int k = threadIdx.x; k < K; k += kJitThreads) s_cnt[k] = 0u;
+  // slice j: columns [sb, se); its active columns get positions by a block
+  // prefix over the threads' counts.  A thread keeps its first kHold active
+  // columns in s_list[t * kHold + i] while counting (the list is filled
+  // from those; a thread with more scans its share again).
+  constexpr int kHold = kEmitList / kJitThreads;
+  const int64_t sb = min((int64_t)j * span * kJitThreads, num_col);
+  const int64_t se = min(sb + span * kJitThreads, num_col);
+  const int64_t nv = (span * kJitThreads + 16 * kJitThreads - 1) / (16 * kJitThreads);
+  uint32_t mine = 0;
+  for_active(ev, sb, se, nv, [&](int64_t c) {
+    if (mine < (uint32_t)kHold) s_list[threadIdx.x * kHold + mine] = (uint32_t)c;
+    ++mine;
+  });
+  uint32_t incl = mine;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, 64);
+    if (lane_id() >= (uint32_t)o) incl += y;
+  }
+  const uint32_t wv = threadIdx.x >> 6;
+  if (lane_id() == 63) s_w[wv] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int q = 0; q < kJitThreads / 64; ++q) {
+      const uint32_t x = s_w[q];
+      s_w[q] = t;
+      t += x;
+    }
+    s_tot = t;
+  }
+  __syncthreads();
+  const uint32_t off = s_w[wv] + incl - mine;
+  const uint32_t total = s_tot;
+  uint32_t held[kHold];
+#pragma unroll
+  for (int i = 0; i < kHold; ++i) held[i] = (uint32_t)i < mine ? s_list[threadIdx.x * kHold + i] : 0u;
+  __syncthreads();
+  // this block's K lists (32-bit offsets within them: K * cap < 2^31)
+  uint16_t* const brow = lrow + (size_t)blockIdx.x * (size_t)K * cap;
+  float* const bw = DIST == BPX_DIST_HOMO ? nullptr : lw + (size_t)blockIdx.x * (size_t)K * cap;
+  for (uint32_t rb = 0; rb < total; rb += kEmitList) {
+    // the round's active columns (positions [rb, rb + kEmitList)) into LDS
+#pragma unroll
+    for (int i = 0; i < kHold; ++i) {
+      const uint32_t p = off + i;
+      if ((uint32_t)i < mine && p >= rb && p < rb + kEmitList) s_list[p - rb] = held[i];
+    }
+    if (mine > (uint32_t)kHold && off + mine > rb && off + kHold < rb + kEmitList) {
+      uint32_t p = off;
+      for_active(ev, sb, se, nv, [&](int64_t c) {
+        if (p >= off + kHold && p >= rb && p < rb + kEmitList) s_list[p - rb] = (uint32_t)c;
+        ++p;
+      });
+    }
+    __syncthreads();
+    const uint32_t n = min(total - rb, (uint32_t)kEmitList);
+    // each connection's list slot is reserved by an LDS atomic whose result
+    // is used one connection later, so its round trip overlaps the next draw
+    uint32_t pat = 0xFFFFFFFFu, ploc = 0u, ppos = 0u;
+    float pv = 0.f;
+    auto put = [&]() {
+      if (pat != 0xFFFFFFFFu && ppos < cap) {
+        const uint32_t at = pat + ppos;
+        brow[at] = (uint16_t)ploc;
+        if constexpr (DIST != BPX_DIST_HOMO) bw[at] = pv;
+      }
+    };
+    for (uint32_t e = threadIdx.x; e < n; e += kJitThreads) {
+      const uint32_t col = s_list[e];
+      Lfsr88 key(seed + col * 32u + l);
+      uint32_t r = next_col<MK>(r0, key.next(), dv, ds, negd);
+      while (r < send) {
+        float v = w0;
+        if constexpr (DIST != BPX_DIST_HOMO) v = draw_weight<DIST>(key, w0, w1);
+        const uint32_t orow = r - (uint32_t)s0;            // row within the stream range
+        const uint32_t k = dT.div(orow);
+        const uint32_t pos = atomicAdd(s_cnt + k, 1u);
+        put();
+        pat = k * cap;
+        ploc = orow - k * (uint32_t)T;
+        pv = v;
+        ppos = pos;
+        r = next_col<MK>(r, key.next(), dv, ds, negd);
+      }
+    }
+    put();
+    __syncthreads();
+  }
+  // every count stored, so the pass after needs no cleared memory
+  for (int k = threadIdx.x; k < K; k += kJitThreads) counts[(size_t)blockIdx.x * K + k] = s_cnt[k];
+}
+
+// FX (uniform and normal weights whose draws are all finite and bounded by
+// the host's bound): the weights are summed in 64-bit fixed point (scale
+// fxs, unit fxi; exact in any order, one rounding at the store) instead of
+// FP32 LDS atomics, which run ~20x slower than integer ones
+template <int DIST, int MK, bool FX>
+__global__ __launch_bounds__(kJitThreads) void k_jit_sum_lists(const uint8_t* __restrict__ ev, int64_t num_col,
+                                                               float w0, float w1, DivU32 dv, DivShort ds,
+                                                               uint32_t seed, int64_t num_row, uint32_t step, int K,
+                                                               int T, uint32_t cap,
+                                                               const uint32_t* __restrict__ counts,
+                                                               const uint16_t* __restrict__ lrow,
+                                                               const float* __restrict__ lw, double fxs, double fxi,
+                                                               float* __restrict__ out) {
+  extern __shared__ uint32_t s_acc[];     // homo: counts; FX: int64 sums; else f32 sums
+  unsigned long long* s_acc64 = reinterpret_cast<unsigned long long*>(s_acc);
+  auto add = [&](uint32_t i, float v) {
+    if constexpr (DIST == BPX_DIST_HOMO)
+      __hip_atomic_fetch_add(s_acc + i, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else if constexpr (FX)
+      __hip_atomic_fetch_add(s_acc64 + i, (unsigned long long)jit_fixed(v, fxs), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+      __hip_atomic_fetch_add(reinterpret_cast<float*>(s_acc) + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+  const uint32_t l = blockIdx.x / (uint32_t)K, k = blockIdx.x % (uint32_t)K;
+  const uint64_t s0 = (uint64_t)step * l;
+  const uint64_t lo64 = s0 + (uint64_t)k * (uint64_t)T;
+  uint64_t hi64 = lo64 + (uint64_t)T;
+  if (hi64 > s0 + step) hi64 = s0 + step;
+  if (hi64 > (uint64_t)num_row) hi64 = (uint64_t)num_row;
+  if (lo64 < hi64) {
+    const uint32_t lo = (uint32_t)lo64, hi = (uint32_t)hi64;
+    for (int i = threadIdx.x; i < (FX ? 2 * T : T); i += kJitThreads) s_acc[i] = 0u;
+    __syncthreads();
+    // the tile's list counts, loaded together
+    uint32_t nl[kEmitSlices];
+#pragma unroll
+    for (int jj = 0; jj < kEmitSlices; ++jj) nl[jj] = counts[((size_t)l * kEmitSlices + jj) * K + k];
+    bool over = false;
+    uint32_t tot = 0;
+#pragma unroll
+    for (int jj = 0; jj < kEmitSlices; ++jj) {
+      over = over || nl[jj] > cap;
+      tot += nl[jj];
+    }
+    if (!over) {
+      // entry g of the lists laid end to end: list jj holds
+      // [start_jj, start_jj + nl[jj]); a thread loads kU entries at a time
+      constexpr int kU = 16;
+      for (uint32_t g0 = 0; g0 < tot; g0 += kJitThreads * kU) {
+        uint32_t loc[kU];
+        float wv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const uint32_t g = g0 + (uint32_t)u * kJitThreads + threadIdx.x;
+          loc[u] = 0xFFFFFFFFu;
+          wv[u] = 0.f;
+          if (g < tot) {
+            uint32_t e = g;
+            int jj = 0;
+#pragma unroll
+            for (int q = 0; q < kEmitSlices - 1; ++q)
+              if (jj == q && e >= nl[q]) {
+                e -= nl[q];
+                jj = q + 1;
+              }
+            const size_t at = (((size_t)l * kEmitSlices + jj) * K + k) * cap + e;
+            loc[u] = lrow[at];
+            if constexpr (DIST != BPX_DIST_HOMO) wv[u] = lw[at];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+          if (loc[u] != 0xFFFFFFFFu) add(loc[u], wv[u]);
+      }
+    } else {
+      // a list overflowed: regenerate every active stream l up to hi
+      const uint32_t send = (uint32_t)min((int64_t)(s0 + step - 1), num_row);
+      const uint32_t hie = min(hi, send);
+      const uint32_t r0 = (uint32_t)s0 - 1u;
+      const int32_t negd = -(int32_t)ds.d;
+      for (int64_t c = threadIdx.x; c < num_col; c += kJitThreads) {
+        if (ev[c] == 0) continue;
+        Lfsr88 key(seed + (uint32_t)c * 32u + l);
+        uint32_t r = next_col<MK>(r0, key.next(), dv, ds, negd);
+        while (r < hie) {
+          float v = w0;
+          if constexpr (DIST != BPX_DIST_HOMO) v = draw_weight<DIST>(key, w0, w1);
+          if (r >= lo) add(r - lo, v);
+          r = next_col<MK>(r, key.next(), dv, ds, negd);
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; lo + i < hi; i += kJitThreads) {
+      if constexpr (DIST == BPX_DIST_HOMO) out[lo + i] = repeat_sum_f(s_acc[i], w0);
+      else if constexpr (FX) out[lo + i] = (float)((double)(long long)s_acc64[i] * fxi);
+      else out[lo + i] = __uint_as_float(s_acc[i]);
+    }
+  }
+  // rows past every stream's range (32 * step <= row < num_row) get no
+  // connection: the last block stores their zeros
+  if (blockIdx.x == gridDim.x - 1)
+    for (int64_t row = (int64_t)step * 32 + threadIdx.x; row < num_row; row += kJitThreads) out[row] = 0.f;
+}
+
+// ---------------------------------------------------------------------------
 // row layout
 // ---------------------------------------------------------------------------
 template <int DIST, int VM>
@@ -632,11 +913,54 @@ static bool bits_fit(int64_t nwords) {
 // workspace of the scatter form's event list: a count, then one u32 per column
 static size_t scatter_ws_bytes(int64_t num_col) { return 256 + (size_t)num_col * 4; }
 
+// Geometry and workspace layout of the two-pass scatter (k_jit_emit,
+// k_jit_sum_lists): K row tiles of T rows per stream range (as the one-pass
+// tiles), slices of 1024 * span columns, list counts at counts_off, the
+// lists' tile-local rows (u16) at rows_off and, for uniform and normal
+// weights, their weights (f32) at w_off.  A list holds `cap` connections:
+// those of 12.5 % active columns with a 25 % margin.  Matrices whose lists
+// would pass 1 GiB in all keep the one-pass tiles.
+struct EmitPlan {
+  int64_t K, T, span;
+  uint64_t lists;
+  uint32_t cap;
+  size_t counts_off, rows_off, w_off, bytes;
+};
+
+static bool emit_plan(int64_t num_row, int64_t num_col, int32_t clen, int dist, EmitPlan* ep) {
+  if (num_row <= 0 || num_col <= 0 || clen < 1 || num_row >= 0xFFFFFFFFll || num_col >= (1ll << 27)) return false;
+  const int64_t step = ((num_row + 1) >> 5) > 1 ? ((num_row + 1) >> 5) : 1;
+  int64_t K = (step + 32767) / 32768;
+  if (K < 8) K = 8;
+  if (K > step) K = step;
+  const int64_t T = (step + K - 1) / K;
+  const int64_t cw = (num_col + kEmitSlices - 1) / kEmitSlices;
+  int64_t span = (cw + kJitThreads - 1) / kJitThreads;     // columns per thread
+  if (span >= 16) span = (span + 15) / 16 * 16;               // whole 16-byte loads
+  // connections of one list with every column of its slice active: columns
+  // x T rows / the mean increment (clen + 1) / 2
+  const double full = (double)span * kJitThreads * (double)T * 2.0 / ((double)clen + 1.0);
+  const double capd = full * 0.125 * 1.25 + 64.0;
+  if (capd > 2.0e9) return false;
+  ep->K = K;
+  ep->T = T;
+  ep->span = span;
+  ep->lists = (uint64_t)32 * kEmitSlices * (uint64_t)K;
+  ep->cap = (uint32_t)capd;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  ep->counts_off = 256;
+  ep->rows_off = al(ep->counts_off + ep->lists * 4);
+  ep->w_off = al(ep->rows_off + ep->lists * ep->cap * 2);
+  ep->bytes = ep->w_off + (dist == BPX_DIST_HOMO ? 0 : ep->lists * ep->cap * 4);
+  return ep->bytes <= ((size_t)1 << 30);
+}
+
 template <int DIST, int VM>
 static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t clen, uint32_t seed, int64_t num_row,
                    int64_t num_col, int outdim, int layout, float* out, void* ws, size_t ws_bytes) {
   const DivU32 dv = DivU32::make((uint32_t)clen);
   const uint32_t step = (uint32_t)(((num_row + 1) >> 5) > 1 ? ((num_row + 1) >> 5) : 1);
+  EmitPlan ep{};
   if (outdim) {
     const int64_t nwords = (num_col + 31) / 32;
     if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && bits_fit(nwords + 1) && ws &&
@@ -704,6 +1028,52 @@ static int run_jit(hipStream_t s, const void* vec, float w0, float w1, int32_t c
       BPX_LAUNCH(KID_JIT_OUTDIM, (k_jit_row_outdim<DIST, VM>), dim3((unsigned)blocks), dim3(256), 0, s, vec, w0, w1,
                  dv, seed, num_row, num_col, out);
     }
+  } else if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && ws && emit_plan(num_row, num_col, clen, DIST, &ep) &&
+             ws_bytes >= ep.bytes) {
+    // two passes: each active stream generated once into per-tile lists
+    // (grid: 32 stream ranges x kEmitSlices column slices), then each tile
+    // summed from its lists (grid: 32 x K tiles)
+    uint32_t* counts = (uint32_t*)((char*)ws + ep.counts_off);
+    uint16_t* lrow = (uint16_t*)((char*)ws + ep.rows_off);
+    float* lw = DIST == BPX_DIST_HOMO ? nullptr : (float*)((char*)ws + ep.w_off);
+    const DivU32 dT = DivU32::make((uint32_t)ep.T);
+    DivShort ds = {1u, 0u, 0u};
+    const bool shrt = DivShort::make((uint32_t)clen, &ds) && DivShort::mad24_ok((uint32_t)clen);
+    // fixed-point sums for uniform and normal weights when every draw is
+    // finite and bounded by bnd (uniform |w0| + |w1 - w0|; normal |w0| + 8|w1|,
+    // Box-Muller with u1 >= 1e-10 bounding |z| by 6.8) and 64-bit sums of a
+    // tile fit the LDS: bnd * 2^shift * num_col < 2^62, terms below 2^51
+    double bnd = 0.0;
+    if (DIST == BPX_DIST_UNIFORM) bnd = fabs((double)w0) + fabs((double)w1 - (double)w0);
+    else if (DIST == BPX_DIST_NORMAL) bnd = fabs((double)w0) + 8.0 * fabs((double)w1);
+    const bool fx = DIST != BPX_DIST_HOMO && isfinite(w0) && isfinite(w1) && bnd < 3.0e38 && ep.T <= 16384;
+    int eb = 0;
+    if (bnd > 0.0) frexp(bnd, &eb);                       // bnd < 2^eb
+    int ec = 0;
+    frexp((double)num_col, &ec);                          // num_col < 2^ec
+    int sh = 61 - ec - eb;
+    if (sh > 50 - eb) sh = 50 - eb;
+    const double fxs = ldexp(1.0, sh), fxi = ldexp(1.0, -sh);
+#define JE(MKV, FXV)                                                                                          \
+  do {                                                                                                        \
+    BPX_LAUNCH(KID_JIT_EMIT, (k_jit_emit<DIST, MKV>), dim3((unsigned)(32 * kEmitSlices)), dim3(kJitThreads),  \
+               (size_t)ep.K * 4, s, (const uint8_t*)vec, num_col, w0, w1, dv, ds, seed, num_row, step, (int)ep.K, \
+               (int)ep.T, dT, ep.span, ep.cap, counts, lrow, lw);                                              \
+    BPX_LAUNCH(KID_JIT_SCATTER, (k_jit_sum_lists<DIST, MKV, FXV>), dim3((unsigned)(32 * ep.K)),               \
+               dim3(kJitThreads), (size_t)ep.T * (FXV ? 8 : 4), s, (const uint8_t*)vec, num_col, w0, w1, dv, ds, \
+               seed, num_row, step, (int)ep.K, (int)ep.T, ep.cap, (const uint32_t*)counts,                    \
+               (const uint16_t*)lrow, (const float*)lw, fxs, fxi, out);                                       \
+  } while (0)
+#define JE_F(MKV)                                \
+  do {                                           \
+    if (fx) JE(MKV, DIST != BPX_DIST_HOMO);      \
+    else JE(MKV, false);                         \
+  } while (0)
+    if (shrt) JE_F(MOD_SHORT24);
+    else if (dv.small_quotient()) JE_F(MOD_GM24);
+    else JE_F(MOD_GM);
+#undef JE_F
+#undef JE
   } else if (layout == BPX_LAYOUT_LANE32 && VM == VM_BOOL && ws && num_col > 0 && num_row > 0 &&
              ws_bytes >= scatter_ws_bytes(num_col) && num_row < 0xFFFFFFFFll && num_col < (1ll << 27)) {
     // exclusive row tiles: K tiles per stream range (at least 8, so the 32
@@ -767,6 +1137,13 @@ extern "C" size_t bpx_jitconn_workspace(int64_t num_row, int64_t num_col) {
   const size_t bits = (size_t)((num_col + 31) / 32) * 4;
   const size_t lst = scatter_ws_bytes(num_col);
   return bits > lst ? bits : lst;
+}
+
+extern "C" size_t bpx_jitconn_workspace_ex(int64_t num_row, int64_t num_col, int32_t clen, int dist) {
+  const size_t base = bpx_jitconn_workspace(num_row, num_col);
+  EmitPlan ep{};
+  if (dist < BPX_DIST_HOMO || dist > BPX_DIST_NORMAL || !emit_plan(num_row, num_col, clen, dist, &ep)) return base;
+  return ep.bytes > base ? ep.bytes : base;
 }
 
 extern "C" int bpx_jitconn_mv(void* stream, const void* vec, int vec_dtype, int event, int dist, float w0, float w1,

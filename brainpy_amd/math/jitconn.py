@@ -85,8 +85,14 @@ def _launch(v, event, dist, w0, w1, clen, seed, num_row, num_col, outdim_paralle
     out = torch.empty(num_row, dtype=torch.float32, device=v.device)
     L = _lib.lib()
     # bool events take the workspace in both forms: the event bitmask of the
-    # row-parallel kernel, the active-column list of the row-tiled scatter
-    wsb = int(L.bpx_jitconn_workspace(num_row, num_col)) if event else 0
+    # row-parallel kernel; for the scatter form (outdim_parallel=False) the
+    # per-tile connection lists of its two passes
+    if not event:
+        wsb = 0
+    elif outdim_parallel:
+        wsb = int(L.bpx_jitconn_workspace(num_row, num_col))
+    else:
+        wsb = int(L.bpx_jitconn_workspace_ex(num_row, num_col, clen, dist))
     ws = torch.empty(max(wsb, 4), dtype=torch.uint8, device=v.device) if wsb else None
     rc = L.bpx_jitconn_mv(_lib.stream_ptr(), _lib.ptr(v), _lib.bpx_dtype(v), int(event), dist,
                           np.float32(w0), np.float32(w1), clen, seed, num_row, num_col,

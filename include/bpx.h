@@ -293,10 +293,22 @@ int bpx_jitconn_mv_batch(void* stream, const void* events, int ev_dtype, int dis
  * and outdim_parallel=0 (lane32) it holds the list of active columns, and
  * each block owns a tile of output rows, regenerating every active column's
  * stream up to its tile and summing in LDS (no global atomics; homogeneous
- * weights exact).  NULL falls back to reading the event bytes (outdim) or
- * to one FP32 atomic per connection (scatter).
+ * weights exact).  With bpx_jitconn_workspace_ex bytes that form runs in two
+ * passes instead: every active stream is generated once and its connections
+ * listed by row tile, then each tile sums its lists in LDS (homogeneous
+ * weights as exact counts; uniform and normal weights in 64-bit fixed point
+ * when all draws are finite).  NULL falls back to reading the event bytes
+ * (outdim) or to one FP32 atomic per connection (scatter).
  */
 size_t bpx_jitconn_workspace(int64_t num_row, int64_t num_col);
+/* No counterpart in the reference: the workspace bytes with which
+ * bpx_jitconn_mv runs outdim_parallel=0 with bool events in two passes
+ * (each active stream generated once, its connections listed by row tile,
+ * then each tile summed in LDS); the lists are sized for about 12.5 %
+ * active columns, and tiles whose lists overflow regenerate their streams
+ * within the call.  Returns bpx_jitconn_workspace's size when the two-pass
+ * form does not apply (layouts other than lane32 take neither). */
+size_t bpx_jitconn_workspace_ex(int64_t num_row, int64_t num_col, int32_t clen, int dist);
 int bpx_jitconn_mv(void* stream, const void* vec, int vec_dtype, int event,
                    int dist, float w0, float w1, int32_t clen, uint32_t seed,
                    int64_t num_row, int64_t num_col, int outdim_parallel, int layout, float* out,
@@ -606,8 +618,11 @@ int bpx_xla_last_status(void);
  * synchronises those events and returns the summed device milliseconds and
  * launch count for one kernel id (0 event scatter, 1 event gather,
  * 2 csr scatter, 3 csr gather, 4 jitconn outdim, 5 jitconn scatter,
- * 6 event info, 7 lif step, 8 fixed order, 10 network step,
- * 11 tiled event scatter, 12 COO products, 13 BSR products).
+ * 6 event info, 7 lif step, 8 fixed order, 9 tiled active-row list,
+ * 10 network step, 11 tiled event scatter, 12 COO products, 13 BSR
+ * products, 14 multi-step network, 15 tiled gather, 16 jitconn scatter
+ * list pass; the two-pass jitconn scatter times its list pass under 16
+ * and its sum pass under 5).
  */
 int bpx_timer_enable(int on);
 int bpx_timer_read(int kernel_id, double* total_ms, int64_t* launches);

@@ -65,6 +65,23 @@ def test_host_only_entry_points():
     # bool fast path's event bits (512 B per tile and row)
     assert lib.bpx_event_info_workspace(1, 4096) == 8 + 512
     assert lib.bpx_event_info_workspace(2, 4097) == 32 + 2048
+    # jitconn: bitmask or active-column list; the two-pass scatter's lists
+    # (1M x 1M, clen 19999: 8 row tiles of 3907 rows per stream range, 2048
+    # lists of 8064 connections: 2 B each, +4 B with weights)
+    lib.bpx_jitconn_workspace.restype = ctypes.c_size_t
+    lib.bpx_jitconn_workspace.argtypes = [ctypes.c_int64, ctypes.c_int64]
+    lib.bpx_jitconn_workspace_ex.restype = ctypes.c_size_t
+    lib.bpx_jitconn_workspace_ex.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_int]
+    n = 1_000_000
+    assert lib.bpx_jitconn_workspace(n, n) == 256 + 4 * n
+    cap = int(131072 * 3907 * 2.0 / 20000 * 0.125 * 1.25 + 64.0)
+    rows_off = (256 + 2048 * 4 + 255) // 256 * 256
+    w_off = (rows_off + 2048 * cap * 2 + 255) // 256 * 256
+    assert lib.bpx_jitconn_workspace_ex(n, n, 19999, 0) == w_off
+    assert lib.bpx_jitconn_workspace_ex(n, n, 19999, 1) == w_off + 2048 * cap * 4
+    # no two-pass form: an unknown distribution, or lists beyond 1 GiB
+    assert lib.bpx_jitconn_workspace_ex(n, n, 19999, 7) == 256 + 4 * n
+    assert lib.bpx_jitconn_workspace_ex(n, n, 1, 0) == 256 + 4 * n
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libbpx.so not built")

@@ -183,6 +183,27 @@ def test_rows_kernel_nonfinite_and_overflowing_weights(dist, w0, w1):
     np.testing.assert_array_equal(y, ref)
 
 
+@pytest.mark.parametrize("dist", ["homo", "uniform"])
+@pytest.mark.parametrize("rate", [0.1, 0.6, 1.0])
+def test_event_scatter_form_list_overflow_vs_oracle(dist, rate):
+    # the scatter form lists each tile's connections (sized for ~12.5 %
+    # active columns); at 60 % and 100 % active columns the lists overflow
+    # and the tiles regenerate their streams instead: same results
+    shape = (6000, 6000)
+    ev = np.random.default_rng(int(rate * 100)).random(shape[1]) < rate
+    kw = dict(shape=shape, outdim_parallel=False)
+    if dist == "homo":
+        y = bm.jitconn.event_mv_prob_homo(ev, 0.75, 0.05, 23, **kw).cpu().numpy()
+        ref = orc.jitconn_mv(ev, "homo", 1, False, orc.LAYOUT_LANE32, shape[0], shape[1], np.float32(0.75),
+                             np.float32(0.0), np_ref.conn_len(0.05), 23)
+        np.testing.assert_array_equal(y, ref)
+    else:
+        y = bm.jitconn.event_mv_prob_uniform(ev, -0.5, 2.0, 0.05, 23, **kw).cpu().numpy()
+        ref = orc.jitconn_mv(ev, "uniform", 1, False, orc.LAYOUT_LANE32, shape[0], shape[1], np.float32(-0.5),
+                             np.float32(2.0), np_ref.conn_len(0.05), 23)
+        np.testing.assert_allclose(y, ref, rtol=1e-5, atol=2e-5)
+
+
 @pytest.mark.parametrize("dist,w0,w1", [("homo", np.inf, 0.0), ("homo", np.nan, 0.0), ("homo", -0.75, 0.0),
                                         ("uniform", -3e38, 3e38), ("uniform", 1e38, 3.3e38),
                                         ("uniform", -0.5, 2.0), ("normal", 0.5, 0.1)])
