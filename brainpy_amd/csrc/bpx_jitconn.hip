@@ -701,8 +701,7 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_emit(const uint8_t* __restr
       }
     };
     for (uint32_t e = threadIdx.x; e < n; e += kJitThreads) {
-      const uint32_t col = s_list[e];
-      Lfsr88 key(seed + col * 32u + l);
+      Lfsr88 key(seed + s_list[e] * 32u + l);
       uint32_t r = next_col<MK>(r0, key.next(), dv, ds, negd);
       while (r < send) {
         float v = w0;
@@ -764,41 +763,28 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_sum_lists(const uint8_t* __
 #pragma unroll
     for (int jj = 0; jj < kEmitSlices; ++jj) nl[jj] = counts[((size_t)l * kEmitSlices + jj) * K + k];
     bool over = false;
-    uint32_t tot = 0;
 #pragma unroll
-    for (int jj = 0; jj < kEmitSlices; ++jj) {
-      over = over || nl[jj] > cap;
-      tot += nl[jj];
-    }
+    for (int jj = 0; jj < kEmitSlices; ++jj) over = over || nl[jj] > cap;
     if (!over) {
-      // entry g of the lists laid end to end: list jj holds
-      // [start_jj, start_jj + nl[jj]); a thread loads kU entries at a time
-      constexpr int kU = 16;
-      for (uint32_t g0 = 0; g0 < tot; g0 += kJitThreads * kU) {
-        uint32_t loc[kU];
-        float wv[kU];
+      // one entry of each list per thread and round: the 8 loads of a
+      // round are in flight together
+      uint32_t nmax = 0;
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          const uint32_t g = g0 + (uint32_t)u * kJitThreads + threadIdx.x;
-          loc[u] = 0xFFFFFFFFu;
-          wv[u] = 0.f;
-          if (g < tot) {
-            uint32_t e = g;
-            int jj = 0;
+      for (int jj = 0; jj < kEmitSlices; ++jj) nmax = max(nmax, nl[jj]);
+      for (uint32_t e0 = 0; e0 < nmax; e0 += kJitThreads) {
+        const uint32_t e = e0 + threadIdx.x;
+        uint32_t loc[kEmitSlices];
+        float wv[kEmitSlices];
 #pragma unroll
-            for (int q = 0; q < kEmitSlices - 1; ++q)
-              if (jj == q && e >= nl[q]) {
-                e -= nl[q];
-                jj = q + 1;
-              }
-            const size_t at = (((size_t)l * kEmitSlices + jj) * K + k) * cap + e;
-            loc[u] = lrow[at];
-            if constexpr (DIST != BPX_DIST_HOMO) wv[u] = lw[at];
-          }
+        for (int jj = 0; jj < kEmitSlices; ++jj) {
+          const size_t at = (((size_t)l * kEmitSlices + jj) * K + k) * cap + e;
+          loc[jj] = e < nl[jj] ? (uint32_t)lrow[at] : 0xFFFFFFFFu;
+          wv[jj] = 0.f;
+          if constexpr (DIST != BPX_DIST_HOMO) wv[jj] = e < nl[jj] ? lw[at] : 0.f;
         }
 #pragma unroll
-        for (int u = 0; u < kU; ++u)
-          if (loc[u] != 0xFFFFFFFFu) add(loc[u], wv[u]);
+        for (int jj = 0; jj < kEmitSlices; ++jj)
+          if (loc[jj] != 0xFFFFFFFFu) add(loc[jj], wv[jj]);
       }
     } else {
       // a list overflowed: regenerate every active stream l up to hi
