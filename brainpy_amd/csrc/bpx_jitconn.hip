@@ -737,7 +737,10 @@ __global__ __launch_bounds__(kJitThreads) void k_jit_sum_lists(const uint8_t* __
                                                                const uint16_t* __restrict__ lrow,
                                                                const float* __restrict__ lw, double fxs, double fxi,
                                                                float* __restrict__ out) {
-  extern __shared__ uint32_t s_acc[];     // homo: counts; FX: int64 sums; else f32 sums
+  // homo: counts; FX: int64 sums (the array is 16-byte aligned, so the
+  // 64-bit LDS atomics stay aligned whatever static LDS precedes it); else
+  // f32 sums
+  extern __shared__ __align__(16) uint32_t s_acc[];
   unsigned long long* s_acc64 = reinterpret_cast<unsigned long long*>(s_acc);
   auto add = [&](uint32_t i, float v) {
     if constexpr (DIST == BPX_DIST_HOMO)
